@@ -1,0 +1,124 @@
+/*
+ * necg.h -- C ABI of the MI355X FP64 energy-and-gradient engine for
+ * non-Born-Oppenheimer explicitly correlated Gaussians.
+ *
+ * Every entry point is plain C: integer sizes and raw pointers.  Array
+ * arguments marked "device" are HIP device pointers (e.g. torch CUDA tensors'
+ * data_ptr()); "stream" is a hipStream_t passed as void* (NULL = the null
+ * stream).  Calls are asynchronous on that stream unless stated otherwise, so
+ * they can be captured in a hipGraph.  Errors that can only be found on the
+ * device (a pair matrix that is not positive definite, C^T S C <= 0, NaN/Inf)
+ * are latched in the context and returned by necg_status().
+ *
+ * Parameter vector x (length K*(blk+1), blk = n(n+1)/2 + 3n): K blocks of
+ * [vech(L_k) column-major lower incl. diagonal | s_k particle-major xyz]
+ * followed by the K linear coefficients C -- the layout of the reference's
+ * outprod(x) (python/outprod.py:40-41, python/matel.py:56-87).
+ */
+#ifndef NECG_H
+#define NECG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct necg_ctx necg_ctx;
+
+enum necg_status_code {
+  NECG_OK = 0,
+  NECG_ERR_ARG = 1,         /* bad pointer, size or dimension                  */
+  NECG_ERR_UNSUPPORTED = 2, /* n outside 1..6, or a system the kernels refuse  */
+  NECG_ERR_DEVICE = 3,      /* HIP runtime error                               */
+  NECG_ERR_OOM = 4,         /* device allocation failed                        */
+  NECG_ERR_NOT_PD = 5,      /* A_k + A_l' not positive definite / L_ii == 0    */
+  NECG_ERR_CSC = 6,         /* C^T S C <= 0 (reference: ZeroDivisionError)     */
+  NECG_ERR_NONFINITE = 7    /* NaN or Inf in x or in a result                  */
+};
+
+/* The physics block of an input file: the module globals filled by
+ * python/read_input.py:16-157 (python/shared.py:15-66). */
+typedef struct {
+  int32_t n;          /* pseudo-particles (reference: ne), 1..6            */
+  int32_t nst;        /* symmetry terms                                     */
+  double efs;         /* electric-field strength                            */
+  const double *mass; /* n*n row-major inverse-mass tensor                  */
+  const double *ncv;  /* n+1 charges, ncv[0] = reference particle           */
+  const double *scvv; /* nst weights ALREADY divided by nst
+                         (python/read_input.py:141)                         */
+  const double *sml;  /* nst*n*n, sml[p*n*n + i*n + j] = reference sml(i,j,p) */
+  const double *sms;  /* nst*(3n)*(3n), same convention                     */
+} necg_system;
+
+/* Create a context bound to HIP device `device` for one system.  Replaces the
+ * module state that read_input fills (python/read_input.py:16). Host pointers. */
+int necg_create(const necg_system *sys, int device, necg_ctx **out);
+
+/* Release the context and all device memory it holds. */
+int necg_destroy(necg_ctx *ctx);
+
+/* Energy and gradient: *f = C^T H C / C^T S C and g = dE/dx (length of x).
+ * Replaces outprod(x) -> (f, g) (python/outprod.py:20-76, with make_grad,
+ * python/make_grad.py:18-90) and the Fortran SFUN outprod(n, x, f, g)
+ * (source/outprod.f90:2).  x, f, g: device. Async. */
+int necg_outprod(necg_ctx *ctx, int32_t K, const double *x, double *f, double *g, void *stream);
+
+/* Forward half of outprod: H, S (K*K row-major, symmetric) and *f = E.
+ * Replaces the matel loop + quotient (python/outprod.py:49-68).  H and S may be
+ * NULL (kept inside the context).  x, f, H, S: device.  Async. */
+int necg_energy(necg_ctx *ctx, int32_t K, const double *x, double *f, double *H, double *S,
+                void *stream);
+
+/* Backward half: gx = gbar * dE/dx for the x of the last necg_energy call on
+ * this context (the make_grad contraction, python/make_grad.py:37-90, as an
+ * adjoint: H-bar = C C^T / C^T S C, S-bar = -E C C^T / C^T S C).  gbar is a
+ * device scalar (NULL = 1).  x, gx: device.  Async. */
+int necg_energy_vjp(necg_ctx *ctx, int32_t K, const double *x, const double *gbar, double *gx,
+                    void *stream);
+
+/* H and S only (python/necg.py:36-58 _build_matrices; python/outprod.py:49-60).
+ * iv has K*blk entries (no C).  Device.  Async. */
+int necg_matrices(necg_ctx *ctx, int32_t K, const double *iv, double *H, double *S, void *stream);
+
+/* Per-term elements: for query t, (stf, i, j) = q[3t..3t+2] (0-based term,
+ * bra, ket) gives out[4t..4t+3] = (ov, tk, ert, zht).  Replaces
+ * matel(stf, ii, jj) (python/matel.py:28, 1-based there).  Device.  Async. */
+int necg_matel(necg_ctx *ctx, int32_t K, const double *iv, int32_t nq, const int32_t *q,
+               double *out, void *stream);
+
+/* Per-term bra gradients: tgvl[blk*t ..], tgvls[blk*t ..] for query t.
+ * Replaces gradel(stf, ii, jj) (python/gradel.py:27).  Device.  Async. */
+int necg_gradel(necg_ctx *ctx, int32_t K, const double *iv, int32_t nq, const int32_t *q,
+                double *tgvl, double *tgvls, void *stream);
+
+/* The derivative matrices of make_grad, materialised:
+ * DH[(l*blk + r)*K + k] = sum_stf scvv * gradel(stf, l, k)[r], DS likewise
+ * (python/make_grad.py:34-55).  Device, K*blk*K each.  Async. */
+int necg_dhds(necg_ctx *ctx, int32_t K, const double *iv, double *DH, double *DS, void *stream);
+
+/* Synchronise `stream` and return the first latched error (NECG_OK if none);
+ * clears the latch.  Blocking. */
+int necg_status(necg_ctx *ctx, void *stream);
+
+/* Launch shape of the pair kernels: kets per work item, term groups, waves per
+ * workgroup; 0 keeps the built-in choice.  Used by the autotuner. */
+int necg_set_tiling(necg_ctx *ctx, int32_t ket_tile, int32_t term_split, int32_t waves);
+
+/* Launch shape that the last pair-kernel launch used (for reports). */
+int necg_get_tiling(necg_ctx *ctx, int32_t *ket_tile, int32_t *term_split, int32_t *waves);
+
+/* Pair-kernel timing with HIP events recorded on the launch stream around
+ * every pair-kernel launch.  mode 1: reset and start recording; mode 0: stop;
+ * mode 2: read.  *ms = summed milliseconds, *launches = launch count since the
+ * last reset (read synchronises the events).  Not for use under graph capture. */
+int necg_timing(necg_ctx *ctx, int32_t mode, double *ms, int64_t *launches);
+
+/* Human-readable text of a status code. */
+const char *necg_strerror(int code);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NECG_H */

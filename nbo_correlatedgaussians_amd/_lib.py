@@ -1,0 +1,101 @@
+"""ctypes binding of ``libnecg_hip.so`` (the C ABI declared in include/necg.h).
+
+The product path has no CPU fallback: if the library or a GPU is missing,
+loading raises."""
+
+from __future__ import annotations
+
+import ctypes as ct
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libnecg_hip.so")
+
+# status codes (include/necg.h)
+NECG_OK = 0
+NECG_ERR_ARG = 1
+NECG_ERR_UNSUPPORTED = 2
+NECG_ERR_DEVICE = 3
+NECG_ERR_OOM = 4
+NECG_ERR_NOT_PD = 5
+NECG_ERR_CSC = 6
+NECG_ERR_NONFINITE = 7
+
+EXPORTS = ("necg_create", "necg_destroy", "necg_outprod", "necg_energy",
+           "necg_energy_vjp", "necg_matrices", "necg_matel", "necg_gradel",
+           "necg_dhds", "necg_status", "necg_set_tiling", "necg_get_tiling",
+           "necg_timing", "necg_strerror")
+
+
+class NecgError(RuntimeError):
+    def __init__(self, code: int, where: str = ""):
+        self.code = code
+        msg = _LIB.necg_strerror(code).decode() if _LIB is not None else f"status {code}"
+        super().__init__(f"{where}: {msg} (status {code})" if where else msg)
+
+
+class NotPositiveDefinite(NecgError, ArithmeticError):
+    pass
+
+
+class ZeroNorm(NecgError, ZeroDivisionError):
+    pass
+
+
+class NonFinite(NecgError, FloatingPointError):
+    pass
+
+
+def raise_for(code: int, where: str = "") -> None:
+    if code == NECG_OK:
+        return
+    cls = {NECG_ERR_NOT_PD: NotPositiveDefinite, NECG_ERR_CSC: ZeroNorm,
+           NECG_ERR_NONFINITE: NonFinite}.get(code, NecgError)
+    raise cls(code, where)
+
+
+class NecgSystem(ct.Structure):
+    _fields_ = [("n", ct.c_int32), ("nst", ct.c_int32), ("efs", ct.c_double),
+                ("mass", ct.c_void_p), ("ncv", ct.c_void_p), ("scvv", ct.c_void_p),
+                ("sml", ct.c_void_p), ("sms", ct.c_void_p)]
+
+
+_LIB = None
+
+
+def library_path() -> str:
+    return LIB_PATH
+
+
+def load() -> ct.CDLL:
+    """Load the HIP library; raises if it has not been built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: run `python -m nbo_correlatedgaussians_amd.build`")
+    lib = ct.CDLL(LIB_PATH, mode=ct.RTLD_GLOBAL)
+    P, I, D = ct.c_void_p, ct.c_int32, ct.c_double
+    sig = {
+        "necg_create": [ct.POINTER(NecgSystem), ct.c_int, ct.POINTER(P)],
+        "necg_destroy": [P],
+        "necg_outprod": [P, I, P, P, P, P],
+        "necg_energy": [P, I, P, P, P, P, P],
+        "necg_energy_vjp": [P, I, P, P, P, P],
+        "necg_matrices": [P, I, P, P, P, P],
+        "necg_matel": [P, I, P, I, P, P, P],
+        "necg_gradel": [P, I, P, I, P, P, P, P],
+        "necg_dhds": [P, I, P, P, P, P],
+        "necg_status": [P, P],
+        "necg_set_tiling": [P, I, I, I],
+        "necg_get_tiling": [P, ct.POINTER(I), ct.POINTER(I), ct.POINTER(I)],
+        "necg_timing": [P, I, ct.POINTER(D), ct.POINTER(ct.c_int64)],
+    }
+    for name, args in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = ct.c_int
+    lib.necg_strerror.argtypes = [ct.c_int]
+    lib.necg_strerror.restype = ct.c_char_p
+    _LIB = lib
+    return lib

@@ -1,0 +1,1053 @@
+// necg.hip -- HIP kernels (gfx950) and the C ABI of include/necg.h.
+//
+// Evaluation pipeline for one parameter vector x on one stream:
+//   prep_bra   one lane per basis function: L, A = L L^T, S, A S, i1, prefactor
+//   prep_ket   one lane per (function, term): the ket under term P
+//              (A' = T^T A T, S' = sms s, A'S', A'MA', A'M, i2, weights)
+//   pair       the hot kernel: lane = bra k, loop over a tile of kets x terms,
+//              ket records read wave-uniformly; H/S elements of the lower
+//              triangle and the per-bra gradient accumulators (deterministic
+//              partial sums per tile, no atomics)
+//   hs_finish  sum term-group partials, mirror to full symmetric H and S
+//   quotient   HC, SC, C^T H C, C^T S C -> E, 1/C^T S C (one workgroup)
+//   grad_final sum tile partials, vech(X L) / A Y transforms, contraction
+//              with C -> g (python/make_grad.py:61-90)
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <vector>
+
+#include "../../include/necg.h"
+#include "necg_pair.hpp"
+
+using namespace necg;
+
+namespace {
+
+enum FlagBits { F_NOT_PD = 1, F_NONFINITE = 2, F_CSC = 4 };
+
+// ------------------------------------------------------------------ prep ----
+
+// One lane per basis function k: the bra record (python/matel.py:56-70, :125-136).
+template <int N>
+__global__ void prep_bra(const double *__restrict__ x, int K, int with_c,
+                         double *__restrict__ bra, int *__restrict__ flag) {
+  using D = Dim<N>;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= K) return;
+  const double *p = x + (size_t)k * D::BLK;
+  double L[N][N];
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = 0; j < N; ++j) L[i][j] = 0.0;
+  int t = 0;
+  bool bad = false, pd = true;
+#pragma unroll
+  for (int j = 0; j < N; ++j)
+#pragma unroll
+    for (int i = j; i < N; ++i, ++t) {
+      L[i][j] = p[t];
+      bad = bad || !isfinite(p[t]);
+    }
+  double *r = bra + (size_t)k * D::BS;
+  double detL = 1.0;
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    detL *= L[i][i];
+    pd = pd && (L[i][i] != 0.0);
+    r[D::B_LD + i] = 1.0 / L[i][i];
+  }
+#pragma unroll
+  for (int v = 0; v < D::NS; ++v) r[D::B_L + v] = p[v];
+  double A[D::NS];
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = 0; j <= i; ++j) {
+      double s = 0.0;
+#pragma unroll
+      for (int m = 0; m <= j; ++m) s = __builtin_fma(L[i][m], L[j][m], s);
+      A[sx(i, j)] = s;
+      r[D::B_A + sx(i, j)] = s;
+    }
+  double S[D::N3];
+#pragma unroll
+  for (int v = 0; v < D::N3; ++v) {
+    S[v] = p[D::NS + v];
+    bad = bad || !isfinite(S[v]);
+    r[D::B_S + v] = S[v];
+  }
+  double i1 = 0.0;
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      double s = 0.0;
+#pragma unroll
+      for (int j = 0; j < N; ++j) s = __builtin_fma(A[sx(i, j)], S[3 * j + c], s);
+      r[D::B_AS + 3 * i + c] = s;
+      i1 = __builtin_fma(S[3 * i + c], s, i1);
+    }
+  r[D::B_I1] = i1;
+  // pk = 8^(n/4) |det L|^(3/2): the bra half of sqrt(8^n) sqrt((|det L_k| |det L_l| / det B)^3)
+  const double ad = fabs(detL);
+  r[D::B_PK] = pow(8.0, 0.25 * N) * ad * sqrt(ad);
+  double c = 0.0;
+  if (with_c) {
+    c = x[(size_t)K * D::BLK + k];
+    bad = bad || !isfinite(c);
+  }
+  r[D::B_C] = c;
+  if (bad) atomicOr(flag, (int)F_NONFINITE);
+  if (!pd) atomicOr(flag, (int)F_NOT_PD);
+}
+
+// One lane per (ket l, term p): the transformed ket record.
+// A' = sml^T A_l sml, S' = sms s_l (python/matel.py:76-87).
+template <int N>
+__global__ void prep_ket(const double *__restrict__ bra, int K, int nst,
+                         const double *__restrict__ sml, const double *__restrict__ sms,
+                         const double *__restrict__ scvv, SysConst sc, double *__restrict__ ket) {
+  using D = Dim<N>;
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= K * nst) return;
+  const int l = idx / nst, p = idx - l * nst;
+  const double *b = bra + (size_t)l * D::BS;
+  const double *T = sml + (size_t)p * N * N;
+  const double *Sm = sms + (size_t)p * D::N3 * D::N3;
+  double Al[N][N], AT[N][N], Ap[N][N];
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = 0; j < N; ++j) Al[i][j] = b[D::B_A + sx(i, j)];
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      double s = 0.0;
+#pragma unroll
+      for (int m = 0; m < N; ++m) s = __builtin_fma(Al[i][m], T[m * N + j], s);
+      AT[i][j] = s;
+    }
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      double s = 0.0;
+#pragma unroll
+      for (int m = 0; m < N; ++m) s = __builtin_fma(T[m * N + i], AT[m][j], s);
+      Ap[i][j] = s;
+    }
+  double *r = ket + (size_t)idx * D::KS;
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = 0; j <= i; ++j) r[D::K_AP + sx(i, j)] = 0.5 * (Ap[i][j] + Ap[j][i]);
+  double Sp[D::N3];
+#pragma unroll
+  for (int u = 0; u < D::N3; ++u) {
+    double s = 0.0;
+#pragma unroll
+    for (int v = 0; v < D::N3; ++v) s = __builtin_fma(Sm[u * D::N3 + v], b[D::B_S + v], s);
+    Sp[u] = s;
+    r[D::K_SP + u] = s;
+  }
+  double i2 = 0.0;
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      double s = 0.0;
+#pragma unroll
+      for (int j = 0; j < N; ++j) s = __builtin_fma(Ap[i][j], Sp[3 * j + c], s);
+      r[D::K_APSP + 3 * i + c] = s;
+      i2 = __builtin_fma(Sp[3 * i + c], s, i2);
+    }
+  double ApM[N][N];
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      double s = 0.0;
+#pragma unroll
+      for (int m = 0; m < N; ++m) s = __builtin_fma(Ap[i][m], sc.mass[m * N + j], s);
+      ApM[i][j] = s;
+      r[D::K_APM + i * N + j] = s;
+    }
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = 0; j <= i; ++j) {
+      double s = 0.0;
+#pragma unroll
+      for (int m = 0; m < N; ++m) s = __builtin_fma(ApM[i][m], Ap[m][j], s);
+      double s2 = 0.0;
+#pragma unroll
+      for (int m = 0; m < N; ++m) s2 = __builtin_fma(ApM[j][m], Ap[m][i], s2);
+      r[D::K_NP + sx(i, j)] = 0.5 * (s + s2);
+    }
+  r[D::K_I2] = i2;
+  r[D::K_PL] = b[D::B_PK];
+  r[D::K_WC] = scvv[p] * b[D::B_C];
+  r[D::K_SC] = scvv[p];
+}
+
+// ------------------------------------------------------------------ pair ----
+
+// Lane = bra k.  blockIdx.y = ket tile, blockIdx.z = term group.
+// part layout: part[((tile * NACC) + v) * K + k], tile = y * TS + z.
+// hpart/spart layout: [z][l][k] for bra k >= ket l.
+
+template <int N, int MODE>
+__global__ void __launch_bounds__(256) pair_kernel(
+    const double *__restrict__ bra, const double *__restrict__ ket, int K, int nst, int kt,
+    int tps, SysConst sc, const double *__restrict__ scal, double *__restrict__ part,
+    double *__restrict__ hpart, double *__restrict__ spart, int *__restrict__ flag) {
+  using D = Dim<N>;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int l0 = blockIdx.y * kt;
+  const int l1 = min(K, l0 + kt);
+  const int p0 = blockIdx.z * tps;
+  const int p1 = min(nst, p0 + tps);
+  const bool active = k < K;
+  const int kk = active ? k : K - 1;
+  Bra<N> b;
+  load_bra<N>(b, bra + (size_t)kk * D::BS);
+  const double eshift = (MODE == MODE_COMBINED) ? scal[0] : 0.0;
+  Acc<N> acc;
+  acc.zero();
+  bool ok = true;
+  for (int l = l0; l < l1; ++l) {
+    double hel = 0.0, sel = 0.0;
+    const double *kl = ket + (size_t)l * nst * D::KS;
+    for (int p = p0; p < p1; ++p)
+      ok &= pair_term<N, MODE>(b, kl + (size_t)p * D::KS, sc, eshift, hel, sel, acc, nullptr);
+    if (MODE != MODE_COMBINED && active && k >= l) {
+      const size_t o = ((size_t)blockIdx.z * K + l) * K + k;
+      hpart[o] = hel;
+      spart[o] = sel;
+    }
+  }
+  if (!active) return;
+  if (!ok) atomicOr(flag, (int)F_NOT_PD);
+  if (MODE == MODE_ELEMENTS) return;
+  const size_t tile = (size_t)blockIdx.y * gridDim.z + blockIdx.z;
+  double *o = part + tile * D::NACC * K + k;
+#pragma unroll
+  for (int t = 0; t < D::NS; ++t) o[(size_t)(D::A_XH + t) * K] = acc.XH[t];
+#pragma unroll
+  for (int t = 0; t < D::N3; ++t) o[(size_t)(D::A_YH + t) * K] = acc.YH[t];
+  o[(size_t)D::A_SH * K] = acc.sH;
+  if (MODE == MODE_SPLIT) {
+#pragma unroll
+    for (int t = 0; t < D::NS; ++t) o[(size_t)(D::A_XS + t) * K] = acc.XS[t];
+#pragma unroll
+    for (int t = 0; t < D::N3; ++t) o[(size_t)(D::A_YS + t) * K] = acc.YS[t];
+    o[(size_t)D::A_SS * K] = acc.sS;
+  }
+}
+
+// Sum term groups, mirror the lower triangle (bra i >= ket j) into full
+// symmetric H and S (python/outprod.py:57-60).  One lane per (i, j).
+__global__ void hs_finish(const double *__restrict__ hpart, const double *__restrict__ spart,
+                          int K, int ts, double *__restrict__ H, double *__restrict__ S) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y;
+  if (j >= K) return;
+  const int lo = min(i, j), hi = max(i, j);
+  double h = 0.0, s = 0.0;
+  for (int z = 0; z < ts; ++z) {
+    const size_t o = ((size_t)z * K + lo) * K + hi;
+    h += hpart[o];
+    s += spart[o];
+  }
+  H[(size_t)i * K + j] = h;
+  S[(size_t)i * K + j] = s;
+}
+
+// HC = H C, SC = S C (lane per column, coalesced over the symmetric H),
+// then C^T H C, C^T S C, E and 1/C^T S C in one workgroup.
+__global__ void hc_sc(const double *__restrict__ H, const double *__restrict__ S,
+                      const double *__restrict__ bra, int bs, int bc, int K,
+                      double *__restrict__ HC, double *__restrict__ SC) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= K) return;
+  double h = 0.0, s = 0.0;
+  for (int j = 0; j < K; ++j) {
+    const double c = bra[(size_t)j * bs + bc];
+    h = __builtin_fma(H[(size_t)j * K + i], c, h);
+    s = __builtin_fma(S[(size_t)j * K + i], c, s);
+  }
+  HC[i] = h;
+  SC[i] = s;
+}
+
+__global__ void __launch_bounds__(1024) quotient(const double *__restrict__ HC,
+                                                 const double *__restrict__ SC,
+                                                 const double *__restrict__ bra, int bs, int bc,
+                                                 int K, double *__restrict__ scal,
+                                                 double *__restrict__ fout,
+                                                 int *__restrict__ flag) {
+  __shared__ double rh[1024], rs[1024];
+  double h = 0.0, s = 0.0;
+  for (int i = threadIdx.x; i < K; i += blockDim.x) {
+    const double c = bra[(size_t)i * bs + bc];
+    h = __builtin_fma(c, HC[i], h);
+    s = __builtin_fma(c, SC[i], s);
+  }
+  rh[threadIdx.x] = h;
+  rs[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      rh[threadIdx.x] += rh[threadIdx.x + w];
+      rs[threadIdx.x] += rs[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double cHc = rh[0], cSc = rs[0];
+    const double E = cHc / cSc;
+    scal[0] = E;
+    scal[1] = 1.0 / cSc;
+    scal[2] = cHc;
+    scal[3] = cSc;
+    if (fout) fout[0] = E;
+    if (!(cSc > 0.0)) atomicOr(flag, (int)F_CSC);
+    if (!isfinite(E)) atomicOr(flag, (int)F_NONFINITE);
+  }
+}
+
+// Lane per bra k: sum the tile partials, transform to the blk-vector, and
+// contract: g_theta = norm (a - E b) (MODE 0) or norm a (MODE 1, E folded),
+// with a = 2 C_k [vech(X_H L) + 1.5 sigma_H diag(1/L) | A Y_H]
+// (python/make_grad.py:61-79); g_C = 2 norm (HC - E SC) (:85).
+// gbar (nullable) scales the whole gradient (vector-Jacobian product).
+template <int N, int MODE>
+__global__ void grad_final(const double *__restrict__ part, int ntiles, int K,
+                           const double *__restrict__ bra, const double *__restrict__ scal,
+                           const double *__restrict__ HC, const double *__restrict__ SC,
+                           const double *__restrict__ gbar, double *__restrict__ g) {
+  using D = Dim<N>;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= K) return;
+  double a[D::NACC];
+#pragma unroll
+  for (int v = 0; v < D::NACC; ++v) a[v] = 0.0;
+  const int nv = (MODE == MODE_SPLIT) ? D::NACC : D::A_XS;
+  for (int t = 0; t < ntiles; ++t) {
+    const double *p = part + (size_t)t * D::NACC * K + k;
+#pragma unroll
+    for (int v = 0; v < D::NACC; ++v)
+      if (v < nv || v == D::A_SH) a[v] += p[(size_t)v * K];
+  }
+  const double *r = bra + (size_t)k * D::BS;
+  const double E = scal[0], nrm = scal[1];
+  const double gb = gbar ? gbar[0] : 1.0;
+  const double ck = r[D::B_C];
+  const double fac = gb * nrm * 2.0 * ck;
+  double oh[D::BLK];
+  grad_L<N>(a + D::A_XH, a[D::A_SH], r + D::B_L, r + D::B_LD, oh);
+  grad_S<N>(r + D::B_A, a + D::A_YH, oh + D::NS);
+  double *gk = g + (size_t)k * D::BLK;
+  if (MODE == MODE_SPLIT) {
+    double os[D::BLK];
+    grad_L<N>(a + D::A_XS, a[D::A_SS], r + D::B_L, r + D::B_LD, os);
+    grad_S<N>(r + D::B_A, a + D::A_YS, os + D::NS);
+#pragma unroll
+    for (int v = 0; v < D::BLK; ++v) gk[v] = fac * (oh[v] - E * os[v]);
+  } else {
+#pragma unroll
+    for (int v = 0; v < D::BLK; ++v) gk[v] = fac * oh[v];
+  }
+  g[(size_t)K * D::BLK + k] = gb * 2.0 * nrm * (HC[k] - E * SC[k]);
+}
+
+// ----------------------------------------------------------- queries -----
+
+// Per-term matel / gradel for (stf, i, j) triples (python/matel.py:28,
+// python/gradel.py:27).  The ket record is rebuilt in registers per query.
+template <int N>
+__device__ void ket_record(const double *__restrict__ bra, int l, int p,
+                           const double *__restrict__ sml, const double *__restrict__ sms,
+                           const SysConst &sc, double *r) {
+  using D = Dim<N>;
+  const double *b = bra + (size_t)l * D::BS;
+  const double *T = sml + (size_t)p * N * N;
+  const double *Sm = sms + (size_t)p * D::N3 * D::N3;
+  double AT[N][N], Ap[N][N];
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      double s = 0.0;
+#pragma unroll
+      for (int m = 0; m < N; ++m) s = __builtin_fma(b[D::B_A + sx(i, m)], T[m * N + j], s);
+      AT[i][j] = s;
+    }
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      double s = 0.0;
+#pragma unroll
+      for (int m = 0; m < N; ++m) s = __builtin_fma(T[m * N + i], AT[m][j], s);
+      Ap[i][j] = s;
+    }
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = 0; j <= i; ++j) r[D::K_AP + sx(i, j)] = 0.5 * (Ap[i][j] + Ap[j][i]);
+  double Sp[D::N3];
+#pragma unroll
+  for (int u = 0; u < D::N3; ++u) {
+    double s = 0.0;
+#pragma unroll
+    for (int v = 0; v < D::N3; ++v) s = __builtin_fma(Sm[u * D::N3 + v], b[D::B_S + v], s);
+    Sp[u] = s;
+    r[D::K_SP + u] = s;
+  }
+  double i2 = 0.0;
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      double s = 0.0;
+#pragma unroll
+      for (int j = 0; j < N; ++j) s = __builtin_fma(Ap[i][j], Sp[3 * j + c], s);
+      r[D::K_APSP + 3 * i + c] = s;
+      i2 = __builtin_fma(Sp[3 * i + c], s, i2);
+    }
+  double ApM[N][N];
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      double s = 0.0;
+#pragma unroll
+      for (int m = 0; m < N; ++m) s = __builtin_fma(Ap[i][m], sc.mass[m * N + j], s);
+      ApM[i][j] = s;
+      r[D::K_APM + i * N + j] = s;
+    }
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = 0; j <= i; ++j) {
+      double s = 0.0, s2 = 0.0;
+#pragma unroll
+      for (int m = 0; m < N; ++m) {
+        s = __builtin_fma(ApM[i][m], Ap[m][j], s);
+        s2 = __builtin_fma(ApM[j][m], Ap[m][i], s2);
+      }
+      r[D::K_NP + sx(i, j)] = 0.5 * (s + s2);
+    }
+  r[D::K_I2] = i2;
+  r[D::K_PL] = b[D::B_PK];
+  r[D::K_WC] = 1.0;
+  r[D::K_SC] = 1.0;
+}
+
+template <int N>
+__global__ void query_kernel(const double *__restrict__ bra, int K, int nst,
+                             const double *__restrict__ sml, const double *__restrict__ sms,
+                             SysConst sc, int nq, const int32_t *__restrict__ q,
+                             double *__restrict__ out4, double *__restrict__ tgvl,
+                             double *__restrict__ tgvls, int *__restrict__ flag) {
+  using D = Dim<N>;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nq) return;
+  const int p = q[3 * t], i = q[3 * t + 1], j = q[3 * t + 2];
+  if (p < 0 || p >= nst || i < 0 || i >= K || j < 0 || j >= K) {
+    atomicOr(flag, (int)F_NONFINITE);
+    return;
+  }
+  double kr[D::KS];
+  ket_record<N>(bra, j, p, sml, sms, sc, kr);
+  Bra<N> b;
+  load_bra<N>(b, bra + (size_t)i * D::BS);
+  Acc<N> acc;
+  acc.zero();
+  double hel = 0.0, sel = 0.0;
+  Parts parts;
+  bool ok;
+  if (tgvl)
+    ok = pair_term<N, MODE_SPLIT>(b, kr, sc, 0.0, hel, sel, acc, &parts);
+  else
+    ok = pair_term<N, MODE_ELEMENTS>(b, kr, sc, 0.0, hel, sel, acc, &parts);
+  if (!ok) atomicOr(flag, (int)F_NOT_PD);
+  if (out4) {
+    out4[4 * t + 0] = parts.ov;
+    out4[4 * t + 1] = parts.tk;
+    out4[4 * t + 2] = parts.ert;
+    out4[4 * t + 3] = parts.zht;
+  }
+  if (tgvl) {
+    const double *r = bra + (size_t)i * D::BS;
+    const double two = (i == j) ? 2.0 : 1.0;  // python/gradel.py:388-390
+    double o[D::BLK];
+    grad_L<N>(acc.XH, acc.sH, r + D::B_L, r + D::B_LD, o);
+    grad_S<N>(r + D::B_A, acc.YH, o + D::NS);
+#pragma unroll
+    for (int v = 0; v < D::BLK; ++v) tgvl[(size_t)t * D::BLK + v] = two * o[v];
+    grad_L<N>(acc.XS, acc.sS, r + D::B_L, r + D::B_LD, o);
+    grad_S<N>(r + D::B_A, acc.YS, o + D::NS);
+#pragma unroll
+    for (int v = 0; v < D::BLK; ++v) tgvls[(size_t)t * D::BLK + v] = two * o[v];
+  }
+}
+
+// Materialised DH/DS (python/make_grad.py:34-55): lane per ordered (l, k),
+// all terms, bra l.  DH[(l*blk + r)*K + k].
+template <int N>
+__global__ void dhds_kernel(const double *__restrict__ bra, const double *__restrict__ ket,
+                            int K, int nst, SysConst sc, double *__restrict__ DH,
+                            double *__restrict__ DS, int *__restrict__ flag) {
+  using D = Dim<N>;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;  // ket (coalesced writes)
+  const int l = blockIdx.y;                              // bra
+  if (k >= K) return;
+  Bra<N> b;
+  load_bra<N>(b, bra + (size_t)l * D::BS);
+  Acc<N> acc;
+  acc.zero();
+  double hel = 0.0, sel = 0.0;
+  bool ok = true;
+  for (int p = 0; p < nst; ++p) {
+    // weight scvv_P only (the ket table carries scvv_P C_k in K_WC)
+    const double *kr = ket + ((size_t)k * nst + p) * D::KS;
+    double rec[D::KS];
+#pragma unroll
+    for (int v = 0; v < D::KS; ++v) rec[v] = kr[v];
+    rec[D::K_WC] = rec[D::K_SC];
+    ok &= pair_term<N, MODE_SPLIT>(b, rec, sc, 0.0, hel, sel, acc, nullptr);
+  }
+  if (!ok) atomicOr(flag, (int)F_NOT_PD);
+  const double *r = bra + (size_t)l * D::BS;
+  const double two = (l == k) ? 2.0 : 1.0;
+  double o[D::BLK];
+  grad_L<N>(acc.XH, acc.sH, r + D::B_L, r + D::B_LD, o);
+  grad_S<N>(r + D::B_A, acc.YH, o + D::NS);
+#pragma unroll
+  for (int v = 0; v < D::BLK; ++v) DH[((size_t)l * D::BLK + v) * K + k] = two * o[v];
+  grad_L<N>(acc.XS, acc.sS, r + D::B_L, r + D::B_LD, o);
+  grad_S<N>(r + D::B_A, acc.YS, o + D::NS);
+#pragma unroll
+  for (int v = 0; v < D::BLK; ++v) DS[((size_t)l * D::BLK + v) * K + k] = two * o[v];
+}
+
+}  // namespace
+
+// =================================================================== host ===
+
+struct necg_ctx {
+  int device = 0;
+  int n = 0, nst = 0;
+  SysConst sc{};
+  double *d_scvv = nullptr, *d_sml = nullptr, *d_sms = nullptr;
+  int *d_flag = nullptr;
+  // per-K workspace
+  int Kcap = 0;
+  double *bra = nullptr, *ket = nullptr, *H = nullptr, *S = nullptr, *HC = nullptr,
+         *SC = nullptr, *scal = nullptr;
+  double *part = nullptr, *hpart = nullptr, *spart = nullptr;
+  size_t part_cap = 0, hpart_cap = 0;
+  int req_kt = 0, req_ts = 0, req_w = 0;
+  int last_kt = 0, last_ts = 0, last_w = 0;
+  int last_K = 0;
+  // pair-kernel timing (necg_timing)
+  bool timing = false;
+  std::vector<hipEvent_t> ev;  // start/stop pairs, reused
+  int nev = 0;
+  std::mutex mu;
+};
+
+namespace {
+
+int hip_status(hipError_t e) {
+  if (e == hipSuccess) return NECG_OK;
+  if (e == hipErrorOutOfMemory) return NECG_ERR_OOM;
+  return NECG_ERR_DEVICE;
+}
+
+#define HIP_TRY(expr)                         \
+  do {                                        \
+    int rc__ = hip_status((expr));            \
+    if (rc__ != NECG_OK) return rc__;         \
+  } while (0)
+
+int dev_alloc(double **p, size_t count) {
+  if (*p) {
+    (void)hipFree(*p);
+    *p = nullptr;
+  }
+  if (count == 0) return NECG_OK;
+  return hip_status(hipMalloc((void **)p, count * sizeof(double)));
+}
+
+int bra_stride(int n) {
+  switch (n) {
+    case 1: return Dim<1>::BS;
+    case 2: return Dim<2>::BS;
+    case 3: return Dim<3>::BS;
+    case 4: return Dim<4>::BS;
+    case 5: return Dim<5>::BS;
+    default: return Dim<6>::BS;
+  }
+}
+int ket_stride(int n) {
+  switch (n) {
+    case 1: return Dim<1>::KS;
+    case 2: return Dim<2>::KS;
+    case 3: return Dim<3>::KS;
+    case 4: return Dim<4>::KS;
+    case 5: return Dim<5>::KS;
+    default: return Dim<6>::KS;
+  }
+}
+int bra_c_off(int n) {
+  switch (n) {
+    case 1: return Dim<1>::B_C;
+    case 2: return Dim<2>::B_C;
+    case 3: return Dim<3>::B_C;
+    case 4: return Dim<4>::B_C;
+    case 5: return Dim<5>::B_C;
+    default: return Dim<6>::B_C;
+  }
+}
+
+int ensure_K(necg_ctx *c, int K) {
+  if (K <= c->Kcap) return NECG_OK;
+  int rc;
+  const size_t k = (size_t)K;
+  if ((rc = dev_alloc(&c->bra, k * bra_stride(c->n)))) return rc;
+  if ((rc = dev_alloc(&c->ket, k * c->nst * ket_stride(c->n)))) return rc;
+  if ((rc = dev_alloc(&c->H, k * k))) return rc;
+  if ((rc = dev_alloc(&c->S, k * k))) return rc;
+  if ((rc = dev_alloc(&c->HC, k))) return rc;
+  if ((rc = dev_alloc(&c->SC, k))) return rc;
+  if ((rc = dev_alloc(&c->scal, 8))) return rc;
+  c->Kcap = K;
+  return NECG_OK;
+}
+
+int ensure_part(necg_ctx *c, size_t count) {
+  if (count <= c->part_cap) return NECG_OK;
+  int rc = dev_alloc(&c->part, count);
+  if (rc) return rc;
+  c->part_cap = count;
+  return NECG_OK;
+}
+int ensure_hpart(necg_ctx *c, size_t count) {
+  if (count <= c->hpart_cap) return NECG_OK;
+  int rc = dev_alloc(&c->hpart, count);
+  if (rc) return rc;
+  rc = dev_alloc(&c->spart, count);
+  if (rc) return rc;
+  c->hpart_cap = count;
+  return NECG_OK;
+}
+
+// Launch shape: lanes are bras (64 per wave); a work item covers `kt` kets
+// and `tps` terms.  Aim for a few thousand waves so that all 256 CUs stay
+// busy without a long tail.
+void choose_tiling(necg_ctx *c, int K, int &kt, int &ts, int &w) {
+  const int nst = c->nst;
+  w = c->req_w > 0 ? c->req_w : 1;
+  const long bra_waves = (K + 63) / 64;
+  const long target = 4096;
+  if (c->req_kt > 0) {
+    kt = c->req_kt;
+  } else {
+    long ket_tiles = std::max<long>(1, target / bra_waves);
+    kt = (int)std::max<long>(1, (K + ket_tiles - 1) / ket_tiles);
+  }
+  if (c->req_ts > 0) {
+    ts = std::min(c->req_ts, nst);
+  } else {
+    long waves = bra_waves * ((K + kt - 1) / kt);
+    ts = 1;
+    while (waves * ts < target && ts < nst) ts *= 2;
+    ts = std::min(ts, nst);
+  }
+}
+
+template <int N>
+int run_prep(necg_ctx *c, int K, const double *x, int with_c, hipStream_t st) {
+  const int tb = 256;
+  hipLaunchKernelGGL((prep_bra<N>), dim3((K + tb - 1) / tb), dim3(tb), 0, st, x, K, with_c,
+                     c->bra, c->d_flag);
+  const int nk = K * c->nst;
+  hipLaunchKernelGGL((prep_ket<N>), dim3((nk + tb - 1) / tb), dim3(tb), 0, st, c->bra, K,
+                     c->nst, c->d_sml, c->d_sms, c->d_scvv, c->sc, c->ket);
+  return hip_status(hipGetLastError());
+}
+
+// Runs the pair kernel in MODE and the finishing kernels.  f, H, S, g may be
+// null where the mode does not produce them.
+template <int N, int MODE>
+int run_pairs(necg_ctx *c, int K, hipStream_t st, int &ntiles, int &ts_out) {
+  int kt, ts, w;
+  choose_tiling(c, K, kt, ts, w);
+  const int tps = (c->nst + ts - 1) / ts;
+  ts = (c->nst + tps - 1) / tps;
+  const int kts = (K + kt - 1) / kt;
+  const int threads = 64 * w;
+  dim3 grid((K + threads - 1) / threads, kts, ts);
+  ntiles = kts * ts;
+  ts_out = ts;
+  int rc;
+  if (MODE != MODE_ELEMENTS) {
+    if ((rc = ensure_part(c, (size_t)ntiles * Dim<N>::NACC * K))) return rc;
+  }
+  if (MODE != MODE_COMBINED) {
+    if ((rc = ensure_hpart(c, (size_t)ts * K * K))) return rc;
+  }
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (c->timing) {
+    if ((size_t)(2 * c->nev + 2) > c->ev.size()) {
+      hipEvent_t a, b;
+      HIP_TRY(hipEventCreate(&a));
+      HIP_TRY(hipEventCreate(&b));
+      c->ev.push_back(a);
+      c->ev.push_back(b);
+    }
+    e0 = c->ev[2 * c->nev];
+    e1 = c->ev[2 * c->nev + 1];
+    c->nev++;
+    HIP_TRY(hipEventRecord(e0, st));
+  }
+  hipLaunchKernelGGL((pair_kernel<N, MODE>), grid, dim3(threads), 0, st, c->bra, c->ket, K,
+                     c->nst, kt, tps, c->sc, c->scal, c->part, c->hpart, c->spart, c->d_flag);
+  if (c->timing) HIP_TRY(hipEventRecord(e1, st));
+  c->last_kt = kt;
+  c->last_ts = ts;
+  c->last_w = w;
+  return hip_status(hipGetLastError());
+}
+
+int finish_hs(necg_ctx *c, int K, int ts, hipStream_t st) {
+  hipLaunchKernelGGL(hs_finish, dim3((K + 255) / 256, K), dim3(256), 0, st, c->hpart, c->spart,
+                     K, ts, c->H, c->S);
+  return hip_status(hipGetLastError());
+}
+
+int finish_quotient(necg_ctx *c, int K, double *f, hipStream_t st) {
+  const int bs = bra_stride(c->n), bc = bra_c_off(c->n);
+  hipLaunchKernelGGL(hc_sc, dim3((K + 255) / 256), dim3(256), 0, st, c->H, c->S, c->bra, bs,
+                     bc, K, c->HC, c->SC);
+  hipLaunchKernelGGL(quotient, dim3(1), dim3(1024), 0, st, c->HC, c->SC, c->bra, bs, bc, K,
+                     c->scal, f, c->d_flag);
+  return hip_status(hipGetLastError());
+}
+
+template <int N, int MODE>
+int finish_grad(necg_ctx *c, int K, int ntiles, const double *gbar, double *g, hipStream_t st) {
+  hipLaunchKernelGGL((grad_final<N, MODE>), dim3((K + 255) / 256), dim3(256), 0, st, c->part,
+                     ntiles, K, c->bra, c->scal, c->HC, c->SC, gbar, g);
+  return hip_status(hipGetLastError());
+}
+
+template <int N>
+int do_outprod(necg_ctx *c, int K, const double *x, double *f, double *g, hipStream_t st) {
+  int rc, ntiles, ts;
+  if ((rc = run_prep<N>(c, K, x, 1, st))) return rc;
+  if ((rc = run_pairs<N, MODE_SPLIT>(c, K, st, ntiles, ts))) return rc;
+  if ((rc = finish_hs(c, K, ts, st))) return rc;
+  if ((rc = finish_quotient(c, K, f, st))) return rc;
+  return finish_grad<N, MODE_SPLIT>(c, K, ntiles, nullptr, g, st);
+}
+
+template <int N>
+int do_energy(necg_ctx *c, int K, const double *x, double *f, double *H, double *S,
+              hipStream_t st) {
+  int rc, ntiles, ts;
+  if ((rc = run_prep<N>(c, K, x, 1, st))) return rc;
+  if ((rc = run_pairs<N, MODE_ELEMENTS>(c, K, st, ntiles, ts))) return rc;
+  if ((rc = finish_hs(c, K, ts, st))) return rc;
+  if ((rc = finish_quotient(c, K, f, st))) return rc;
+  if (H) HIP_TRY(hipMemcpyAsync(H, c->H, sizeof(double) * K * K, hipMemcpyDeviceToDevice, st));
+  if (S) HIP_TRY(hipMemcpyAsync(S, c->S, sizeof(double) * K * K, hipMemcpyDeviceToDevice, st));
+  return NECG_OK;
+}
+
+template <int N>
+int do_vjp(necg_ctx *c, int K, const double *x, const double *gbar, double *gx,
+           hipStream_t st) {
+  int rc, ntiles, ts;
+  // the bra/ket tables of x are rebuilt: x may have been rewritten in place
+  if ((rc = run_prep<N>(c, K, x, 1, st))) return rc;
+  if ((rc = run_pairs<N, MODE_COMBINED>(c, K, st, ntiles, ts))) return rc;
+  return finish_grad<N, MODE_COMBINED>(c, K, ntiles, gbar, gx, st);
+}
+
+template <int N>
+int do_matrices(necg_ctx *c, int K, const double *iv, double *H, double *S, hipStream_t st) {
+  int rc, ntiles, ts;
+  if ((rc = run_prep<N>(c, K, iv, 0, st))) return rc;
+  if ((rc = run_pairs<N, MODE_ELEMENTS>(c, K, st, ntiles, ts))) return rc;
+  if ((rc = finish_hs(c, K, ts, st))) return rc;
+  if (H) HIP_TRY(hipMemcpyAsync(H, c->H, sizeof(double) * K * K, hipMemcpyDeviceToDevice, st));
+  if (S) HIP_TRY(hipMemcpyAsync(S, c->S, sizeof(double) * K * K, hipMemcpyDeviceToDevice, st));
+  return NECG_OK;
+}
+
+template <int N>
+int do_query(necg_ctx *c, int K, const double *iv, int nq, const int32_t *q, double *out4,
+             double *tgvl, double *tgvls, hipStream_t st) {
+  int rc;
+  if ((rc = run_prep<N>(c, K, iv, 0, st))) return rc;
+  if (nq == 0) return NECG_OK;
+  hipLaunchKernelGGL((query_kernel<N>), dim3((nq + 63) / 64), dim3(64), 0, st, c->bra, K,
+                     c->nst, c->d_sml, c->d_sms, c->sc, nq, q, out4, tgvl, tgvls, c->d_flag);
+  return hip_status(hipGetLastError());
+}
+
+template <int N>
+int do_dhds(necg_ctx *c, int K, const double *iv, double *DH, double *DS, hipStream_t st) {
+  int rc;
+  if ((rc = run_prep<N>(c, K, iv, 0, st))) return rc;
+  hipLaunchKernelGGL((dhds_kernel<N>), dim3((K + 63) / 64, K), dim3(64), 0, st, c->bra, c->ket,
+                     K, c->nst, c->sc, DH, DS, c->d_flag);
+  return hip_status(hipGetLastError());
+}
+
+#define DISPATCH_N(n, call)                 \
+  switch (n) {                              \
+    case 1: { constexpr int NN = 1; return call; } \
+    case 2: { constexpr int NN = 2; return call; } \
+    case 3: { constexpr int NN = 3; return call; } \
+    case 4: { constexpr int NN = 4; return call; } \
+    case 5: { constexpr int NN = 5; return call; } \
+    case 6: { constexpr int NN = 6; return call; } \
+    default: return NECG_ERR_UNSUPPORTED;   \
+  }
+
+struct DeviceGuard {
+  int prev = -1;
+  explicit DeviceGuard(int dev) {
+    if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+    if (prev != dev) (void)hipSetDevice(dev);
+  }
+  ~DeviceGuard() {
+    if (prev >= 0) (void)hipSetDevice(prev);
+  }
+};
+
+}  // namespace
+
+// ================================================================= C ABI ===
+
+extern "C" {
+
+const char *necg_strerror(int code) {
+  switch (code) {
+    case NECG_OK: return "ok";
+    case NECG_ERR_ARG: return "invalid argument";
+    case NECG_ERR_UNSUPPORTED: return "unsupported system";
+    case NECG_ERR_DEVICE: return "HIP runtime error";
+    case NECG_ERR_OOM: return "device out of memory";
+    case NECG_ERR_NOT_PD: return "pair matrix not positive definite (zero Cholesky diagonal?)";
+    case NECG_ERR_CSC: return "C^T S C <= 0";
+    case NECG_ERR_NONFINITE: return "NaN or Inf in input or result";
+    default: return "unknown status";
+  }
+}
+
+int necg_create(const necg_system *sys, int device, necg_ctx **out) {
+  if (!sys || !out) return NECG_ERR_ARG;
+  *out = nullptr;
+  const int n = sys->n, nst = sys->nst;
+  if (n < 1 || n > kMaxN) return NECG_ERR_UNSUPPORTED;
+  if (nst < 1 || !sys->mass || !sys->ncv || !sys->scvv || !sys->sml || !sys->sms)
+    return NECG_ERR_ARG;
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j)
+      if (sys->mass[i * n + j] != sys->mass[j * n + i]) return NECG_ERR_ARG;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev)
+    return NECG_ERR_DEVICE;
+  necg_ctx *c = new (std::nothrow) necg_ctx();
+  if (!c) return NECG_ERR_OOM;
+  c->device = device;
+  c->n = n;
+  c->nst = nst;
+  c->sc.n = n;
+  c->sc.nst = nst;
+  c->sc.efs = sys->efs;
+  for (int i = 0; i < n * n; ++i) c->sc.mass[i] = sys->mass[i];
+  for (int i = 0; i <= n; ++i) c->sc.q[i] = sys->ncv[i];
+  int pi = 0;
+  for (int p = 0; p <= n; ++p)
+    for (int q = p + 1; q <= n; ++q) c->sc.cpair[pi++] = sys->ncv[p] * sys->ncv[q];
+  DeviceGuard g(device);
+  int rc = NECG_OK;
+  const size_t nsml = (size_t)nst * n * n, nsms = (size_t)nst * 9 * n * n;
+  if ((rc = dev_alloc(&c->d_scvv, nst)) || (rc = dev_alloc(&c->d_sml, nsml)) ||
+      (rc = dev_alloc(&c->d_sms, nsms))) {
+    necg_destroy(c);
+    return rc;
+  }
+  if ((rc = hip_status(hipMemcpy(c->d_scvv, sys->scvv, sizeof(double) * nst,
+                                 hipMemcpyHostToDevice))) ||
+      (rc = hip_status(hipMemcpy(c->d_sml, sys->sml, sizeof(double) * nsml,
+                                 hipMemcpyHostToDevice))) ||
+      (rc = hip_status(hipMemcpy(c->d_sms, sys->sms, sizeof(double) * nsms,
+                                 hipMemcpyHostToDevice))) ||
+      (rc = hip_status(hipMalloc((void **)&c->d_flag, sizeof(int)))) ||
+      (rc = hip_status(hipMemset(c->d_flag, 0, sizeof(int))))) {
+    necg_destroy(c);
+    return rc;
+  }
+  *out = c;
+  return NECG_OK;
+}
+
+int necg_destroy(necg_ctx *c) {
+  if (!c) return NECG_ERR_ARG;
+  {
+    DeviceGuard g(c->device);
+    double *bufs[] = {c->d_scvv, c->d_sml, c->d_sms, c->bra, c->ket, c->H, c->S,
+                      c->HC, c->SC, c->scal, c->part, c->hpart, c->spart};
+    for (double *p : bufs)
+      if (p) (void)hipFree(p);
+    if (c->d_flag) (void)hipFree(c->d_flag);
+    for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
+  }
+  delete c;
+  return NECG_OK;
+}
+
+int necg_outprod(necg_ctx *c, int32_t K, const double *x, double *f, double *g, void *stream) {
+  if (!c || K < 1 || !x || !g) return NECG_ERR_ARG;
+  std::lock_guard<std::mutex> lk(c->mu);
+  DeviceGuard dg(c->device);
+  int rc = ensure_K(c, K);
+  if (rc) return rc;
+  c->last_K = K;
+  DISPATCH_N(c->n, (do_outprod<NN>(c, K, x, f, g, (hipStream_t)stream)));
+}
+
+int necg_energy(necg_ctx *c, int32_t K, const double *x, double *f, double *H, double *S,
+                void *stream) {
+  if (!c || K < 1 || !x) return NECG_ERR_ARG;
+  std::lock_guard<std::mutex> lk(c->mu);
+  DeviceGuard dg(c->device);
+  int rc = ensure_K(c, K);
+  if (rc) return rc;
+  c->last_K = K;
+  DISPATCH_N(c->n, (do_energy<NN>(c, K, x, f, H, S, (hipStream_t)stream)));
+}
+
+int necg_energy_vjp(necg_ctx *c, int32_t K, const double *x, const double *gbar, double *gx,
+                    void *stream) {
+  if (!c || K < 1 || !x || !gx) return NECG_ERR_ARG;
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (c->last_K != K || K > c->Kcap) return NECG_ERR_ARG;  // needs necg_energy first
+  DeviceGuard dg(c->device);
+  DISPATCH_N(c->n, (do_vjp<NN>(c, K, x, gbar, gx, (hipStream_t)stream)));
+}
+
+int necg_matrices(necg_ctx *c, int32_t K, const double *iv, double *H, double *S, void *stream) {
+  if (!c || K < 1 || !iv) return NECG_ERR_ARG;
+  std::lock_guard<std::mutex> lk(c->mu);
+  DeviceGuard dg(c->device);
+  int rc = ensure_K(c, K);
+  if (rc) return rc;
+  DISPATCH_N(c->n, (do_matrices<NN>(c, K, iv, H, S, (hipStream_t)stream)));
+}
+
+int necg_matel(necg_ctx *c, int32_t K, const double *iv, int32_t nq, const int32_t *q,
+               double *out, void *stream) {
+  if (!c || K < 1 || !iv || nq < 0 || (nq > 0 && (!q || !out))) return NECG_ERR_ARG;
+  std::lock_guard<std::mutex> lk(c->mu);
+  DeviceGuard dg(c->device);
+  int rc = ensure_K(c, K);
+  if (rc) return rc;
+  DISPATCH_N(c->n, (do_query<NN>(c, K, iv, nq, q, out, nullptr, nullptr, (hipStream_t)stream)));
+}
+
+int necg_gradel(necg_ctx *c, int32_t K, const double *iv, int32_t nq, const int32_t *q,
+                double *tgvl, double *tgvls, void *stream) {
+  if (!c || K < 1 || !iv || nq < 0 || (nq > 0 && (!q || !tgvl || !tgvls))) return NECG_ERR_ARG;
+  std::lock_guard<std::mutex> lk(c->mu);
+  DeviceGuard dg(c->device);
+  int rc = ensure_K(c, K);
+  if (rc) return rc;
+  DISPATCH_N(c->n,
+             (do_query<NN>(c, K, iv, nq, q, nullptr, tgvl, tgvls, (hipStream_t)stream)));
+}
+
+int necg_dhds(necg_ctx *c, int32_t K, const double *iv, double *DH, double *DS, void *stream) {
+  if (!c || K < 1 || !iv || !DH || !DS) return NECG_ERR_ARG;
+  std::lock_guard<std::mutex> lk(c->mu);
+  DeviceGuard dg(c->device);
+  int rc = ensure_K(c, K);
+  if (rc) return rc;
+  DISPATCH_N(c->n, (do_dhds<NN>(c, K, iv, DH, DS, (hipStream_t)stream)));
+}
+
+int necg_status(necg_ctx *c, void *stream) {
+  if (!c) return NECG_ERR_ARG;
+  std::lock_guard<std::mutex> lk(c->mu);
+  DeviceGuard dg(c->device);
+  HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+  int flags = 0;
+  HIP_TRY(hipMemcpy(&flags, c->d_flag, sizeof(int), hipMemcpyDeviceToHost));
+  if (flags) HIP_TRY(hipMemset(c->d_flag, 0, sizeof(int)));
+  if (flags & F_NONFINITE) return NECG_ERR_NONFINITE;
+  if (flags & F_NOT_PD) return NECG_ERR_NOT_PD;
+  if (flags & F_CSC) return NECG_ERR_CSC;
+  return NECG_OK;
+}
+
+int necg_set_tiling(necg_ctx *c, int32_t ket_tile, int32_t term_split, int32_t waves) {
+  if (!c || ket_tile < 0 || term_split < 0 || waves < 0 || waves > 4) return NECG_ERR_ARG;
+  std::lock_guard<std::mutex> lk(c->mu);
+  c->req_kt = ket_tile;
+  c->req_ts = term_split;
+  c->req_w = waves;
+  return NECG_OK;
+}
+
+int necg_timing(necg_ctx *c, int32_t mode, double *ms, int64_t *launches) {
+  if (!c || mode < 0 || mode > 2) return NECG_ERR_ARG;
+  std::lock_guard<std::mutex> lk(c->mu);
+  DeviceGuard dg(c->device);
+  if (mode == 1) {
+    c->timing = true;
+    c->nev = 0;
+    return NECG_OK;
+  }
+  if (mode == 0) {
+    c->timing = false;
+    return NECG_OK;
+  }
+  double tot = 0.0;
+  for (int i = 0; i < c->nev; ++i) {
+    HIP_TRY(hipEventSynchronize(c->ev[2 * i + 1]));
+    float t = 0.f;
+    HIP_TRY(hipEventElapsedTime(&t, c->ev[2 * i], c->ev[2 * i + 1]));
+    tot += t;
+  }
+  if (ms) *ms = tot;
+  if (launches) *launches = c->nev;
+  return NECG_OK;
+}
+
+int necg_get_tiling(necg_ctx *c, int32_t *ket_tile, int32_t *term_split, int32_t *waves) {
+  if (!c || !ket_tile || !term_split || !waves) return NECG_ERR_ARG;
+  std::lock_guard<std::mutex> lk(c->mu);
+  *ket_tile = c->last_kt;
+  *term_split = c->last_ts;
+  *waves = c->last_w;
+  return NECG_OK;
+}
+
+}  // extern "C"

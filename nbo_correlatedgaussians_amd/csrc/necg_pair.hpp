@@ -1,0 +1,470 @@
+// necg_pair.hpp -- the per (bra, ket, symmetry term) FP64 algebra of the ECG
+// energy path, written for one lane of a CDNA4 wavefront.
+//
+// Notation (n-space: every 3n x 3n operator kron(X, I3) of the reference acts
+// as the n x n matrix X on an n x 3 matrix whose row i is the xyz of
+// pseudo-particle i):
+//   A = L L^T (bra), A' = T^T A_l T and S' = sms s_l (ket under term P,
+//   python/matel.py:76-87), B = A + A', E = A S + A' S', st = B^-1 E,
+//   dif1 = st - S, dif2 = st - S', M = inverse-mass tensor, q = charges.
+//
+// Elements (python/matel.py:92-208):
+//   ov  = pk pl det(B)^-3/2 exp(i3 - i1 - i2)            (:125-136)
+//   tk  = 2 ov (2 <A dif1, M A' dif2> + 3 tr(M A B^-1 A')) (:144-155)
+//   ert = 2 ov sum_{p<q} q_p q_q F0(|st^T w|^2 / w^T B^-1 w) / sqrt(pi w^T B^-1 w)
+//         with w = e_q (p = 0) or e_p - e_q                    (:162-197)
+//   zht = ov efs sum_i q_i st[i, z]                            (:205-208)
+// so that tk + ert - zht = ov * alpha with
+//   alpha = 2 tfac + 2 sum c f r0 - efs zsum,  tfac = 2 i3k + T2.
+//
+// Bra gradient (python/gradel.py:108-390).  Every term of gradel has the form
+// vech(X L) for the vech(L) block and A Y for the shift block, so the gradient
+// of one term is carried as a symmetric n x n X and an n x 3 Y:
+//   S part:  X_S = ov (1.5 A^-1 + X0),  Y_S = 2 ov dif1,
+//            X0  = -3 B^-1 - 2 dif1 dif1^T          (ogl :114-148, ogs :108)
+//   H part:  X_H = alpha ov (1.5 A^-1 + X0) + ov B^-1 Gam B^-1 - (dif1 Phi^T + Phi dif1^T)
+//            Y_H = 2 alpha ov dif1 + Phi
+//            Gam = 12 A' M A' + La,  Phi = ov B^-1 F^T,
+//            F^T = 4 A' M (A dif1 - A' dif2) + Lb st - efs q e_z^T
+//   where La = sum_c c (2 f + 4 f' z) r0 / i2pq  w w^T and Lb = sum_c c 4 f' r0 / i2pq w w^T.
+// These identities follow from gradel's own terms: the six kcm products
+// (:233-254) collapse to -V dif1^T with V = B^-1 A' M (A dif1 - A' dif2), the
+// two "6 ov" products (:189-197) to 6 B^-1 A' M A' B^-1, the Coulomb rank-one
+// updates (:296-335) to B^-1 La B^-1 and dif1 st^T Lb B^-1 (+ transpose), and
+// the Stark updates (:346-370) to the efs q e_z^T column of F.  The 1.5 A^-1
+// part is kept as a scalar: vech(A^-1 L) = vech(L^-T) = diag(1 / L_ii).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace necg {
+
+constexpr double kPi = 3.14159265358979311599796346854;  // python/shared.py:71
+constexpr double kZcut = 16.3578;                         // python/utils.py:85
+// Pade constants of F0 (python/shared.py:76-86)
+constexpr double kC1 = 0.564691197e-04, kC2 = 0.758433197e-03, kC3 = 0.769838037e-02,
+                 kC4 = 0.629344460e-01, kC5 = 0.213271302e+00, kC6 = 0.720266520e-04,
+                 kC7 = 0.955528842e-03, kC8 = 0.101431553e-01, kC9 = 0.738522953e-01,
+                 kC10 = 0.338450368e+00, kC11 = 0.879937801e+00;
+
+constexpr int kMaxN = 6;
+
+__host__ __device__ constexpr int sx(int i, int j) {
+  return i >= j ? i * (i + 1) / 2 + j : j * (j + 1) / 2 + i;
+}
+
+template <int N>
+struct Dim {
+  static constexpr int NS = N * (N + 1) / 2;  // packed symmetric / vech length
+  static constexpr int N3 = 3 * N;            // shift block length
+  static constexpr int BLK = NS + N3;         // parameters per basis function
+  static constexpr int NP = N * (N + 1) / 2;  // Coulomb pairs incl. reference
+  // ket-table record for one (ket, term): A' | S' | A'S' | A'MA' | A'M | i2 pl wc sc
+  static constexpr int K_AP = 0;
+  static constexpr int K_SP = K_AP + NS;
+  static constexpr int K_APSP = K_SP + N3;
+  static constexpr int K_NP = K_APSP + N3;
+  static constexpr int K_APM = K_NP + NS;
+  static constexpr int K_I2 = K_APM + N * N;
+  static constexpr int K_PL = K_I2 + 1;
+  static constexpr int K_WC = K_PL + 1;  // scvv_P * C_l
+  static constexpr int K_SC = K_WC + 1;  // scvv_P
+  static constexpr int KS = (K_SC + 1 + 1) & ~1;
+  // bra-table record: A | S | AS | i1 pk C | L (vech) | 1/L_ii
+  static constexpr int B_A = 0;
+  static constexpr int B_S = B_A + NS;
+  static constexpr int B_AS = B_S + N3;
+  static constexpr int B_I1 = B_AS + N3;
+  static constexpr int B_PK = B_I1 + 1;
+  static constexpr int B_C = B_PK + 1;
+  static constexpr int B_L = B_C + 1;
+  static constexpr int B_LD = B_L + NS;
+  static constexpr int BS = (B_LD + N + 1) & ~1;
+  // accumulator record of one bra: X_H Y_H X_S Y_S sigma_H sigma_S
+  static constexpr int A_XH = 0;
+  static constexpr int A_YH = A_XH + NS;
+  static constexpr int A_XS = A_YH + N3;
+  static constexpr int A_YS = A_XS + NS;
+  static constexpr int A_SH = A_YS + N3;
+  static constexpr int A_SS = A_SH + 1;
+  static constexpr int NACC = A_SS + 1;
+};
+
+// Kernel-argument block: the physics constants of one system.
+struct SysConst {
+  int n, nst;
+  double efs;
+  double mass[kMaxN * kMaxN];
+  double q[kMaxN + 1];                      // ncv
+  double cpair[kMaxN * (kMaxN + 1) / 2];    // q_p q_q in pair order
+};
+
+// 1/x: hardware seed + two Newton steps (relative error ~1 ulp for the
+// normal range that reaches here).
+__device__ __forceinline__ double rcp(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = __builtin_fma(-x, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  e = __builtin_fma(-x, r, 1.0);
+  return __builtin_fma(r, e, r);
+}
+
+__device__ __forceinline__ double rsq(double x) { return rsqrt(x); }
+
+// Boys F0 with the reference's Pade form and its derivative
+// (python/utils.py:78-106): F0 = sqrt(P5/Q6) for z <= 16.3578, else
+// sqrt(pi/(4z)); dF0 = (Q P' - P Q') / (2 F0 Q^2), or -F0/(2z).
+__device__ __forceinline__ void boys_f0(double z, double &f, double &fp) {
+  if (z <= kZcut) {
+    double P = __builtin_fma(__builtin_fma(__builtin_fma(__builtin_fma(__builtin_fma(kC1, z, kC2), z, kC3), z, kC4), z, kC5), z, 1.0);
+    double Q = __builtin_fma(__builtin_fma(__builtin_fma(__builtin_fma(__builtin_fma(__builtin_fma(kC6, z, kC7), z, kC8), z, kC9), z, kC10), z, kC11), z, 1.0);
+    double dP = __builtin_fma(__builtin_fma(__builtin_fma(__builtin_fma(5.0 * kC1, z, 4.0 * kC2), z, 3.0 * kC3), z, 2.0 * kC4), z, kC5);
+    double dQ = __builtin_fma(__builtin_fma(__builtin_fma(__builtin_fma(__builtin_fma(6.0 * kC6, z, 5.0 * kC7), z, 4.0 * kC8), z, 3.0 * kC9), z, 2.0 * kC10), z, kC11);
+    double rQ = rcp(Q);
+    double t = P * rQ;
+    double rt = rsq(t);
+    f = t * rt;
+    fp = (dP - t * dQ) * rQ * (0.5 * rt);
+  } else {
+    double rz = rsq(z);
+    f = 0.88622692545275801365 * rz;  // sqrt(pi)/2 / sqrt(z)
+    fp = -0.5 * f * (rz * rz);
+  }
+}
+
+// Bra-side data of one basis function, held in registers for a whole tile.
+template <int N>
+struct Bra {
+  double A[Dim<N>::NS], S[Dim<N>::N3], AS[Dim<N>::N3], i1, pk;
+};
+
+template <int N>
+__device__ __forceinline__ void load_bra(Bra<N> &b, const double *__restrict__ rec) {
+  using D = Dim<N>;
+#pragma unroll
+  for (int t = 0; t < D::NS; ++t) b.A[t] = rec[D::B_A + t];
+#pragma unroll
+  for (int t = 0; t < D::N3; ++t) {
+    b.S[t] = rec[D::B_S + t];
+    b.AS[t] = rec[D::B_AS + t];
+  }
+  b.i1 = rec[D::B_I1];
+  b.pk = rec[D::B_PK];
+}
+
+// Accumulators of one bra.  MODE 0 keeps the H and S parts apart (E not yet
+// known); MODE 1 folds them with the known energy: X = X_H - E X_S, etc.
+template <int N>
+struct Acc {
+  double XH[Dim<N>::NS], YH[Dim<N>::N3], XS[Dim<N>::NS], YS[Dim<N>::N3], sH, sS;
+  __device__ __forceinline__ void zero() {
+#pragma unroll
+    for (int t = 0; t < Dim<N>::NS; ++t) { XH[t] = 0.0; XS[t] = 0.0; }
+#pragma unroll
+    for (int t = 0; t < Dim<N>::N3; ++t) { YH[t] = 0.0; YS[t] = 0.0; }
+    sH = 0.0;
+    sS = 0.0;
+  }
+};
+
+// Separate element parts for the per-term query entry points.
+struct Parts {
+  double ov, tk, ert, zht;
+};
+
+enum { MODE_SPLIT = 0, MODE_COMBINED = 1, MODE_ELEMENTS = 2 };
+
+// One (bra, ket-under-term) contribution.  kd points at the ket-table record.
+// Adds sc*ov*alpha to hel and sc*ov to sel; for the gradient modes adds
+// wc*ov-weighted X/Y parts to acc (wc = scvv_P C_l; eshift = E in MODE 1).
+// Returns false if B is not positive definite.
+template <int N, int MODE>
+__device__ __forceinline__ bool pair_term(const Bra<N> &b, const double *__restrict__ kd,
+                                          const SysConst &sc_, double eshift, double &hel,
+                                          double &sel, Acc<N> &acc, Parts *parts) {
+  using D = Dim<N>;
+  constexpr int NS = D::NS;
+  const double *__restrict__ Ap = kd + D::K_AP;
+  const double *__restrict__ Sp = kd + D::K_SP;
+  const double *__restrict__ ApSp = kd + D::K_APSP;
+  const double *__restrict__ Np = kd + D::K_NP;
+  const double *__restrict__ ApM = kd + D::K_APM;
+  const double i2 = kd[D::K_I2], pl = kd[D::K_PL], wc = kd[D::K_WC], scv = kd[D::K_SC];
+
+  // Cholesky of B = A + A' (lower, packed), rd[j] = 1 / R_jj.
+  double R[NS], rd[N];
+  bool ok = true;
+#pragma unroll
+  for (int j = 0; j < N; ++j) {
+    double s = b.A[sx(j, j)] + Ap[sx(j, j)];
+#pragma unroll
+    for (int m = 0; m < j; ++m) s = __builtin_fma(-R[sx(j, m)], R[sx(j, m)], s);
+    ok = ok && (s > 0.0);
+    double r = rsq(s);
+    rd[j] = r;
+    R[sx(j, j)] = s * r;
+#pragma unroll
+    for (int i = j + 1; i < N; ++i) {
+      double t = b.A[sx(i, j)] + Ap[sx(i, j)];
+#pragma unroll
+      for (int m = 0; m < j; ++m) t = __builtin_fma(-R[sx(i, m)], R[sx(j, m)], t);
+      R[sx(i, j)] = t * r;
+    }
+  }
+  // R^-1 (lower, packed) and B^-1 = R^-T R^-1 (symmetric, packed).
+  double Ri[NS];
+#pragma unroll
+  for (int j = 0; j < N; ++j) {
+    Ri[sx(j, j)] = rd[j];
+#pragma unroll
+    for (int i = j + 1; i < N; ++i) {
+      double t = 0.0;
+#pragma unroll
+      for (int m = j; m < i; ++m) t = __builtin_fma(R[sx(i, m)], Ri[sx(m, j)], t);
+      Ri[sx(i, j)] = -t * rd[i];
+    }
+  }
+  double Bi[NS];
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = 0; j <= i; ++j) {
+      double t = 0.0;
+#pragma unroll
+      for (int m = i; m < N; ++m) t = __builtin_fma(Ri[sx(m, i)], Ri[sx(m, j)], t);
+      Bi[sx(i, j)] = t;
+    }
+  double det32 = 1.0;  // det(B)^-3/2
+#pragma unroll
+  for (int j = 0; j < N; ++j) det32 *= rd[j] * rd[j] * rd[j];
+
+  // E, st = B^-1 E, i3 (python/matel.py:125-133)
+  double Em[3 * N], st[3 * N];
+#pragma unroll
+  for (int t = 0; t < 3 * N; ++t) Em[t] = b.AS[t] + ApSp[t];
+  double i3 = 0.0;
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int x = 0; x < 3; ++x) {
+      double t = 0.0;
+#pragma unroll
+      for (int j = 0; j < N; ++j) t = __builtin_fma(Bi[sx(i, j)], Em[3 * j + x], t);
+      st[3 * i + x] = t;
+      i3 = __builtin_fma(Em[3 * i + x], t, i3);
+    }
+  const double ov = b.pk * pl * det32 * exp(i3 - b.i1 - i2);
+
+  // Kinetic (python/matel.py:144-155): i3k = <A dif1, M A' dif2>,
+  // T2 = 3 tr(M A B^-1 A') = 3 sum_ij (A'M A)_ij B^-1_ji.
+  double d1[3 * N], a1[3 * N], a2[3 * N];
+#pragma unroll
+  for (int t = 0; t < 3 * N; ++t) d1[t] = st[t] - b.S[t];
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int x = 0; x < 3; ++x) {
+      double u = 0.0, v = 0.0;
+#pragma unroll
+      for (int j = 0; j < N; ++j) {
+        u = __builtin_fma(b.A[sx(i, j)], d1[3 * j + x], u);
+        v = __builtin_fma(Ap[sx(i, j)], st[3 * j + x] - Sp[3 * j + x], v);
+      }
+      a1[3 * i + x] = u;
+      a2[3 * i + x] = v;
+    }
+  double i3k = 0.0;
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int x = 0; x < 3; ++x) {
+      double t = 0.0;  // (M a2)[i][x] = sum_j (A'M)_ji a2? no: M a2 = (A'M)^T dif2
+#pragma unroll
+      for (int j = 0; j < N; ++j) t = __builtin_fma(sc_.mass[i * N + j], a2[3 * j + x], t);
+      i3k = __builtin_fma(a1[3 * i + x], t, i3k);
+    }
+  double T2 = 0.0;
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      double t = 0.0;
+#pragma unroll
+      for (int m = 0; m < N; ++m) t = __builtin_fma(ApM[i * N + m], b.A[sx(m, j)], t);
+      T2 = __builtin_fma(t, Bi[sx(j, i)], T2);
+    }
+  T2 *= 3.0;
+  const double tfac = 2.0 * i3k + T2;
+
+  // Coulomb (python/matel.py:162-197; gradients python/gradel.py:266-337).
+  double csum = 0.0;  // sum_c c f r0
+  double La[NS], Lb[NS];
+  if (MODE != MODE_ELEMENTS) {
+#pragma unroll
+    for (int t = 0; t < NS; ++t) { La[t] = 0.0; Lb[t] = 0.0; }
+  }
+  {
+    int pi = 0;
+#pragma unroll
+    for (int p = 0; p <= N; ++p)
+#pragma unroll
+      for (int q = p + 1; q <= N; ++q, ++pi) {
+        const int a = p - 1, bq = q - 1;
+        double i2p, dx, dy, dz;
+        if (p == 0) {
+          i2p = Bi[sx(bq, bq)];
+          dx = st[3 * bq];
+          dy = st[3 * bq + 1];
+          dz = st[3 * bq + 2];
+        } else {
+          i2p = Bi[sx(a, a)] + Bi[sx(bq, bq)] - 2.0 * Bi[sx(a, bq)];
+          dx = st[3 * a] - st[3 * bq];
+          dy = st[3 * a + 1] - st[3 * bq + 1];
+          dz = st[3 * a + 2] - st[3 * bq + 2];
+        }
+        const double ri = rsq(i2p);           // 1/sqrt(i2p)
+        const double inv2 = ri * ri;           // 1/i2p
+        const double z = (dx * dx + dy * dy + dz * dz) * inv2;
+        double f, fp;
+        boys_f0(z, f, fp);
+        const double c = sc_.cpair[pi];
+        const double r0 = ri * 0.56418958354775628695;  // 1/sqrt(pi i2p)
+        csum = __builtin_fma(c * f, r0, csum);
+        if (MODE != MODE_ELEMENTS) {
+          const double ca = c * (2.0 * f + 4.0 * fp * z) * r0 * inv2;
+          const double cb = c * 4.0 * fp * r0 * inv2;
+          if (p == 0) {
+            La[sx(bq, bq)] += ca;
+            Lb[sx(bq, bq)] += cb;
+          } else {
+            La[sx(a, a)] += ca;
+            La[sx(bq, bq)] += ca;
+            La[sx(a, bq)] -= ca;
+            Lb[sx(a, a)] += cb;
+            Lb[sx(bq, bq)] += cb;
+            Lb[sx(a, bq)] -= cb;
+          }
+        }
+      }
+  }
+  // Stark (python/matel.py:205-208)
+  double zsum = 0.0;
+#pragma unroll
+  for (int i = 0; i < N; ++i) zsum = __builtin_fma(sc_.q[i + 1], st[3 * i + 2], zsum);
+  const double alpha = 2.0 * tfac + 2.0 * csum - sc_.efs * zsum;
+  hel = __builtin_fma(scv * ov, alpha, hel);
+  sel = __builtin_fma(scv, ov, sel);
+  if (parts) {
+    parts->ov = ov;
+    parts->tk = 2.0 * ov * tfac;
+    parts->ert = 2.0 * ov * csum;
+    parts->zht = ov * zsum * sc_.efs;
+  }
+  if (MODE == MODE_ELEMENTS) return ok;
+
+  // Gradient.  h = A dif1 - A' dif2; F^T = 4 A'M h + Lb st - efs q e_z^T.
+  double Ft[3 * N];
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int x = 0; x < 3; ++x) {
+      double t = 0.0;
+#pragma unroll
+      for (int m = 0; m < N; ++m) {
+        t = __builtin_fma(4.0 * ApM[i * N + m], a1[3 * m + x] - a2[3 * m + x], t);
+        t = __builtin_fma(Lb[sx(i, m)], st[3 * m + x], t);
+      }
+      if (x == 2) t = __builtin_fma(-sc_.efs, sc_.q[i + 1], t);
+      Ft[3 * i + x] = t;
+    }
+  // Phi (without the ov factor) = B^-1 F^T
+  double Phi[3 * N];
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int x = 0; x < 3; ++x) {
+      double t = 0.0;
+#pragma unroll
+      for (int m = 0; m < N; ++m) t = __builtin_fma(Bi[sx(i, m)], Ft[3 * m + x], t);
+      Phi[3 * i + x] = t;
+    }
+  // G = B^-1 (12 A'MA' + La) B^-1 (symmetric)
+  double Tm[N * N];
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      double t = 0.0;
+#pragma unroll
+      for (int m = 0; m < N; ++m)
+        t = __builtin_fma(__builtin_fma(12.0, Np[sx(i, m)], La[sx(i, m)]), Bi[sx(m, j)], t);
+      Tm[i * N + j] = t;
+    }
+  const double wo = wc * ov;
+  const double am = (MODE == MODE_COMBINED) ? (alpha - eshift) : alpha;
+  const double woa = wo * am;
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = 0; j <= i; ++j) {
+      double g = 0.0;
+#pragma unroll
+      for (int m = 0; m < N; ++m) g = __builtin_fma(Bi[sx(i, m)], Tm[m * N + j], g);
+      double dd = 0.0, dp = 0.0;
+#pragma unroll
+      for (int x = 0; x < 3; ++x) {
+        dd = __builtin_fma(d1[3 * i + x], d1[3 * j + x], dd);
+        dp = __builtin_fma(d1[3 * i + x], Phi[3 * j + x], dp);
+        dp = __builtin_fma(Phi[3 * i + x], d1[3 * j + x], dp);
+      }
+      const double x0 = __builtin_fma(-3.0, Bi[sx(i, j)], -2.0 * dd);
+      const int t = sx(i, j);
+      acc.XH[t] = __builtin_fma(woa, x0, __builtin_fma(wo, g - dp, acc.XH[t]));
+      if (MODE == MODE_SPLIT) acc.XS[t] = __builtin_fma(wo, x0, acc.XS[t]);
+    }
+#pragma unroll
+  for (int t = 0; t < 3 * N; ++t) {
+    acc.YH[t] = __builtin_fma(2.0 * woa, d1[t], __builtin_fma(wo, Phi[t], acc.YH[t]));
+    if (MODE == MODE_SPLIT) acc.YS[t] = __builtin_fma(2.0 * wo, d1[t], acc.YS[t]);
+  }
+  acc.sH += woa;
+  if (MODE == MODE_SPLIT) acc.sS += wo;
+  return ok;
+}
+
+// vech(X L) + 1.5 s diag(1/L_ii) for symmetric X (packed) and lower L (vech
+// order, column-major): the vech(L) block of one gradient.
+template <int N>
+__device__ __forceinline__ void grad_L(const double *X, double s, const double *Lv,
+                                       const double *Ld, double *out) {
+  // Lv is vech order: column j, rows j..N-1.
+  int k = 0;
+#pragma unroll
+  for (int j = 0; j < N; ++j) {
+    const int c0 = j * N - j * (j - 1) / 2;  // start of column j in vech order
+#pragma unroll
+    for (int i = j; i < N; ++i, ++k) {
+      double t = 0.0;
+#pragma unroll
+      for (int m = j; m < N; ++m) t = __builtin_fma(X[sx(i, m)], Lv[c0 + (m - j)], t);
+      if (i == j) t = __builtin_fma(1.5 * s, Ld[j], t);
+      out[k] = t;
+    }
+  }
+}
+
+// A Y for symmetric A (packed) and Y (n x 3): the shift block of one gradient.
+template <int N>
+__device__ __forceinline__ void grad_S(const double *A, const double *Y, double *out) {
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int x = 0; x < 3; ++x) {
+      double t = 0.0;
+#pragma unroll
+      for (int m = 0; m < N; ++m) t = __builtin_fma(A[sx(i, m)], Y[3 * m + x], t);
+      out[3 * i + x] = t;
+    }
+}
+
+}  // namespace necg

@@ -1,0 +1,231 @@
+"""Device engine: one system on one GPU behind the C ABI of ``libnecg_hip.so``.
+
+PyTorch provides the device memory and the stream; every kernel is the
+library's own.  Methods ending in ``_t`` take and return device tensors and
+are asynchronous on the current torch stream; the others take NumPy arrays,
+synchronise, and raise on any latched device error.
+"""
+
+from __future__ import annotations
+
+import ctypes as ct
+
+import numpy as np
+
+from . import _lib
+from .system import System
+
+
+def _torch():
+    import torch
+    return torch
+
+
+class Engine:
+    """FP64 energy-and-gradient evaluations of one system on one GPU."""
+
+    def __init__(self, system: System, device=None):
+        torch = _torch()
+        if not torch.cuda.is_available():
+            raise RuntimeError("nbo_correlatedgaussians_amd needs a HIP GPU; none is visible")
+        system.validate()
+        self.system = system
+        self.device = torch.device(device if device is not None else "cuda:0")
+        if self.device.type != "cuda":
+            raise ValueError("device must be a GPU device")
+        self.index = self.device.index if self.device.index is not None else torch.cuda.current_device()
+        self.device = torch.device("cuda", self.index)
+        self.lib = _lib.load()
+        self._host = [np.ascontiguousarray(a, dtype=np.float64) for a in
+                      (system.mass, system.ncv, system.scvv, system.sml, system.sms)]
+        m, q, c, l, s = self._host
+        self._desc = _lib.NecgSystem(system.n, system.nst, float(system.efs), m.ctypes.data,
+                                     q.ctypes.data, c.ctypes.data, l.ctypes.data, s.ctypes.data)
+        ctx = ct.c_void_p()
+        with torch.cuda.device(self.index):
+            torch.cuda.init()
+            rc = self.lib.necg_create(ct.byref(self._desc), self.index, ct.byref(ctx))
+        _lib.raise_for(rc, "necg_create")
+        self._ctx = ctx
+        self.blk = system.blk
+
+    # -- plumbing ----------------------------------------------------------
+    def close(self):
+        if getattr(self, "_ctx", None) and self._ctx.value:
+            self.lib.necg_destroy(self._ctx)
+            self._ctx = ct.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _stream(self):
+        torch = _torch()
+        return ct.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+
+    def _dev(self, a, dtype=None):
+        torch = _torch()
+        dtype = dtype or torch.float64
+        if isinstance(a, torch.Tensor):
+            t = a.to(device=self.device, dtype=dtype)
+        else:
+            t = torch.as_tensor(np.ascontiguousarray(a), dtype=dtype, device=self.device)
+        return t.contiguous()
+
+    def _empty(self, *shape):
+        torch = _torch()
+        return torch.empty(*shape, dtype=torch.float64, device=self.device)
+
+    def K_of(self, nx: int) -> int:
+        if nx % (self.blk + 1):
+            raise ValueError(f"x has length {nx}, not a multiple of blk+1 = {self.blk + 1}")
+        return nx // (self.blk + 1)
+
+    def check(self, where: str = "") -> None:
+        """Synchronise the current stream and raise on a latched device error."""
+        _lib.raise_for(self.lib.necg_status(self._ctx, self._stream()), where)
+
+    def set_tiling(self, ket_tile: int = 0, term_split: int = 0, waves: int = 0) -> None:
+        _lib.raise_for(self.lib.necg_set_tiling(self._ctx, ket_tile, term_split, waves),
+                       "necg_set_tiling")
+
+    def tiling(self):
+        a, b, c = ct.c_int32(), ct.c_int32(), ct.c_int32()
+        self.lib.necg_get_tiling(self._ctx, ct.byref(a), ct.byref(b), ct.byref(c))
+        return a.value, b.value, c.value
+
+    def timing(self, mode: int):
+        """Pair-kernel HIP-event timing: 1 start, 0 stop, 2 read ->
+        (milliseconds, launches)."""
+        ms, n = ct.c_double(), ct.c_int64()
+        _lib.raise_for(self.lib.necg_timing(self._ctx, mode, ct.byref(ms), ct.byref(n)),
+                       "necg_timing")
+        return ms.value, n.value
+
+    @staticmethod
+    def _ptr(t):
+        return ct.c_void_p(t.data_ptr()) if t is not None else None
+
+    # -- asynchronous tensor API -------------------------------------------
+    def outprod_t(self, x, f=None, g=None):
+        """(E, g) as device tensors for a device x (python/outprod.py:20)."""
+        x = self._dev(x)
+        K = self.K_of(x.numel())
+        f = self._empty(1) if f is None else f
+        g = self._empty(x.numel()) if g is None else g
+        rc = self.lib.necg_outprod(self._ctx, K, self._ptr(x), self._ptr(f), self._ptr(g),
+                                   self._stream())
+        _lib.raise_for(rc, "necg_outprod")
+        return f, g
+
+    def energy_t(self, x, with_hs: bool = False):
+        x = self._dev(x)
+        K = self.K_of(x.numel())
+        f = self._empty(1)
+        H = self._empty(K, K) if with_hs else None
+        S = self._empty(K, K) if with_hs else None
+        rc = self.lib.necg_energy(self._ctx, K, self._ptr(x), self._ptr(f), self._ptr(H),
+                                  self._ptr(S), self._stream())
+        _lib.raise_for(rc, "necg_energy")
+        return (f, H, S) if with_hs else f
+
+    def vjp_t(self, x, gbar=None):
+        x = self._dev(x)
+        K = self.K_of(x.numel())
+        gx = self._empty(x.numel())
+        gb = None if gbar is None else self._dev(gbar).reshape(1)
+        rc = self.lib.necg_energy_vjp(self._ctx, K, self._ptr(x), self._ptr(gb), self._ptr(gx),
+                                      self._stream())
+        _lib.raise_for(rc, "necg_energy_vjp")
+        return gx
+
+    def matrices_t(self, iv, K: int):
+        iv = self._dev(iv)
+        if iv.numel() != K * self.blk:
+            raise ValueError("iv must hold K*blk numbers")
+        H = self._empty(K, K)
+        S = self._empty(K, K)
+        rc = self.lib.necg_matrices(self._ctx, K, self._ptr(iv), self._ptr(H), self._ptr(S),
+                                    self._stream())
+        _lib.raise_for(rc, "necg_matrices")
+        return H, S
+
+    # -- synchronous NumPy API ---------------------------------------------
+    @staticmethod
+    def _host_x(x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        if x.ndim != 1:
+            raise ValueError("x must be one-dimensional")
+        if not np.all(np.isfinite(x)):
+            raise FloatingPointError("x contains NaN or Inf")
+        return x
+
+    def outprod(self, x):
+        """Drop-in for the reference ``outprod(x) -> (f, g)``
+        (python/outprod.py:20-76): E = C^T H C / C^T S C and its gradient."""
+        x = self._host_x(x)
+        f, g = self.outprod_t(x)
+        self.check("outprod")
+        return float(f.item()), g.cpu().numpy()
+
+    def energy(self, x, with_hs: bool = False):
+        x = self._host_x(x)
+        out = self.energy_t(x, with_hs)
+        self.check("energy")
+        if with_hs:
+            f, H, S = out
+            return float(f.item()), H.cpu().numpy(), S.cpu().numpy()
+        return float(out.item())
+
+    def matrices(self, iv, K: int):
+        iv = self._host_x(iv)
+        H, S = self.matrices_t(iv, K)
+        self.check("matrices")
+        return H.cpu().numpy(), S.cpu().numpy()
+
+    def _queries(self, iv, K, queries):
+        torch = _torch()
+        q = np.ascontiguousarray(np.asarray(queries, dtype=np.int32).reshape(-1, 3))
+        if q.size and (q[:, 0].min() < 0 or q[:, 0].max() >= self.system.nst or
+                       q[:, 1:].min() < 0 or q[:, 1:].max() >= K):
+            raise IndexError("query (term, bra, ket) out of range")
+        iv = self._dev(self._host_x(iv))
+        if iv.numel() != K * self.blk:
+            raise ValueError("iv must hold K*blk numbers")
+        return iv, torch.as_tensor(q, device=self.device), q.shape[0]
+
+    def matel(self, iv, K: int, queries):
+        """Rows (ov, tk, ert, zht) for 0-based (term, bra, ket) queries
+        (python/matel.py:28)."""
+        ivd, qd, nq = self._queries(iv, K, queries)
+        out = self._empty(max(nq, 1), 4)
+        rc = self.lib.necg_matel(self._ctx, K, self._ptr(ivd), nq, self._ptr(qd), self._ptr(out),
+                                 self._stream())
+        _lib.raise_for(rc, "necg_matel")
+        self.check("matel")
+        return out[:nq].cpu().numpy()
+
+    def gradel(self, iv, K: int, queries):
+        """(tgvl, tgvls) rows for 0-based (term, bra, ket) queries
+        (python/gradel.py:27)."""
+        ivd, qd, nq = self._queries(iv, K, queries)
+        a = self._empty(max(nq, 1), self.blk)
+        b = self._empty(max(nq, 1), self.blk)
+        rc = self.lib.necg_gradel(self._ctx, K, self._ptr(ivd), nq, self._ptr(qd), self._ptr(a),
+                                  self._ptr(b), self._stream())
+        _lib.raise_for(rc, "necg_gradel")
+        self.check("gradel")
+        return a[:nq].cpu().numpy(), b[:nq].cpu().numpy()
+
+    def dhds(self, iv, K: int):
+        """DH, DS of shape (K*blk, K) (python/make_grad.py:34-55)."""
+        ivd = self._dev(self._host_x(iv))
+        DH = self._empty(K * self.blk, K)
+        DS = self._empty(K * self.blk, K)
+        rc = self.lib.necg_dhds(self._ctx, K, self._ptr(ivd), self._ptr(DH), self._ptr(DS),
+                                self._stream())
+        _lib.raise_for(rc, "necg_dhds")
+        self.check("dhds")
+        return DH.cpu().numpy(), DS.cpu().numpy()

@@ -1,0 +1,207 @@
+"""Parity of the HIP path with the reference (golden vectors) and the oracle.
+
+Tolerances (FP64 throughout): |dE| <= 1e-12 max(|E|, 1) and
+max|dg| <= 1e-9 max(max|g|, 1); element matrices 1e-12 of their largest
+entry; per-term gradients 1e-10 of their largest entry."""
+
+import hashlib
+import math
+
+import numpy as np
+import pytest
+
+import nbo_correlatedgaussians_amd as necg
+from conftest import FULL_CASES, golden, system_by_name
+
+pytestmark = pytest.mark.gpu
+
+RTOL_E = 1e-12
+RTOL_G = 1e-9
+
+_engines = {}
+
+
+def engine_for(name):
+    from nbo_correlatedgaussians_amd.engine import Engine
+    if name not in _engines:
+        _engines[name] = Engine(system_by_name(name), "cuda:0")
+    return _engines[name]
+
+
+def close_E(E, Eref):
+    return abs(E - Eref) <= RTOL_E * max(abs(Eref), 1.0)
+
+
+def close_g(g, gref):
+    return np.max(np.abs(g - gref)) <= RTOL_G * max(np.max(np.abs(gref)), 1.0)
+
+
+@pytest.mark.parametrize("case", FULL_CASES)
+def test_outprod_vs_reference_golden(case):
+    d = golden(case)
+    eng = engine_for(str(d["system"]))
+    print(f"case {case}", flush=True)
+    E, g = eng.outprod(d["x"])
+    assert close_E(E, float(d["E"])), (E, float(d["E"]))
+    assert close_g(g, d["g"])
+
+
+@pytest.mark.parametrize("case", FULL_CASES)
+def test_energy_and_matrices_vs_reference_golden(case):
+    d = golden(case)
+    eng = engine_for(str(d["system"]))
+    E, H, S = eng.energy(d["x"], with_hs=True)
+    assert close_E(E, float(d["E"]))
+    for a, b in ((H, d["H"]), (S, d["S"])):
+        assert np.max(np.abs(a - b)) <= 1e-12 * np.max(np.abs(b))
+        np.testing.assert_array_equal(a, a.T)
+
+
+@pytest.mark.parametrize("case", FULL_CASES)
+def test_matel_gradel_terms_vs_reference_golden(case):
+    d = golden(case)
+    s = system_by_name(d["system"])
+    eng = engine_for(str(d["system"]))
+    K = len(d["x"]) // (s.blk + 1)
+    iv = d["x"][:K * s.blk]
+    q = d["queries"] - 1
+    m = eng.matel(iv, K, q)
+    scale = np.max(np.abs(d["matel"]), axis=0)
+    scale[scale == 0] = 1.0
+    assert np.all(np.max(np.abs(m - d["matel"]), axis=0) <= 1e-12 * scale)
+    a, b = eng.gradel(iv, K, q)
+    assert np.max(np.abs(a - d["tgvl"])) <= 1e-10 * np.max(np.abs(d["tgvl"]))
+    assert np.max(np.abs(b - d["tgvls"])) <= 1e-10 * np.max(np.abs(d["tgvls"]))
+
+
+def _sha(x):
+    return hashlib.sha256(np.ascontiguousarray(x, dtype="<f8").tobytes()).hexdigest()
+
+
+@pytest.mark.parametrize("case", ["lih_k256_s0_pairs", "lih_k2048_s1_pairs"])
+def test_pair_sums_vs_reference_golden(case):
+    """H_ij, S_ij and the DH/DS blocks of sampled pairs at K = 256 / 2048,
+    summed over the 24 terms from the per-term entry points."""
+    d = golden(case)
+    s = system_by_name(d["system"])
+    K = int(d["K"])
+    iv, C = necg.generate_parameters(s, K, int(d["seed"]))
+    assert _sha(necg.pack_x(iv, C)) == str(d["x_sha256"])
+    eng = engine_for(str(d["system"]))
+    pairs = d["pairs"] - 1
+    P = len(pairs)
+    q, qe = [], []
+    for i, j in pairs:
+        for p in range(s.nst):
+            q.append((p, i, j))
+            qe.append((p, max(i, j), min(i, j)))
+    m = eng.matel(iv, K, qe).reshape(P, s.nst, 4)
+    a, b = eng.gradel(iv, K, q)
+    sc = s.scvv
+    H = np.einsum("p,tp->t", sc, m[:, :, 1] + m[:, :, 2] - m[:, :, 3])
+    S = np.einsum("p,tp->t", sc, m[:, :, 0])
+    dh = np.einsum("p,tpr->tr", sc, a.reshape(P, s.nst, -1))
+    ds = np.einsum("p,tpr->tr", sc, b.reshape(P, s.nst, -1))
+    assert np.max(np.abs(H - d["H"])) <= 1e-12 * np.max(np.abs(d["H"]))
+    assert np.max(np.abs(S - d["S"])) <= 1e-12 * np.max(np.abs(d["S"]))
+    assert np.max(np.abs(dh - d["DH"])) <= 1e-10 * np.max(np.abs(d["DH"]))
+    assert np.max(np.abs(ds - d["DS"])) <= 1e-10 * np.max(np.abs(d["DS"]))
+
+
+@pytest.mark.parametrize("name,K,seed", [("LiH", 40, 3), ("He", 64, 4), ("H2", 64, 5),
+                                         ("LiH", 130, 6)])
+def test_outprod_vs_oracle(oracle_lib, name, K, seed):
+    s = system_by_name(name)
+    iv, C = necg.generate_parameters(s, K, seed)
+    x = necg.pack_x(iv, C)
+    print(f"{name} K={K} seed={seed}", flush=True)
+    E, g = engine_for(name).outprod(x)
+    Eo, go = oracle_lib.Oracle(s).outprod(x)
+    assert close_E(E, Eo), (E, Eo)
+    assert close_g(g, go)
+
+
+def test_dhds_vs_oracle(oracle_lib):
+    s = necg.lih_system()
+    K = 9
+    iv, C = necg.generate_parameters(s, K, 8)
+    DH, DS = engine_for("LiH").dhds(iv, K)
+    o = oracle_lib.Oracle(s)
+    pairs = [(l, k) for l in range(K) for k in range(K)]
+    _, _, dh, ds = o.pair_sums(iv, pairs)
+    DHo = np.zeros_like(DH)
+    DSo = np.zeros_like(DS)
+    for t, (l, k) in enumerate(pairs):
+        DHo[l * s.blk:(l + 1) * s.blk, k] = dh[t]
+        DSo[l * s.blk:(l + 1) * s.blk, k] = ds[t]
+    assert np.max(np.abs(DH - DHo)) <= 1e-11 * np.max(np.abs(DHo))
+    assert np.max(np.abs(DS - DSo)) <= 1e-11 * np.max(np.abs(DSo))
+
+
+def test_k256_full_vs_oracle_and_fd(oracle_lib):
+    """Config: LiH K = 256, seed 0; gradient against the oracle and central
+    finite differences (h = 1e-6) on 64 seeded parameters."""
+    s = necg.lih_system()
+    K = 256
+    iv, C = necg.generate_parameters(s, K, 0)
+    x = necg.pack_x(iv, C)
+    eng = engine_for("LiH")
+    E, g = eng.outprod(x)
+    Eo, go = oracle_lib.Oracle(s).outprod(x, nthreads=16)
+    assert close_E(E, Eo), (E, Eo)
+    assert close_g(g, go)
+    rng = np.random.default_rng(0)
+    idx = rng.choice(x.size, 64, replace=False)
+    gmax = np.max(np.abs(g))
+    worst = 0.0
+    for i in idx:
+        h = 1e-6 * max(1.0, abs(x[i]))
+        xp, xm = x.copy(), x.copy()
+        xp[i] += h
+        xm[i] -= h
+        fd = (eng.energy(xp) - eng.energy(xm)) / (2 * h)
+        worst = max(worst, abs(fd - g[i]) / max(abs(g[i]), 1e-3 * gmax))
+    assert worst < 1e-5, worst
+
+
+def test_k2048_properties():
+    """Config: LiH K = 2048, seed 1.  Size-independent checks: the split
+    (outprod) and two-pass (energy + vjp) paths agree, C . g_C = 0, and
+    scaling C leaves E and g_theta unchanged."""
+    s = necg.lih_system()
+    K = 2048
+    iv, C = necg.generate_parameters(s, K, 1)
+    x = necg.pack_x(iv, C)
+    eng = engine_for("LiH")
+    E, g = eng.outprod(x)
+    assert math.isfinite(E) and np.all(np.isfinite(g))
+    E2 = eng.energy(x)
+    assert abs(E2 - E) <= 1e-13 * max(abs(E), 1.0)
+    gx = eng.vjp_t(x).cpu().numpy()
+    eng.check()
+    assert close_g(gx, g)
+    gC = g[K * s.blk:]
+    assert abs(C @ gC) <= 1e-11 * (np.abs(C) @ np.abs(gC))
+    E3, g3 = eng.outprod(necg.pack_x(iv, 2.0 * C))
+    assert abs(E3 - E) <= 1e-12 * max(abs(E), 1.0)
+    assert close_g(g3[:K * s.blk], g[:K * s.blk])
+
+
+def test_error_paths():
+    from nbo_correlatedgaussians_amd import _lib
+    s = necg.he_system()
+    eng = engine_for("He")
+    iv, C = necg.generate_parameters(s, 4, 0)
+    bad = iv.copy()
+    bad[0] = 0.0          # L_11 of function 1 = 0: A_1 singular
+    with pytest.raises(_lib.NotPositiveDefinite):
+        eng.outprod(necg.pack_x(bad, C))
+    with pytest.raises(ZeroDivisionError):
+        eng.outprod(necg.pack_x(iv, np.zeros(4)))
+    x = necg.pack_x(iv, C)
+    x[3] = np.nan
+    with pytest.raises(FloatingPointError):
+        eng.outprod(x)
+    # the latch is cleared: a good call afterwards succeeds
+    E, g = eng.outprod(necg.pack_x(iv, C))
+    assert math.isfinite(E)
