@@ -30,7 +30,9 @@ using namespace necg;
 
 namespace {
 
-enum FlagBits { F_NOT_PD = 1, F_NONFINITE = 2, F_CSC = 4 };
+// latched device errors; necg_status reports the root cause first:
+// bad input, then a singular pair matrix, then C^T S C <= 0, then NaN results
+enum FlagBits { F_NOT_PD = 1, F_NONFINITE = 2, F_CSC = 4, F_BAD_INPUT = 8 };
 
 // ------------------------------------------------------------------ prep ----
 
@@ -105,7 +107,7 @@ __global__ void prep_bra(const double *__restrict__ x, int K, int with_c,
     bad = bad || !isfinite(c);
   }
   r[D::B_C] = c;
-  if (bad) atomicOr(flag, (int)F_NONFINITE);
+  if (bad) atomicOr(flag, (int)F_BAD_INPUT);
   if (!pd) atomicOr(flag, (int)F_NOT_PD);
 }
 
@@ -465,7 +467,7 @@ __global__ void query_kernel(const double *__restrict__ bra, int K, int nst,
   if (t >= nq) return;
   const int p = q[3 * t], i = q[3 * t + 1], j = q[3 * t + 2];
   if (p < 0 || p >= nst || i < 0 || i >= K || j < 0 || j >= K) {
-    atomicOr(flag, (int)F_NONFINITE);
+    atomicOr(flag, (int)F_BAD_INPUT);
     return;
   }
   double kr[D::KS];
@@ -1001,9 +1003,10 @@ int necg_status(necg_ctx *c, void *stream) {
   int flags = 0;
   HIP_TRY(hipMemcpy(&flags, c->d_flag, sizeof(int), hipMemcpyDeviceToHost));
   if (flags) HIP_TRY(hipMemset(c->d_flag, 0, sizeof(int)));
-  if (flags & F_NONFINITE) return NECG_ERR_NONFINITE;
+  if (flags & F_BAD_INPUT) return NECG_ERR_NONFINITE;
   if (flags & F_NOT_PD) return NECG_ERR_NOT_PD;
   if (flags & F_CSC) return NECG_ERR_CSC;
+  if (flags & F_NONFINITE) return NECG_ERR_NONFINITE;
   return NECG_OK;
 }
 
