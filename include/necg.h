@@ -101,8 +101,9 @@ int necg_dhds(necg_ctx *ctx, int32_t K, const double *iv, double *DH, double *DS
  * clears the latch.  Blocking. */
 int necg_status(necg_ctx *ctx, void *stream);
 
-/* Launch shape of the pair kernels: kets per work item, term groups, waves per
- * workgroup; 0 keeps the built-in choice.  Used by the autotuner. */
+/* Launch shape of the pair kernels: kets per workgroup and term groups (0 keeps
+ * the built-in choice).  The waves per workgroup are fixed per n (4 for n <= 5,
+ * 2 for n = 6): `waves` must be 0 or that value.  Used by the autotuner. */
 int necg_set_tiling(necg_ctx *ctx, int32_t ket_tile, int32_t term_split, int32_t waves);
 
 /* Launch shape that the last pair-kernel launch used (for reports). */

@@ -9,7 +9,8 @@ import ctypes as ct
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libnecg_hip.so")
+# NECG_LIB_PATH selects another build of the same library (e.g. a tuning variant)
+LIB_PATH = os.environ.get("NECG_LIB_PATH") or os.path.join(HERE, "libnecg_hip.so")
 
 # status codes (include/necg.h)
 NECG_OK = 0

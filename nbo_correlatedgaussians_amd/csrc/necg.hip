@@ -207,102 +207,145 @@ __global__ void prep_ket(const double *__restrict__ bra, int K, int nst,
 // part layout: part[((tile * NACC) + v) * K + k], tile = y * TS + z.
 // hpart/spart layout: [z][l][k] for bra k >= ket l.
 
+// Waves per pair-kernel workgroup: they share the 64 bra records in LDS and
+// each owns a private set of accumulator slots, so the sums are formed in a
+// fixed order (bitwise reproducible for a given launch shape).  LDS per
+// workgroup: (B_PK+1)*512 B + W*NACC*512 B = 151 KB for n = 5, W = 4.
+__host__ __device__ constexpr int pair_waves(int n) { return n <= 5 ? 4 : 2; }
+
+// One wave per SIMD: the term needs ~300 live FP64 values; at this
+// occupancy the compiler keeps the overflow in AGPRs instead of scratch.
+#ifndef NECG_MIN_WAVES
+#define NECG_MIN_WAVES 1
+#endif
+
 template <int N, int MODE>
-__global__ void __launch_bounds__(256) pair_kernel(
+__global__ void __launch_bounds__(64 * pair_waves(N), NECG_MIN_WAVES) pair_kernel(
     const double *__restrict__ bra, const double *__restrict__ ket, int K, int nst, int kt,
     int tps, SysConst sc, const double *__restrict__ scal, double *__restrict__ part,
     double *__restrict__ hpart, double *__restrict__ spart, int *__restrict__ flag) {
   using D = Dim<N>;
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  constexpr int W = pair_waves(N);
+  constexpr int NA = (MODE == MODE_ELEMENTS) ? 1 : D::NACC * 64;
+  // the 64 bras of this workgroup, [field][lane], and per-wave accumulators
+  // [wave][slot][lane]; the waves take alternate kets of the tile
+  __shared__ double bra_s[(D::B_PK + 1) * 64];
+  __shared__ double acc_s[W * NA];
+  const int lane = threadIdx.x & 63;
+  // wave index: uniform across the wave, so that the ket index and every
+  // ket-record address are scalar (s_load into SGPRs)
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int k = blockIdx.x * 64 + lane;
   const int l0 = blockIdx.y * kt;
   const int l1 = min(K, l0 + kt);
   const int p0 = blockIdx.z * tps;
   const int p1 = min(nst, p0 + tps);
   const bool active = k < K;
   const int kk = active ? k : K - 1;
-  Bra<N> b;
-  load_bra<N>(b, bra + (size_t)kk * D::BS);
+  for (int v = wv; v <= D::B_PK; v += W) bra_s[v * 64 + lane] = bra[(size_t)kk * D::BS + v];
+  if (MODE != MODE_ELEMENTS)
+    for (int v = 0; v < D::NACC; ++v) acc_s[wv * NA + v * 64 + lane] = 0.0;
+  __syncthreads();
+  const BraLds<N> b{bra_s + lane};
   const double eshift = (MODE == MODE_COMBINED) ? scal[0] : 0.0;
-  Acc<N> acc;
-  acc.zero();
+  LdsSink sink{acc_s + wv * NA + lane};
   bool ok = true;
-  for (int l = l0; l < l1; ++l) {
+  for (int l = l0 + wv; l < l1; l += W) {
     double hel = 0.0, sel = 0.0;
     const double *kl = ket + (size_t)l * nst * D::KS;
-    for (int p = p0; p < p1; ++p)
-      ok &= pair_term<N, MODE>(b, kl + (size_t)p * D::KS, sc, eshift, hel, sel, acc, nullptr);
+    for (int p = p0; p < p1; ++p) {
+      // keep the bra in LDS: re-read it for every term instead of holding
+      // 47 doubles in registers across the loop
+      __asm__ volatile("" ::: "memory");
+      ok &= pair_term<N, MODE>(b, kl + (size_t)p * D::KS, sc, eshift, hel, sel, sink, nullptr);
+    }
     if (MODE != MODE_COMBINED && active && k >= l) {
       const size_t o = ((size_t)blockIdx.z * K + l) * K + k;
       hpart[o] = hel;
       spart[o] = sel;
     }
   }
-  if (!active) return;
-  if (!ok) atomicOr(flag, (int)F_NOT_PD);
+  if (active && !ok) atomicOr(flag, (int)F_NOT_PD);
   if (MODE == MODE_ELEMENTS) return;
+  __syncthreads();
+  if (!active) return;
   const size_t tile = (size_t)blockIdx.y * gridDim.z + blockIdx.z;
   double *o = part + tile * D::NACC * K + k;
+  for (int v = wv; v < D::NACC; v += W) {
+    if (MODE == MODE_COMBINED && v >= D::A_XS && v != D::A_SH) continue;
+    double t = 0.0;
 #pragma unroll
-  for (int t = 0; t < D::NS; ++t) o[(size_t)(D::A_XH + t) * K] = acc.XH[t];
-#pragma unroll
-  for (int t = 0; t < D::N3; ++t) o[(size_t)(D::A_YH + t) * K] = acc.YH[t];
-  o[(size_t)D::A_SH * K] = acc.sH;
-  if (MODE == MODE_SPLIT) {
-#pragma unroll
-    for (int t = 0; t < D::NS; ++t) o[(size_t)(D::A_XS + t) * K] = acc.XS[t];
-#pragma unroll
-    for (int t = 0; t < D::N3; ++t) o[(size_t)(D::A_YS + t) * K] = acc.YS[t];
-    o[(size_t)D::A_SS * K] = acc.sS;
+    for (int w = 0; w < W; ++w) t += acc_s[w * NA + v * 64 + lane];
+    o[(size_t)v * K] = t;
   }
+}
+
+// Sum of a value over the 256 lanes of a workgroup (4 waves): wave
+// reduction by shuffles, then one LDS slot per wave.
+__device__ __forceinline__ double block_sum256(double v, double *red4) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  if ((threadIdx.x & 63) == 0) red4[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x == 0) t = (red4[0] + red4[1]) + (red4[2] + red4[3]);
+  __syncthreads();
+  return t;
 }
 
 // Sum term groups, mirror the lower triangle (bra i >= ket j) into full
-// symmetric H and S (python/outprod.py:57-60).  One lane per (i, j).
-__global__ void hs_finish(const double *__restrict__ hpart, const double *__restrict__ spart,
-                          int K, int ts, double *__restrict__ H, double *__restrict__ S) {
+// symmetric H and S (python/outprod.py:57-60), and form the partial row sums
+// of H C and S C of this 256-column chunk.  Grid (ceil(K/256), K), 256 lanes.
+__global__ void __launch_bounds__(256) hs_finish(
+    const double *__restrict__ hpart, const double *__restrict__ spart, int K, int ts,
+    const double *__restrict__ bra, int bs, int bc, double *__restrict__ H,
+    double *__restrict__ S, double *__restrict__ hcp, double *__restrict__ scp) {
+  __shared__ double red[8];
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   const int i = blockIdx.y;
-  if (j >= K) return;
-  const int lo = min(i, j), hi = max(i, j);
-  double h = 0.0, s = 0.0;
-  for (int z = 0; z < ts; ++z) {
-    const size_t o = ((size_t)z * K + lo) * K + hi;
-    h += hpart[o];
-    s += spart[o];
-  }
-  H[(size_t)i * K + j] = h;
-  S[(size_t)i * K + j] = s;
-}
-
-// HC = H C, SC = S C (lane per column, coalesced over the symmetric H),
-// then C^T H C, C^T S C, E and 1/C^T S C in one workgroup.
-__global__ void hc_sc(const double *__restrict__ H, const double *__restrict__ S,
-                      const double *__restrict__ bra, int bs, int bc, int K,
-                      double *__restrict__ HC, double *__restrict__ SC) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= K) return;
-  double h = 0.0, s = 0.0;
-  for (int j = 0; j < K; ++j) {
+  double hc = 0.0, sc = 0.0;
+  if (j < K) {
+    const int lo = min(i, j), hi = max(i, j);
+    double h = 0.0, s = 0.0;
+    for (int z = 0; z < ts; ++z) {
+      const size_t o = ((size_t)z * K + lo) * K + hi;
+      h += hpart[o];
+      s += spart[o];
+    }
+    H[(size_t)i * K + j] = h;
+    S[(size_t)i * K + j] = s;
     const double c = bra[(size_t)j * bs + bc];
-    h = __builtin_fma(H[(size_t)j * K + i], c, h);
-    s = __builtin_fma(S[(size_t)j * K + i], c, s);
+    hc = h * c;
+    sc = s * c;
   }
-  HC[i] = h;
-  SC[i] = s;
+  hc = block_sum256(hc, red);
+  sc = block_sum256(sc, red + 4);
+  if (threadIdx.x == 0) {
+    hcp[(size_t)blockIdx.x * K + i] = hc;
+    scp[(size_t)blockIdx.x * K + i] = sc;
+  }
 }
 
-__global__ void __launch_bounds__(1024) quotient(const double *__restrict__ HC,
-                                                 const double *__restrict__ SC,
-                                                 const double *__restrict__ bra, int bs, int bc,
-                                                 int K, double *__restrict__ scal,
-                                                 double *__restrict__ fout,
-                                                 int *__restrict__ flag) {
+// HC, SC from the chunk partials, then C^T H C, C^T S C, E and 1/C^T S C
+// (python/outprod.py:65-68) in one workgroup of 1024 lanes.
+__global__ void __launch_bounds__(1024) quotient(
+    const double *__restrict__ hcp, const double *__restrict__ scp, int nchunk,
+    const double *__restrict__ bra, int bs, int bc, int K, double *__restrict__ HC,
+    double *__restrict__ SC, double *__restrict__ scal, double *__restrict__ fout,
+    int *__restrict__ flag) {
   __shared__ double rh[1024], rs[1024];
   double h = 0.0, s = 0.0;
   for (int i = threadIdx.x; i < K; i += blockDim.x) {
-    const double c = bra[(size_t)i * bs + bc];
-    h = __builtin_fma(c, HC[i], h);
-    s = __builtin_fma(c, SC[i], s);
+    double a = 0.0, b = 0.0;
+    for (int c = 0; c < nchunk; ++c) {
+      a += hcp[(size_t)c * K + i];
+      b += scp[(size_t)c * K + i];
+    }
+    HC[i] = a;
+    SC[i] = b;
+    const double ci = bra[(size_t)i * bs + bc];
+    h = __builtin_fma(ci, a, h);
+    s = __builtin_fma(ci, b, s);
   }
   rh[threadIdx.x] = h;
   rs[threadIdx.x] = s;
@@ -327,13 +370,25 @@ __global__ void __launch_bounds__(1024) quotient(const double *__restrict__ HC,
   }
 }
 
-// Lane per bra k: sum the tile partials, transform to the blk-vector, and
-// contract: g_theta = norm (a - E b) (MODE 0) or norm a (MODE 1, E folded),
-// with a = 2 C_k [vech(X_H L) + 1.5 sigma_H diag(1/L) | A Y_H]
+// Sum of the tile partials of every accumulator slot: red[v*K + k].
+// Grid (ceil(K/256), nv), 256 lanes; coalesced over k.
+__global__ void __launch_bounds__(256) tile_reduce(const double *__restrict__ part, int ntiles,
+                                                   int K, int nacc, double *__restrict__ red) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int v = blockIdx.y;
+  if (k >= K) return;
+  double a = 0.0;
+  for (int t = 0; t < ntiles; ++t) a += part[((size_t)t * nacc + v) * K + k];
+  red[(size_t)v * K + k] = a;
+}
+
+// Lane per bra k: transform the summed accumulators and contract,
+// g_theta = norm (a - E b) (MODE 0) or norm a (MODE 1, E folded), with
+// a = 2 C_k [vech(X_H L) + 1.5 sigma_H diag(1/L) | A Y_H]
 // (python/make_grad.py:61-79); g_C = 2 norm (HC - E SC) (:85).
 // gbar (nullable) scales the whole gradient (vector-Jacobian product).
 template <int N, int MODE>
-__global__ void grad_final(const double *__restrict__ part, int ntiles, int K,
+__global__ void grad_final(const double *__restrict__ red, int K,
                            const double *__restrict__ bra, const double *__restrict__ scal,
                            const double *__restrict__ HC, const double *__restrict__ SC,
                            const double *__restrict__ gbar, double *__restrict__ g) {
@@ -342,14 +397,8 @@ __global__ void grad_final(const double *__restrict__ part, int ntiles, int K,
   if (k >= K) return;
   double a[D::NACC];
 #pragma unroll
-  for (int v = 0; v < D::NACC; ++v) a[v] = 0.0;
-  const int nv = (MODE == MODE_SPLIT) ? D::NACC : D::A_XS;
-  for (int t = 0; t < ntiles; ++t) {
-    const double *p = part + (size_t)t * D::NACC * K + k;
-#pragma unroll
-    for (int v = 0; v < D::NACC; ++v)
-      if (v < nv || v == D::A_SH) a[v] += p[(size_t)v * K];
-  }
+  for (int v = 0; v < D::NACC; ++v)
+    a[v] = (MODE == MODE_SPLIT || v < D::A_XS || v == D::A_SH) ? red[(size_t)v * K + k] : 0.0;
   const double *r = bra + (size_t)k * D::BS;
   const double E = scal[0], nrm = scal[1];
   const double gb = gbar ? gbar[0] : 1.0;
@@ -472,9 +521,9 @@ __global__ void query_kernel(const double *__restrict__ bra, int K, int nst,
   }
   double kr[D::KS];
   ket_record<N>(bra, j, p, sml, sms, sc, kr);
-  Bra<N> b;
+  BraReg<N> b;
   load_bra<N>(b, bra + (size_t)i * D::BS);
-  Acc<N> acc;
+  RegSink<N> acc;
   acc.zero();
   double hel = 0.0, sel = 0.0;
   Parts parts;
@@ -494,12 +543,12 @@ __global__ void query_kernel(const double *__restrict__ bra, int K, int nst,
     const double *r = bra + (size_t)i * D::BS;
     const double two = (i == j) ? 2.0 : 1.0;  // python/gradel.py:388-390
     double o[D::BLK];
-    grad_L<N>(acc.XH, acc.sH, r + D::B_L, r + D::B_LD, o);
-    grad_S<N>(r + D::B_A, acc.YH, o + D::NS);
+    grad_L<N>(acc.a + D::A_XH, acc.a[D::A_SH], r + D::B_L, r + D::B_LD, o);
+    grad_S<N>(r + D::B_A, acc.a + D::A_YH, o + D::NS);
 #pragma unroll
     for (int v = 0; v < D::BLK; ++v) tgvl[(size_t)t * D::BLK + v] = two * o[v];
-    grad_L<N>(acc.XS, acc.sS, r + D::B_L, r + D::B_LD, o);
-    grad_S<N>(r + D::B_A, acc.YS, o + D::NS);
+    grad_L<N>(acc.a + D::A_XS, acc.a[D::A_SS], r + D::B_L, r + D::B_LD, o);
+    grad_S<N>(r + D::B_A, acc.a + D::A_YS, o + D::NS);
 #pragma unroll
     for (int v = 0; v < D::BLK; ++v) tgvls[(size_t)t * D::BLK + v] = two * o[v];
   }
@@ -515,9 +564,9 @@ __global__ void dhds_kernel(const double *__restrict__ bra, const double *__rest
   const int k = blockIdx.x * blockDim.x + threadIdx.x;  // ket (coalesced writes)
   const int l = blockIdx.y;                              // bra
   if (k >= K) return;
-  Bra<N> b;
+  BraReg<N> b;
   load_bra<N>(b, bra + (size_t)l * D::BS);
-  Acc<N> acc;
+  RegSink<N> acc;
   acc.zero();
   double hel = 0.0, sel = 0.0;
   bool ok = true;
@@ -534,12 +583,12 @@ __global__ void dhds_kernel(const double *__restrict__ bra, const double *__rest
   const double *r = bra + (size_t)l * D::BS;
   const double two = (l == k) ? 2.0 : 1.0;
   double o[D::BLK];
-  grad_L<N>(acc.XH, acc.sH, r + D::B_L, r + D::B_LD, o);
-  grad_S<N>(r + D::B_A, acc.YH, o + D::NS);
+  grad_L<N>(acc.a + D::A_XH, acc.a[D::A_SH], r + D::B_L, r + D::B_LD, o);
+  grad_S<N>(r + D::B_A, acc.a + D::A_YH, o + D::NS);
 #pragma unroll
   for (int v = 0; v < D::BLK; ++v) DH[((size_t)l * D::BLK + v) * K + k] = two * o[v];
-  grad_L<N>(acc.XS, acc.sS, r + D::B_L, r + D::B_LD, o);
-  grad_S<N>(r + D::B_A, acc.YS, o + D::NS);
+  grad_L<N>(acc.a + D::A_XS, acc.a[D::A_SS], r + D::B_L, r + D::B_LD, o);
+  grad_S<N>(r + D::B_A, acc.a + D::A_YS, o + D::NS);
 #pragma unroll
   for (int v = 0; v < D::BLK; ++v) DS[((size_t)l * D::BLK + v) * K + k] = two * o[v];
 }
@@ -559,6 +608,7 @@ struct necg_ctx {
   double *bra = nullptr, *ket = nullptr, *H = nullptr, *S = nullptr, *HC = nullptr,
          *SC = nullptr, *scal = nullptr;
   double *part = nullptr, *hpart = nullptr, *spart = nullptr;
+  double *hcp = nullptr, *scp = nullptr, *red = nullptr;
   size_t part_cap = 0, hpart_cap = 0;
   int req_kt = 0, req_ts = 0, req_w = 0;
   int last_kt = 0, last_ts = 0, last_w = 0;
@@ -635,6 +685,10 @@ int ensure_K(necg_ctx *c, int K) {
   if ((rc = dev_alloc(&c->HC, k))) return rc;
   if ((rc = dev_alloc(&c->SC, k))) return rc;
   if ((rc = dev_alloc(&c->scal, 8))) return rc;
+  const size_t nchunk = (k + 255) / 256;
+  if ((rc = dev_alloc(&c->hcp, nchunk * k))) return rc;
+  if ((rc = dev_alloc(&c->scp, nchunk * k))) return rc;
+  if ((rc = dev_alloc(&c->red, (size_t)Dim<kMaxN>::NACC * k))) return rc;
   c->Kcap = K;
   return NECG_OK;
 }
@@ -661,7 +715,7 @@ int ensure_hpart(necg_ctx *c, size_t count) {
 // busy without a long tail.
 void choose_tiling(necg_ctx *c, int K, int &kt, int &ts, int &w) {
   const int nst = c->nst;
-  w = c->req_w > 0 ? c->req_w : 1;
+  w = pair_waves(c->n);
   const long bra_waves = (K + 63) / 64;
   const long target = 4096;
   if (c->req_kt > 0) {
@@ -701,7 +755,7 @@ int run_pairs(necg_ctx *c, int K, hipStream_t st, int &ntiles, int &ts_out) {
   ts = (c->nst + tps - 1) / tps;
   const int kts = (K + kt - 1) / kt;
   const int threads = 64 * w;
-  dim3 grid((K + threads - 1) / threads, kts, ts);
+  dim3 grid((K + 63) / 64, kts, ts);
   ntiles = kts * ts;
   ts_out = ts;
   int rc;
@@ -735,24 +789,27 @@ int run_pairs(necg_ctx *c, int K, hipStream_t st, int &ntiles, int &ts_out) {
 }
 
 int finish_hs(necg_ctx *c, int K, int ts, hipStream_t st) {
+  const int bs = bra_stride(c->n), bc = bra_c_off(c->n);
   hipLaunchKernelGGL(hs_finish, dim3((K + 255) / 256, K), dim3(256), 0, st, c->hpart, c->spart,
-                     K, ts, c->H, c->S);
+                     K, ts, c->bra, bs, bc, c->H, c->S, c->hcp, c->scp);
   return hip_status(hipGetLastError());
 }
 
 int finish_quotient(necg_ctx *c, int K, double *f, hipStream_t st) {
   const int bs = bra_stride(c->n), bc = bra_c_off(c->n);
-  hipLaunchKernelGGL(hc_sc, dim3((K + 255) / 256), dim3(256), 0, st, c->H, c->S, c->bra, bs,
-                     bc, K, c->HC, c->SC);
-  hipLaunchKernelGGL(quotient, dim3(1), dim3(1024), 0, st, c->HC, c->SC, c->bra, bs, bc, K,
-                     c->scal, f, c->d_flag);
+  hipLaunchKernelGGL(quotient, dim3(1), dim3(1024), 0, st, c->hcp, c->scp, (K + 255) / 256,
+                     c->bra, bs, bc, K, c->HC, c->SC, c->scal, f, c->d_flag);
   return hip_status(hipGetLastError());
 }
 
 template <int N, int MODE>
 int finish_grad(necg_ctx *c, int K, int ntiles, const double *gbar, double *g, hipStream_t st) {
-  hipLaunchKernelGGL((grad_final<N, MODE>), dim3((K + 255) / 256), dim3(256), 0, st, c->part,
-                     ntiles, K, c->bra, c->scal, c->HC, c->SC, gbar, g);
+  using D = Dim<N>;
+  const int nv = (MODE == MODE_SPLIT) ? D::NACC : D::A_SH + 1;  // MODE 1 uses slots < A_XS and A_SH
+  hipLaunchKernelGGL(tile_reduce, dim3((K + 255) / 256, nv), dim3(256), 0, st, c->part, ntiles, K,
+                     D::NACC, c->red);
+  hipLaunchKernelGGL((grad_final<N, MODE>), dim3((K + 255) / 256), dim3(256), 0, st, c->red, K,
+                     c->bra, c->scal, c->HC, c->SC, gbar, g);
   return hip_status(hipGetLastError());
 }
 
@@ -884,6 +941,8 @@ int necg_create(const necg_system *sys, int device, necg_ctx **out) {
   c->sc.nst = nst;
   c->sc.efs = sys->efs;
   for (int i = 0; i < n * n; ++i) c->sc.mass[i] = sys->mass[i];
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j <= i; ++j) c->sc.msym[sx(i, j)] = sys->mass[i * n + j];
   for (int i = 0; i <= n; ++i) c->sc.q[i] = sys->ncv[i];
   int pi = 0;
   for (int p = 0; p <= n; ++p)
@@ -916,7 +975,7 @@ int necg_destroy(necg_ctx *c) {
   {
     DeviceGuard g(c->device);
     double *bufs[] = {c->d_scvv, c->d_sml, c->d_sms, c->bra, c->ket, c->H, c->S,
-                      c->HC, c->SC, c->scal, c->part, c->hpart, c->spart};
+                      c->HC, c->SC, c->scal, c->part, c->hpart, c->spart, c->hcp, c->scp, c->red};
     for (double *p : bufs)
       if (p) (void)hipFree(p);
     if (c->d_flag) (void)hipFree(c->d_flag);
@@ -1011,7 +1070,8 @@ int necg_status(necg_ctx *c, void *stream) {
 }
 
 int necg_set_tiling(necg_ctx *c, int32_t ket_tile, int32_t term_split, int32_t waves) {
-  if (!c || ket_tile < 0 || term_split < 0 || waves < 0 || waves > 4) return NECG_ERR_ARG;
+  if (!c || ket_tile < 0 || term_split < 0) return NECG_ERR_ARG;
+  if (waves != 0 && waves != pair_waves(c->n)) return NECG_ERR_ARG;  // fixed per n
   std::lock_guard<std::mutex> lk(c->mu);
   c->req_kt = ket_tile;
   c->req_ts = term_split;

@@ -49,6 +49,22 @@ constexpr double kC1 = 0.564691197e-04, kC2 = 0.758433197e-03, kC3 = 0.769838037
 
 constexpr int kMaxN = 6;
 
+// Scheduling fence between the phases of one pair term.  The fully unrolled
+// term is one basic block; without fences the machine scheduler interleaves
+// independent phases (notably the Coulomb pairs) for ILP and the live set
+// grows far beyond the register file.
+#ifndef NECG_FENCE_MASK
+#define NECG_FENCE_MASK 0
+#endif
+#ifndef NECG_NO_FENCES
+#define NECG_FENCE() do { __asm__ volatile("" ::: "memory"); __builtin_amdgcn_sched_barrier(NECG_FENCE_MASK); } while (0)
+#else
+#define NECG_FENCE() ((void)0)
+#endif
+#ifndef NECG_PAIR_FENCE_EVERY
+#define NECG_PAIR_FENCE_EVERY 1
+#endif
+
 __host__ __device__ constexpr int sx(int i, int j) {
   return i >= j ? i * (i + 1) / 2 + j : j * (j + 1) / 2 + i;
 }
@@ -94,13 +110,14 @@ struct Dim {
 struct SysConst {
   int n, nst;
   double efs;
-  double mass[kMaxN * kMaxN];
-  double q[kMaxN + 1];                      // ncv
-  double cpair[kMaxN * (kMaxN + 1) / 2];    // q_p q_q in pair order
+  double mass[kMaxN * kMaxN];             // row-major (ket-table products)
+  double msym[kMaxN * (kMaxN + 1) / 2];   // packed symmetric copy of mass
+  double q[kMaxN + 1];                    // ncv
+  double cpair[kMaxN * (kMaxN + 1) / 2];  // q_p q_q in pair order
 };
 
 // 1/x: hardware seed + two Newton steps (relative error ~1 ulp for the
-// normal range that reaches here).
+// positive normal arguments that reach here).
 __device__ __forceinline__ double rcp(double x) {
   double r = __builtin_amdgcn_rcp(x);
   double e = __builtin_fma(-x, r, 1.0);
@@ -109,61 +126,87 @@ __device__ __forceinline__ double rcp(double x) {
   return __builtin_fma(r, e, r);
 }
 
-__device__ __forceinline__ double rsq(double x) { return rsqrt(x); }
+// 1/sqrt(x) for positive normal x: hardware seed + two Newton steps
+// y <- y + y (1/2 - x y^2 / 2).
+__device__ __forceinline__ double rsq(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  const double hx = 0.5 * x;
+  double r = __builtin_fma(-hx * y, y, 0.5);
+  y = __builtin_fma(y, r, y);
+  r = __builtin_fma(-hx * y, y, 0.5);
+  return __builtin_fma(y, r, y);
+}
+
+constexpr double kSqrtPiHalf = 0.88622692545275801365;   // sqrt(pi)/2
+constexpr double kInvSqrtPi = 0.56418958354775628695;    // 1/sqrt(pi)
 
 // Boys F0 with the reference's Pade form and its derivative
 // (python/utils.py:78-106): F0 = sqrt(P5/Q6) for z <= 16.3578, else
 // sqrt(pi/(4z)); dF0 = (Q P' - P Q') / (2 F0 Q^2), or -F0/(2z).
+// One reciprocal square root serves both branches and both outputs:
+// with v = (P Q^3)^-1/2, F0 = P Q v and dF0 = (Q P' - P Q') v / 2; with
+// v = z^-1/2, F0 = sqrt(pi)/2 v and dF0 = -F0 v^2 / 2.  Branch-free.
 __device__ __forceinline__ void boys_f0(double z, double &f, double &fp) {
-  if (z <= kZcut) {
-    double P = __builtin_fma(__builtin_fma(__builtin_fma(__builtin_fma(__builtin_fma(kC1, z, kC2), z, kC3), z, kC4), z, kC5), z, 1.0);
-    double Q = __builtin_fma(__builtin_fma(__builtin_fma(__builtin_fma(__builtin_fma(__builtin_fma(kC6, z, kC7), z, kC8), z, kC9), z, kC10), z, kC11), z, 1.0);
-    double dP = __builtin_fma(__builtin_fma(__builtin_fma(__builtin_fma(5.0 * kC1, z, 4.0 * kC2), z, 3.0 * kC3), z, 2.0 * kC4), z, kC5);
-    double dQ = __builtin_fma(__builtin_fma(__builtin_fma(__builtin_fma(__builtin_fma(6.0 * kC6, z, 5.0 * kC7), z, 4.0 * kC8), z, 3.0 * kC9), z, 2.0 * kC10), z, kC11);
-    double rQ = rcp(Q);
-    double t = P * rQ;
-    double rt = rsq(t);
-    f = t * rt;
-    fp = (dP - t * dQ) * rQ * (0.5 * rt);
-  } else {
-    double rz = rsq(z);
-    f = 0.88622692545275801365 * rz;  // sqrt(pi)/2 / sqrt(z)
-    fp = -0.5 * f * (rz * rz);
-  }
+  const double P = __builtin_fma(__builtin_fma(__builtin_fma(__builtin_fma(__builtin_fma(kC1, z, kC2), z, kC3), z, kC4), z, kC5), z, 1.0);
+  const double Q = __builtin_fma(__builtin_fma(__builtin_fma(__builtin_fma(__builtin_fma(__builtin_fma(kC6, z, kC7), z, kC8), z, kC9), z, kC10), z, kC11), z, 1.0);
+  const double dP = __builtin_fma(__builtin_fma(__builtin_fma(__builtin_fma(5.0 * kC1, z, 4.0 * kC2), z, 3.0 * kC3), z, 2.0 * kC4), z, kC5);
+  const double dQ = __builtin_fma(__builtin_fma(__builtin_fma(__builtin_fma(__builtin_fma(6.0 * kC6, z, 5.0 * kC7), z, 4.0 * kC8), z, 3.0 * kC9), z, 2.0 * kC10), z, kC11);
+  const bool big = z > kZcut;
+  const double PQ = P * Q;
+  const double v = rsq(big ? z : PQ * (Q * Q));
+  const double fa = kSqrtPiHalf * v;
+  f = big ? fa : PQ * v;
+  fp = big ? -0.5 * fa * (v * v) : 0.5 * __builtin_fma(Q, dP, -P * dQ) * v;
 }
 
-// Bra-side data of one basis function, held in registers for a whole tile.
+// Bra-side data of one basis function.  BraReg holds it in registers (query
+// kernels); BraLds reads it from an LDS table laid out [field][64 lanes] (pair
+// kernels, where the waves of a workgroup share 64 bras).
 template <int N>
-struct Bra {
-  double A[Dim<N>::NS], S[Dim<N>::N3], AS[Dim<N>::N3], i1, pk;
+struct BraReg {
+  double a[Dim<N>::B_PK + 1];
+  __device__ __forceinline__ double A(int t) const { return a[Dim<N>::B_A + t]; }
+  __device__ __forceinline__ double S(int t) const { return a[Dim<N>::B_S + t]; }
+  __device__ __forceinline__ double AS(int t) const { return a[Dim<N>::B_AS + t]; }
+  __device__ __forceinline__ double i1() const { return a[Dim<N>::B_I1]; }
+  __device__ __forceinline__ double pk() const { return a[Dim<N>::B_PK]; }
 };
 
 template <int N>
-__device__ __forceinline__ void load_bra(Bra<N> &b, const double *__restrict__ rec) {
-  using D = Dim<N>;
+__device__ __forceinline__ void load_bra(BraReg<N> &b, const double *__restrict__ rec) {
 #pragma unroll
-  for (int t = 0; t < D::NS; ++t) b.A[t] = rec[D::B_A + t];
-#pragma unroll
-  for (int t = 0; t < D::N3; ++t) {
-    b.S[t] = rec[D::B_S + t];
-    b.AS[t] = rec[D::B_AS + t];
-  }
-  b.i1 = rec[D::B_I1];
-  b.pk = rec[D::B_PK];
+  for (int t = 0; t <= Dim<N>::B_PK; ++t) b.a[t] = rec[t];
 }
 
-// Accumulators of one bra.  MODE 0 keeps the H and S parts apart (E not yet
-// known); MODE 1 folds them with the known energy: X = X_H - E X_S, etc.
 template <int N>
-struct Acc {
-  double XH[Dim<N>::NS], YH[Dim<N>::N3], XS[Dim<N>::NS], YS[Dim<N>::N3], sH, sS;
+struct BraLds {
+  const double *p;  // &table[0 * 64 + lane]
+  __device__ __forceinline__ double A(int t) const { return p[64 * (Dim<N>::B_A + t)]; }
+  __device__ __forceinline__ double S(int t) const { return p[64 * (Dim<N>::B_S + t)]; }
+  __device__ __forceinline__ double AS(int t) const { return p[64 * (Dim<N>::B_AS + t)]; }
+  __device__ __forceinline__ double i1() const { return p[64 * Dim<N>::B_I1]; }
+  __device__ __forceinline__ double pk() const { return p[64 * Dim<N>::B_PK]; }
+};
+
+// Accumulator sinks.  Slot layout (Dim<N>::A_*): X_H | Y_H | X_S | Y_S | sH | sS.
+// MODE 0 keeps the H and S parts apart (E not yet known); MODE 1 folds them
+// with the known energy (X = X_H - E X_S, ...) into the H slots.
+template <int N>
+struct RegSink {  // per-lane registers (query kernels)
+  double a[Dim<N>::NACC];
   __device__ __forceinline__ void zero() {
 #pragma unroll
-    for (int t = 0; t < Dim<N>::NS; ++t) { XH[t] = 0.0; XS[t] = 0.0; }
-#pragma unroll
-    for (int t = 0; t < Dim<N>::N3; ++t) { YH[t] = 0.0; YS[t] = 0.0; }
-    sH = 0.0;
-    sS = 0.0;
+    for (int t = 0; t < Dim<N>::NACC; ++t) a[t] = 0.0;
+  }
+  __device__ __forceinline__ void add(int t, double v) { a[t] += v; }
+};
+
+struct LdsSink {  // this lane's slots in LDS, stride 64 lanes (pair kernels)
+  double *p;
+  // ds_add_f64: a read-modify-write in the LDS unit that holds no register
+  // for the old value; the slots are private to the lane
+  __device__ __forceinline__ void add(int t, double v) {
+    __hip_atomic_fetch_add(p + 64 * t, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
   }
 };
 
@@ -175,17 +218,21 @@ struct Parts {
 enum { MODE_SPLIT = 0, MODE_COMBINED = 1, MODE_ELEMENTS = 2 };
 
 // One (bra, ket-under-term) contribution.  kd points at the ket-table record.
-// Adds sc*ov*alpha to hel and sc*ov to sel; for the gradient modes adds
-// wc*ov-weighted X/Y parts to acc (wc = scvv_P C_l; eshift = E in MODE 1).
-// Returns false if B is not positive definite.
-template <int N, int MODE>
-__device__ __forceinline__ bool pair_term(const Bra<N> &b, const double *__restrict__ kd,
+// Adds sc*ov*alpha to hel and sc*ov to sel; for the gradient modes adds the
+// wc*ov-weighted X/Y parts to the sink (wc = scvv_P C_l; eshift = E in
+// MODE 1).  Returns false if B is not positive definite.
+//
+// Two identities shorten the reference's sequence: A' dif2 = -A dif1 (because
+// A dif1 + A' dif2 = B st - E = 0), so the kinetic quadratic is
+// <A dif1, M A' dif2> = -<a1, M a1> with a1 = A dif1, and the vector
+// A dif1 - A' dif2 of the gradient is 2 a1.
+template <int N, int MODE, class Bra, class Sink>
+__device__ __forceinline__ bool pair_term(const Bra &b, const double *__restrict__ kd,
                                           const SysConst &sc_, double eshift, double &hel,
-                                          double &sel, Acc<N> &acc, Parts *parts) {
+                                          double &sel, Sink &sink, Parts *parts) {
   using D = Dim<N>;
   constexpr int NS = D::NS;
   const double *__restrict__ Ap = kd + D::K_AP;
-  const double *__restrict__ Sp = kd + D::K_SP;
   const double *__restrict__ ApSp = kd + D::K_APSP;
   const double *__restrict__ Np = kd + D::K_NP;
   const double *__restrict__ ApM = kd + D::K_APM;
@@ -196,16 +243,16 @@ __device__ __forceinline__ bool pair_term(const Bra<N> &b, const double *__restr
   bool ok = true;
 #pragma unroll
   for (int j = 0; j < N; ++j) {
-    double s = b.A[sx(j, j)] + Ap[sx(j, j)];
+    double s = b.A(sx(j, j)) + Ap[sx(j, j)];
 #pragma unroll
     for (int m = 0; m < j; ++m) s = __builtin_fma(-R[sx(j, m)], R[sx(j, m)], s);
     ok = ok && (s > 0.0);
-    double r = rsq(s);
+    const double r = rsq(s);
     rd[j] = r;
     R[sx(j, j)] = s * r;
 #pragma unroll
     for (int i = j + 1; i < N; ++i) {
-      double t = b.A[sx(i, j)] + Ap[sx(i, j)];
+      double t = b.A(sx(i, j)) + Ap[sx(i, j)];
 #pragma unroll
       for (int m = 0; m < j; ++m) t = __builtin_fma(-R[sx(i, m)], R[sx(j, m)], t);
       R[sx(i, j)] = t * r;
@@ -234,6 +281,7 @@ __device__ __forceinline__ bool pair_term(const Bra<N> &b, const double *__restr
       for (int m = i; m < N; ++m) t = __builtin_fma(Ri[sx(m, i)], Ri[sx(m, j)], t);
       Bi[sx(i, j)] = t;
     }
+  NECG_FENCE();
   double det32 = 1.0;  // det(B)^-3/2
 #pragma unroll
   for (int j = 0; j < N; ++j) det32 *= rd[j] * rd[j] * rd[j];
@@ -241,7 +289,7 @@ __device__ __forceinline__ bool pair_term(const Bra<N> &b, const double *__restr
   // E, st = B^-1 E, i3 (python/matel.py:125-133)
   double Em[3 * N], st[3 * N];
 #pragma unroll
-  for (int t = 0; t < 3 * N; ++t) Em[t] = b.AS[t] + ApSp[t];
+  for (int t = 0; t < 3 * N; ++t) Em[t] = b.AS(t) + ApSp[t];
   double i3 = 0.0;
 #pragma unroll
   for (int i = 0; i < N; ++i)
@@ -253,55 +301,62 @@ __device__ __forceinline__ bool pair_term(const Bra<N> &b, const double *__restr
       st[3 * i + x] = t;
       i3 = __builtin_fma(Em[3 * i + x], t, i3);
     }
-  const double ov = b.pk * pl * det32 * exp(i3 - b.i1 - i2);
+  const double ov = b.pk() * pl * det32 * exp(i3 - b.i1() - i2);
+  NECG_FENCE();
 
-  // Kinetic (python/matel.py:144-155): i3k = <A dif1, M A' dif2>,
-  // T2 = 3 tr(M A B^-1 A') = 3 sum_ij (A'M A)_ij B^-1_ji.
-  double d1[3 * N], a1[3 * N], a2[3 * N];
+  // Kinetic (python/matel.py:144-155).  a1 = A dif1; the gradient needs
+  // only u = 8 A'M a1 from it, formed here so that dif1, a1 and A'M are dead
+  // before the Coulomb loop.
+  double u8[3 * N];
+  double i3k = 0.0;  // -<a1, M a1>
+  {
+    double d1[3 * N], a1[3 * N];
 #pragma unroll
-  for (int t = 0; t < 3 * N; ++t) d1[t] = st[t] - b.S[t];
+    for (int t = 0; t < 3 * N; ++t) d1[t] = st[t] - b.S(t);
 #pragma unroll
-  for (int i = 0; i < N; ++i)
+    for (int i = 0; i < N; ++i)
 #pragma unroll
-    for (int x = 0; x < 3; ++x) {
-      double u = 0.0, v = 0.0;
+      for (int x = 0; x < 3; ++x) {
+        double v = 0.0;
 #pragma unroll
-      for (int j = 0; j < N; ++j) {
-        u = __builtin_fma(b.A[sx(i, j)], d1[3 * j + x], u);
-        v = __builtin_fma(Ap[sx(i, j)], st[3 * j + x] - Sp[3 * j + x], v);
+        for (int j = 0; j < N; ++j) v = __builtin_fma(b.A(sx(i, j)), d1[3 * j + x], v);
+        a1[3 * i + x] = v;
       }
-      a1[3 * i + x] = u;
-      a2[3 * i + x] = v;
-    }
-  double i3k = 0.0;
 #pragma unroll
-  for (int i = 0; i < N; ++i)
+    for (int i = 0; i < N; ++i)
 #pragma unroll
-    for (int x = 0; x < 3; ++x) {
-      double t = 0.0;  // (M a2)[i][x] = sum_j (A'M)_ji a2? no: M a2 = (A'M)^T dif2
+      for (int x = 0; x < 3; ++x) {
+        double t = 0.0, v = 0.0;
 #pragma unroll
-      for (int j = 0; j < N; ++j) t = __builtin_fma(sc_.mass[i * N + j], a2[3 * j + x], t);
-      i3k = __builtin_fma(a1[3 * i + x], t, i3k);
-    }
-  double T2 = 0.0;
+        for (int j = 0; j < N; ++j) {
+          t = __builtin_fma(sc_.msym[sx(i, j)], a1[3 * j + x], t);
+          if (MODE != MODE_ELEMENTS) v = __builtin_fma(ApM[i * N + j], a1[3 * j + x], v);
+        }
+        i3k = __builtin_fma(-a1[3 * i + x], t, i3k);
+        u8[3 * i + x] = 8.0 * v;
+      }
+  }
+  double T2 = 0.0;  // 3 tr(M A B^-1 A') = 3 sum_ij (A'M A)_ij B^-1_ji
 #pragma unroll
   for (int i = 0; i < N; ++i)
 #pragma unroll
     for (int j = 0; j < N; ++j) {
       double t = 0.0;
 #pragma unroll
-      for (int m = 0; m < N; ++m) t = __builtin_fma(ApM[i * N + m], b.A[sx(m, j)], t);
+      for (int m = 0; m < N; ++m) t = __builtin_fma(ApM[i * N + m], b.A(sx(m, j)), t);
       T2 = __builtin_fma(t, Bi[sx(j, i)], T2);
     }
   T2 *= 3.0;
   const double tfac = 2.0 * i3k + T2;
+  NECG_FENCE();
 
   // Coulomb (python/matel.py:162-197; gradients python/gradel.py:266-337).
   double csum = 0.0;  // sum_c c f r0
   double La[NS], Lb[NS];
-  if (MODE != MODE_ELEMENTS) {
 #pragma unroll
-    for (int t = 0; t < NS; ++t) { La[t] = 0.0; Lb[t] = 0.0; }
+  for (int t = 0; t < NS; ++t) {
+    La[t] = 0.0;
+    Lb[t] = 0.0;
   }
   {
     int pi = 0;
@@ -322,17 +377,17 @@ __device__ __forceinline__ bool pair_term(const Bra<N> &b, const double *__restr
           dy = st[3 * a + 1] - st[3 * bq + 1];
           dz = st[3 * a + 2] - st[3 * bq + 2];
         }
-        const double ri = rsq(i2p);           // 1/sqrt(i2p)
-        const double inv2 = ri * ri;           // 1/i2p
+        const double ri = rsq(i2p);  // 1/sqrt(i2p)
+        const double inv2 = ri * ri;
         const double z = (dx * dx + dy * dy + dz * dz) * inv2;
         double f, fp;
         boys_f0(z, f, fp);
         const double c = sc_.cpair[pi];
-        const double r0 = ri * 0.56418958354775628695;  // 1/sqrt(pi i2p)
-        csum = __builtin_fma(c * f, r0, csum);
+        const double cr0 = c * ri * kInvSqrtPi;  // c / sqrt(pi i2p)
+        csum = __builtin_fma(cr0, f, csum);
         if (MODE != MODE_ELEMENTS) {
-          const double ca = c * (2.0 * f + 4.0 * fp * z) * r0 * inv2;
-          const double cb = c * 4.0 * fp * r0 * inv2;
+          const double cb = 4.0 * fp * cr0 * inv2;
+          const double ca = __builtin_fma(cb, z, 2.0 * f * cr0 * inv2);
           if (p == 0) {
             La[sx(bq, bq)] += ca;
             Lb[sx(bq, bq)] += cb;
@@ -345,6 +400,7 @@ __device__ __forceinline__ bool pair_term(const Bra<N> &b, const double *__restr
             Lb[sx(a, bq)] -= cb;
           }
         }
+        if ((pi + 1) % NECG_PAIR_FENCE_EVERY == 0) NECG_FENCE();
       }
   }
   // Stark (python/matel.py:205-208)
@@ -362,22 +418,20 @@ __device__ __forceinline__ bool pair_term(const Bra<N> &b, const double *__restr
   }
   if (MODE == MODE_ELEMENTS) return ok;
 
-  // Gradient.  h = A dif1 - A' dif2; F^T = 4 A'M h + Lb st - efs q e_z^T.
+  // Gradient.  F^T = 4 A'M (A dif1 - A' dif2) + Lb st - efs q e_z^T
+  //                = 8 A'M a1 + Lb st - efs q e_z^T;  Phi = B^-1 F^T.
   double Ft[3 * N];
 #pragma unroll
   for (int i = 0; i < N; ++i)
 #pragma unroll
     for (int x = 0; x < 3; ++x) {
-      double t = 0.0;
+      double t = u8[3 * i + x];
 #pragma unroll
-      for (int m = 0; m < N; ++m) {
-        t = __builtin_fma(4.0 * ApM[i * N + m], a1[3 * m + x] - a2[3 * m + x], t);
-        t = __builtin_fma(Lb[sx(i, m)], st[3 * m + x], t);
-      }
+      for (int m = 0; m < N; ++m) t = __builtin_fma(Lb[sx(i, m)], st[3 * m + x], t);
       if (x == 2) t = __builtin_fma(-sc_.efs, sc_.q[i + 1], t);
       Ft[3 * i + x] = t;
     }
-  // Phi (without the ov factor) = B^-1 F^T
+  NECG_FENCE();
   double Phi[3 * N];
 #pragma unroll
   for (int i = 0; i < N; ++i)
@@ -388,6 +442,7 @@ __device__ __forceinline__ bool pair_term(const Bra<N> &b, const double *__restr
       for (int m = 0; m < N; ++m) t = __builtin_fma(Bi[sx(i, m)], Ft[3 * m + x], t);
       Phi[3 * i + x] = t;
     }
+  NECG_FENCE();
   // G = B^-1 (12 A'MA' + La) B^-1 (symmetric)
   double Tm[N * N];
 #pragma unroll
@@ -400,6 +455,10 @@ __device__ __forceinline__ bool pair_term(const Bra<N> &b, const double *__restr
         t = __builtin_fma(__builtin_fma(12.0, Np[sx(i, m)], La[sx(i, m)]), Bi[sx(m, j)], t);
       Tm[i * N + j] = t;
     }
+  NECG_FENCE();
+  double d1[3 * N];
+#pragma unroll
+  for (int t = 0; t < 3 * N; ++t) d1[t] = st[t] - b.S(t);
   const double wo = wc * ov;
   const double am = (MODE == MODE_COMBINED) ? (alpha - eshift) : alpha;
   const double woa = wo * am;
@@ -419,16 +478,16 @@ __device__ __forceinline__ bool pair_term(const Bra<N> &b, const double *__restr
       }
       const double x0 = __builtin_fma(-3.0, Bi[sx(i, j)], -2.0 * dd);
       const int t = sx(i, j);
-      acc.XH[t] = __builtin_fma(woa, x0, __builtin_fma(wo, g - dp, acc.XH[t]));
-      if (MODE == MODE_SPLIT) acc.XS[t] = __builtin_fma(wo, x0, acc.XS[t]);
+      sink.add(D::A_XH + t, __builtin_fma(woa, x0, wo * (g - dp)));
+      if (MODE == MODE_SPLIT) sink.add(D::A_XS + t, wo * x0);
     }
 #pragma unroll
   for (int t = 0; t < 3 * N; ++t) {
-    acc.YH[t] = __builtin_fma(2.0 * woa, d1[t], __builtin_fma(wo, Phi[t], acc.YH[t]));
-    if (MODE == MODE_SPLIT) acc.YS[t] = __builtin_fma(2.0 * wo, d1[t], acc.YS[t]);
+    sink.add(D::A_YH + t, __builtin_fma(2.0 * woa, d1[t], wo * Phi[t]));
+    if (MODE == MODE_SPLIT) sink.add(D::A_YS + t, 2.0 * wo * d1[t]);
   }
-  acc.sH += woa;
-  if (MODE == MODE_SPLIT) acc.sS += wo;
+  sink.add(D::A_SH, woa);
+  if (MODE == MODE_SPLIT) sink.add(D::A_SS, wo);
   return ok;
 }
 
