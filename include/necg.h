@@ -115,6 +115,37 @@ int necg_get_tiling(necg_ctx *ctx, int32_t *ket_tile, int32_t *term_split, int32
  * last reset (read synchronises the events).  Not for use under graph capture. */
 int necg_timing(necg_ctx *ctx, int32_t mode, double *ms, int64_t *launches);
 
+/* ---- batches: many independent systems in one evaluation ---------------- */
+
+typedef struct necg_batch necg_batch;
+
+/* A batch of nsys systems (uneven n, nst, K).  Systems are laid out back to
+ * back in one x vector (and one g vector of the same layout); see
+ * necg_batch_layout.  No counterpart in the reference, which evaluates one
+ * system per process; each system's numbers are those of outprod. */
+int necg_batch_create(const necg_system *systems, int32_t nsys, const int32_t *K, int device,
+                      necg_batch **out);
+int necg_batch_destroy(necg_batch *batch);
+
+/* x_offsets[s] = start of system s in x and g; x_offsets[nsys] = total. */
+int necg_batch_layout(necg_batch *batch, int64_t *x_offsets);
+
+/* Energy and gradient of every system: f[s] = E_s, g = dE_s/dx_s per segment.
+ * A fixed sequence of launches (no host sync, no memset): capturable in a
+ * hipGraph.  x, f, g: device.  Async. */
+int necg_batch_outprod(necg_batch *batch, const double *x, double *f, double *g, void *stream);
+
+/* H and S (K_s x K_s) of system s from the last evaluation.  Device.  Async. */
+int necg_batch_matrices(necg_batch *batch, int32_t s, double *H, double *S, void *stream);
+
+/* Pair-tile cost target in model FLOP (0: default 6e7) and persistent
+ * workgroups per CU (0: occupancy limit).  Re-plans the work lists; used by
+ * the autotuner.  Blocking. */
+int necg_batch_set_tiling(necg_batch *batch, double item_flops, int32_t grid_per_cu);
+
+/* As necg_status, for a batch.  Blocking. */
+int necg_batch_status(necg_batch *batch, void *stream);
+
 /* Human-readable text of a status code. */
 const char *necg_strerror(int code);
 

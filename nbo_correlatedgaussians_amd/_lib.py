@@ -25,7 +25,9 @@ NECG_ERR_NONFINITE = 7
 EXPORTS = ("necg_create", "necg_destroy", "necg_outprod", "necg_energy",
            "necg_energy_vjp", "necg_matrices", "necg_matel", "necg_gradel",
            "necg_dhds", "necg_status", "necg_set_tiling", "necg_get_tiling",
-           "necg_timing", "necg_strerror")
+           "necg_timing", "necg_strerror", "necg_batch_create", "necg_batch_destroy",
+           "necg_batch_layout", "necg_batch_outprod", "necg_batch_matrices",
+           "necg_batch_set_tiling", "necg_batch_status")
 
 
 class NecgError(RuntimeError):
@@ -91,6 +93,13 @@ def load() -> ct.CDLL:
         "necg_set_tiling": [P, I, I, I],
         "necg_get_tiling": [P, ct.POINTER(I), ct.POINTER(I), ct.POINTER(I)],
         "necg_timing": [P, I, ct.POINTER(D), ct.POINTER(ct.c_int64)],
+        "necg_batch_create": [P, I, P, ct.c_int, ct.POINTER(P)],
+        "necg_batch_destroy": [P],
+        "necg_batch_layout": [P, P],
+        "necg_batch_outprod": [P, P, P, P, P],
+        "necg_batch_matrices": [P, I, P, P, P],
+        "necg_batch_set_tiling": [P, D, I],
+        "necg_batch_status": [P, P],
     }
     for name, args in sig.items():
         fn = getattr(lib, name)

@@ -11,9 +11,7 @@ import subprocess
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SRC = os.path.join(HERE, "csrc", "necg.hip")
-HDRS = [os.path.join(HERE, "csrc", "necg_pair.hpp"),
-        os.path.join(HERE, "..", "include", "necg.h")]
+SRCS = [os.path.join(HERE, "csrc", "necg.hip"), os.path.join(HERE, "csrc", "necg_batch.hip")]
 OUT = os.path.join(HERE, "libnecg_hip.so")
 ARCH = os.environ.get("NECG_OFFLOAD_ARCH", "gfx950")
 
@@ -33,7 +31,7 @@ def hipcc() -> str:
 
 def build(verbose: bool = False, extra=()) -> str:
     tmp = OUT + ".tmp"
-    cmd = [hipcc(), *HIPCC_FLAGS, *extra, SRC, "-o", tmp]
+    cmd = [hipcc(), *HIPCC_FLAGS, *extra, *SRCS, "-o", tmp]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

@@ -24,95 +24,27 @@
 #include <vector>
 
 #include "../../include/necg.h"
-#include "necg_pair.hpp"
+#include "necg_finish.hpp"
 
 using namespace necg;
 
 namespace {
 
-// latched device errors; necg_status reports the root cause first:
-// bad input, then a singular pair matrix, then C^T S C <= 0, then NaN results
-enum FlagBits { F_NOT_PD = 1, F_NONFINITE = 2, F_CSC = 4, F_BAD_INPUT = 8 };
 
 // ------------------------------------------------------------------ prep ----
 
-// One lane per basis function k: the bra record (python/matel.py:56-70, :125-136).
+// One lane per basis function k: the bra record.
 template <int N>
 __global__ void prep_bra(const double *__restrict__ x, int K, int with_c,
                          double *__restrict__ bra, int *__restrict__ flag) {
   using D = Dim<N>;
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= K) return;
-  const double *p = x + (size_t)k * D::BLK;
-  double L[N][N];
-#pragma unroll
-  for (int i = 0; i < N; ++i)
-#pragma unroll
-    for (int j = 0; j < N; ++j) L[i][j] = 0.0;
-  int t = 0;
-  bool bad = false, pd = true;
-#pragma unroll
-  for (int j = 0; j < N; ++j)
-#pragma unroll
-    for (int i = j; i < N; ++i, ++t) {
-      L[i][j] = p[t];
-      bad = bad || !isfinite(p[t]);
-    }
-  double *r = bra + (size_t)k * D::BS;
-  double detL = 1.0;
-#pragma unroll
-  for (int i = 0; i < N; ++i) {
-    detL *= L[i][i];
-    pd = pd && (L[i][i] != 0.0);
-    r[D::B_LD + i] = 1.0 / L[i][i];
-  }
-#pragma unroll
-  for (int v = 0; v < D::NS; ++v) r[D::B_L + v] = p[v];
-  double A[D::NS];
-#pragma unroll
-  for (int i = 0; i < N; ++i)
-#pragma unroll
-    for (int j = 0; j <= i; ++j) {
-      double s = 0.0;
-#pragma unroll
-      for (int m = 0; m <= j; ++m) s = __builtin_fma(L[i][m], L[j][m], s);
-      A[sx(i, j)] = s;
-      r[D::B_A + sx(i, j)] = s;
-    }
-  double S[D::N3];
-#pragma unroll
-  for (int v = 0; v < D::N3; ++v) {
-    S[v] = p[D::NS + v];
-    bad = bad || !isfinite(S[v]);
-    r[D::B_S + v] = S[v];
-  }
-  double i1 = 0.0;
-#pragma unroll
-  for (int i = 0; i < N; ++i)
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      double s = 0.0;
-#pragma unroll
-      for (int j = 0; j < N; ++j) s = __builtin_fma(A[sx(i, j)], S[3 * j + c], s);
-      r[D::B_AS + 3 * i + c] = s;
-      i1 = __builtin_fma(S[3 * i + c], s, i1);
-    }
-  r[D::B_I1] = i1;
-  // pk = 8^(n/4) |det L|^(3/2): the bra half of sqrt(8^n) sqrt((|det L_k| |det L_l| / det B)^3)
-  const double ad = fabs(detL);
-  r[D::B_PK] = pow(8.0, 0.25 * N) * ad * sqrt(ad);
-  double c = 0.0;
-  if (with_c) {
-    c = x[(size_t)K * D::BLK + k];
-    bad = bad || !isfinite(c);
-  }
-  r[D::B_C] = c;
-  if (bad) atomicOr(flag, (int)F_BAD_INPUT);
-  if (!pd) atomicOr(flag, (int)F_NOT_PD);
+  make_bra<N>(x + (size_t)k * D::BLK, with_c ? x[(size_t)K * D::BLK + k] : 0.0, with_c != 0,
+              bra + (size_t)k * D::BS, flag);
 }
 
 // One lane per (ket l, term p): the transformed ket record.
-// A' = sml^T A_l sml, S' = sms s_l (python/matel.py:76-87).
 template <int N>
 __global__ void prep_ket(const double *__restrict__ bra, int K, int nst,
                          const double *__restrict__ sml, const double *__restrict__ sms,
@@ -121,84 +53,8 @@ __global__ void prep_ket(const double *__restrict__ bra, int K, int nst,
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= K * nst) return;
   const int l = idx / nst, p = idx - l * nst;
-  const double *b = bra + (size_t)l * D::BS;
-  const double *T = sml + (size_t)p * N * N;
-  const double *Sm = sms + (size_t)p * D::N3 * D::N3;
-  double Al[N][N], AT[N][N], Ap[N][N];
-#pragma unroll
-  for (int i = 0; i < N; ++i)
-#pragma unroll
-    for (int j = 0; j < N; ++j) Al[i][j] = b[D::B_A + sx(i, j)];
-#pragma unroll
-  for (int i = 0; i < N; ++i)
-#pragma unroll
-    for (int j = 0; j < N; ++j) {
-      double s = 0.0;
-#pragma unroll
-      for (int m = 0; m < N; ++m) s = __builtin_fma(Al[i][m], T[m * N + j], s);
-      AT[i][j] = s;
-    }
-#pragma unroll
-  for (int i = 0; i < N; ++i)
-#pragma unroll
-    for (int j = 0; j < N; ++j) {
-      double s = 0.0;
-#pragma unroll
-      for (int m = 0; m < N; ++m) s = __builtin_fma(T[m * N + i], AT[m][j], s);
-      Ap[i][j] = s;
-    }
-  double *r = ket + (size_t)idx * D::KS;
-#pragma unroll
-  for (int i = 0; i < N; ++i)
-#pragma unroll
-    for (int j = 0; j <= i; ++j) r[D::K_AP + sx(i, j)] = 0.5 * (Ap[i][j] + Ap[j][i]);
-  double Sp[D::N3];
-#pragma unroll
-  for (int u = 0; u < D::N3; ++u) {
-    double s = 0.0;
-#pragma unroll
-    for (int v = 0; v < D::N3; ++v) s = __builtin_fma(Sm[u * D::N3 + v], b[D::B_S + v], s);
-    Sp[u] = s;
-    r[D::K_SP + u] = s;
-  }
-  double i2 = 0.0;
-#pragma unroll
-  for (int i = 0; i < N; ++i)
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      double s = 0.0;
-#pragma unroll
-      for (int j = 0; j < N; ++j) s = __builtin_fma(Ap[i][j], Sp[3 * j + c], s);
-      r[D::K_APSP + 3 * i + c] = s;
-      i2 = __builtin_fma(Sp[3 * i + c], s, i2);
-    }
-  double ApM[N][N];
-#pragma unroll
-  for (int i = 0; i < N; ++i)
-#pragma unroll
-    for (int j = 0; j < N; ++j) {
-      double s = 0.0;
-#pragma unroll
-      for (int m = 0; m < N; ++m) s = __builtin_fma(Ap[i][m], sc.mass[m * N + j], s);
-      ApM[i][j] = s;
-      r[D::K_APM + i * N + j] = s;
-    }
-#pragma unroll
-  for (int i = 0; i < N; ++i)
-#pragma unroll
-    for (int j = 0; j <= i; ++j) {
-      double s = 0.0;
-#pragma unroll
-      for (int m = 0; m < N; ++m) s = __builtin_fma(ApM[i][m], Ap[m][j], s);
-      double s2 = 0.0;
-#pragma unroll
-      for (int m = 0; m < N; ++m) s2 = __builtin_fma(ApM[j][m], Ap[m][i], s2);
-      r[D::K_NP + sx(i, j)] = 0.5 * (s + s2);
-    }
-  r[D::K_I2] = i2;
-  r[D::K_PL] = b[D::B_PK];
-  r[D::K_WC] = scvv[p] * b[D::B_C];
-  r[D::K_SC] = scvv[p];
+  make_ket<N>(bra + (size_t)l * D::BS, sml + (size_t)p * N * N, sms + (size_t)p * D::N3 * D::N3,
+              scvv[p], sc, ket + (size_t)idx * D::KS);
 }
 
 // ------------------------------------------------------------------ pair ----
@@ -207,304 +63,72 @@ __global__ void prep_ket(const double *__restrict__ bra, int K, int nst,
 // part layout: part[((tile * NACC) + v) * K + k], tile = y * TS + z.
 // hpart/spart layout: [z][l][k] for bra k >= ket l.
 
-// Waves per pair-kernel workgroup: they share the 64 bra records in LDS and
-// each owns a private set of accumulator slots, so the sums are formed in a
-// fixed order (bitwise reproducible for a given launch shape).  LDS per
-// workgroup: (B_PK+1)*512 B + W*NACC*512 B = 151 KB for n = 5, W = 4.
-__host__ __device__ constexpr int pair_waves(int n) { return n <= 5 ? 4 : 2; }
-
 // One wave per SIMD: the term needs ~300 live FP64 values; at this
 // occupancy the compiler keeps the overflow in AGPRs instead of scratch.
 #ifndef NECG_MIN_WAVES
 #define NECG_MIN_WAVES 1
 #endif
 
+// Lane = bra k.  blockIdx.y = ket tile, blockIdx.z = term group.
+// part layout: part[((tile * NACC) + v) * K + k], tile = y * TS + z.
+// hpart/spart layout: [z][l][k] for bra k >= ket l.
 template <int N, int MODE>
 __global__ void __launch_bounds__(64 * pair_waves(N), NECG_MIN_WAVES) pair_kernel(
     const double *__restrict__ bra, const double *__restrict__ ket, int K, int nst, int kt,
     int tps, SysConst sc, const double *__restrict__ scal, double *__restrict__ part,
     double *__restrict__ hpart, double *__restrict__ spart, int *__restrict__ flag) {
   using D = Dim<N>;
-  constexpr int W = pair_waves(N);
-  constexpr int NA = (MODE == MODE_ELEMENTS) ? 1 : D::NACC * 64;
-  // the 64 bras of this workgroup, [field][lane], and per-wave accumulators
-  // [wave][slot][lane]; the waves take alternate kets of the tile
-  __shared__ double bra_s[(D::B_PK + 1) * 64];
-  __shared__ double acc_s[W * NA];
-  const int lane = threadIdx.x & 63;
-  // wave index: uniform across the wave, so that the ket index and every
-  // ket-record address are scalar (s_load into SGPRs)
-  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int k = blockIdx.x * 64 + lane;
+  __shared__ double bra_s[PairLds<N, MODE>::BRA];
+  __shared__ double acc_s[PairLds<N, MODE>::ACC];
   const int l0 = blockIdx.y * kt;
-  const int l1 = min(K, l0 + kt);
   const int p0 = blockIdx.z * tps;
-  const int p1 = min(nst, p0 + tps);
-  const bool active = k < K;
-  const int kk = active ? k : K - 1;
-  for (int v = wv; v <= D::B_PK; v += W) bra_s[v * 64 + lane] = bra[(size_t)kk * D::BS + v];
-  if (MODE != MODE_ELEMENTS)
-    for (int v = 0; v < D::NACC; ++v) acc_s[wv * NA + v * 64 + lane] = 0.0;
-  __syncthreads();
-  const BraLds<N> b{bra_s + lane};
-  const double eshift = (MODE == MODE_COMBINED) ? scal[0] : 0.0;
-  LdsSink sink{acc_s + wv * NA + lane};
-  bool ok = true;
-  for (int l = l0 + wv; l < l1; l += W) {
-    double hel = 0.0, sel = 0.0;
-    const double *kl = ket + (size_t)l * nst * D::KS;
-    for (int p = p0; p < p1; ++p) {
-      // keep the bra in LDS: re-read it for every term instead of holding
-      // 47 doubles in registers across the loop
-      __asm__ volatile("" ::: "memory");
-      ok &= pair_term<N, MODE>(b, kl + (size_t)p * D::KS, sc, eshift, hel, sel, sink, nullptr);
-    }
-    if (MODE != MODE_COMBINED && active && k >= l) {
-      const size_t o = ((size_t)blockIdx.z * K + l) * K + k;
-      hpart[o] = hel;
-      spart[o] = sel;
-    }
-  }
-  if (active && !ok) atomicOr(flag, (int)F_NOT_PD);
-  if (MODE == MODE_ELEMENTS) return;
-  __syncthreads();
-  if (!active) return;
   const size_t tile = (size_t)blockIdx.y * gridDim.z + blockIdx.z;
-  double *o = part + tile * D::NACC * K + k;
-  for (int v = wv; v < D::NACC; v += W) {
-    if (MODE == MODE_COMBINED && v >= D::A_XS && v != D::A_SH) continue;
-    double t = 0.0;
-#pragma unroll
-    for (int w = 0; w < W; ++w) t += acc_s[w * NA + v * 64 + lane];
-    o[(size_t)v * K] = t;
-  }
+  pair_tile<N, MODE>(bra, ket, K, nst, blockIdx.x * 64, l0, min(K, l0 + kt), p0,
+                     min(nst, p0 + tps), sc, (MODE == MODE_COMBINED) ? scal[0] : 0.0,
+                     part + tile * D::NACC * K, hpart + (size_t)blockIdx.z * K * K,
+                     spart + (size_t)blockIdx.z * K * K, flag, bra_s, acc_s);
 }
 
-// Sum of a value over the 256 lanes of a workgroup (4 waves): wave
-// reduction by shuffles, then one LDS slot per wave.
-__device__ __forceinline__ double block_sum256(double v, double *red4) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-  if ((threadIdx.x & 63) == 0) red4[threadIdx.x >> 6] = v;
-  __syncthreads();
-  double t = 0.0;
-  if (threadIdx.x == 0) t = (red4[0] + red4[1]) + (red4[2] + red4[3]);
-  __syncthreads();
-  return t;
-}
-
-// Sum term groups, mirror the lower triangle (bra i >= ket j) into full
-// symmetric H and S (python/outprod.py:57-60), and form the partial row sums
-// of H C and S C of this 256-column chunk.  Grid (ceil(K/256), K), 256 lanes.
+// Full symmetric H and S and the per-chunk partials of H C and S C.
+// Grid (ceil(K/256), K), 256 lanes.
 __global__ void __launch_bounds__(256) hs_finish(
     const double *__restrict__ hpart, const double *__restrict__ spart, int K, int ts,
     const double *__restrict__ bra, int bs, int bc, double *__restrict__ H,
     double *__restrict__ S, double *__restrict__ hcp, double *__restrict__ scp) {
   __shared__ double red[8];
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  const int i = blockIdx.y;
-  double hc = 0.0, sc = 0.0;
-  if (j < K) {
-    const int lo = min(i, j), hi = max(i, j);
-    double h = 0.0, s = 0.0;
-    for (int z = 0; z < ts; ++z) {
-      const size_t o = ((size_t)z * K + lo) * K + hi;
-      h += hpart[o];
-      s += spart[o];
-    }
-    H[(size_t)i * K + j] = h;
-    S[(size_t)i * K + j] = s;
-    const double c = bra[(size_t)j * bs + bc];
-    hc = h * c;
-    sc = s * c;
-  }
-  hc = block_sum256(hc, red);
-  sc = block_sum256(sc, red + 4);
-  if (threadIdx.x == 0) {
-    hcp[(size_t)blockIdx.x * K + i] = hc;
-    scp[(size_t)blockIdx.x * K + i] = sc;
-  }
+  hs_row_chunk(hpart, spart, K, ts, bra, bs, bc, blockIdx.y, blockIdx.x, H, S, hcp, scp, red);
 }
 
-// HC, SC from the chunk partials, then C^T H C, C^T S C, E and 1/C^T S C
-// (python/outprod.py:65-68) in one workgroup of 1024 lanes.
 __global__ void __launch_bounds__(1024) quotient(
     const double *__restrict__ hcp, const double *__restrict__ scp, int nchunk,
     const double *__restrict__ bra, int bs, int bc, int K, double *__restrict__ HC,
     double *__restrict__ SC, double *__restrict__ scal, double *__restrict__ fout,
     int *__restrict__ flag) {
   __shared__ double rh[1024], rs[1024];
-  double h = 0.0, s = 0.0;
-  for (int i = threadIdx.x; i < K; i += blockDim.x) {
-    double a = 0.0, b = 0.0;
-    for (int c = 0; c < nchunk; ++c) {
-      a += hcp[(size_t)c * K + i];
-      b += scp[(size_t)c * K + i];
-    }
-    HC[i] = a;
-    SC[i] = b;
-    const double ci = bra[(size_t)i * bs + bc];
-    h = __builtin_fma(ci, a, h);
-    s = __builtin_fma(ci, b, s);
-  }
-  rh[threadIdx.x] = h;
-  rs[threadIdx.x] = s;
-  __syncthreads();
-  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) {
-      rh[threadIdx.x] += rh[threadIdx.x + w];
-      rs[threadIdx.x] += rs[threadIdx.x + w];
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    const double cHc = rh[0], cSc = rs[0];
-    const double E = cHc / cSc;
-    scal[0] = E;
-    scal[1] = 1.0 / cSc;
-    scal[2] = cHc;
-    scal[3] = cSc;
-    if (fout) fout[0] = E;
-    if (!(cSc > 0.0)) atomicOr(flag, (int)F_CSC);
-    if (!isfinite(E)) atomicOr(flag, (int)F_NONFINITE);
-  }
+  quotient_block(hcp, scp, nchunk, bra, bs, bc, K, HC, SC, scal, fout, flag, rh, rs);
 }
 
-// Sum of the tile partials of every accumulator slot: red[v*K + k].
-// Grid (ceil(K/256), nv), 256 lanes; coalesced over k.
+// Grid (ceil(K/256), slots), 256 lanes; coalesced over k.
 __global__ void __launch_bounds__(256) tile_reduce(const double *__restrict__ part, int ntiles,
                                                    int K, int nacc, double *__restrict__ red) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  const int v = blockIdx.y;
-  if (k >= K) return;
-  double a = 0.0;
-  for (int t = 0; t < ntiles; ++t) a += part[((size_t)t * nacc + v) * K + k];
-  red[(size_t)v * K + k] = a;
+  if (k < K) tile_reduce_one(part, ntiles, K, nacc, blockIdx.y, k, red);
 }
 
-// Lane per bra k: transform the summed accumulators and contract,
-// g_theta = norm (a - E b) (MODE 0) or norm a (MODE 1, E folded), with
-// a = 2 C_k [vech(X_H L) + 1.5 sigma_H diag(1/L) | A Y_H]
-// (python/make_grad.py:61-79); g_C = 2 norm (HC - E SC) (:85).
-// gbar (nullable) scales the whole gradient (vector-Jacobian product).
 template <int N, int MODE>
 __global__ void grad_final(const double *__restrict__ red, int K,
                            const double *__restrict__ bra, const double *__restrict__ scal,
                            const double *__restrict__ HC, const double *__restrict__ SC,
                            const double *__restrict__ gbar, double *__restrict__ g) {
-  using D = Dim<N>;
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= K) return;
-  double a[D::NACC];
-#pragma unroll
-  for (int v = 0; v < D::NACC; ++v)
-    a[v] = (MODE == MODE_SPLIT || v < D::A_XS || v == D::A_SH) ? red[(size_t)v * K + k] : 0.0;
-  const double *r = bra + (size_t)k * D::BS;
-  const double E = scal[0], nrm = scal[1];
-  const double gb = gbar ? gbar[0] : 1.0;
-  const double ck = r[D::B_C];
-  const double fac = gb * nrm * 2.0 * ck;
-  double oh[D::BLK];
-  grad_L<N>(a + D::A_XH, a[D::A_SH], r + D::B_L, r + D::B_LD, oh);
-  grad_S<N>(r + D::B_A, a + D::A_YH, oh + D::NS);
-  double *gk = g + (size_t)k * D::BLK;
-  if (MODE == MODE_SPLIT) {
-    double os[D::BLK];
-    grad_L<N>(a + D::A_XS, a[D::A_SS], r + D::B_L, r + D::B_LD, os);
-    grad_S<N>(r + D::B_A, a + D::A_YS, os + D::NS);
-#pragma unroll
-    for (int v = 0; v < D::BLK; ++v) gk[v] = fac * (oh[v] - E * os[v]);
-  } else {
-#pragma unroll
-    for (int v = 0; v < D::BLK; ++v) gk[v] = fac * oh[v];
-  }
-  g[(size_t)K * D::BLK + k] = gb * 2.0 * nrm * (HC[k] - E * SC[k]);
+  if (k < K) grad_one<N, MODE>(red, K, k, bra, scal, HC, SC, gbar ? gbar[0] : 1.0, g);
 }
 
 // ----------------------------------------------------------- queries -----
 
 // Per-term matel / gradel for (stf, i, j) triples (python/matel.py:28,
-// python/gradel.py:27).  The ket record is rebuilt in registers per query.
-template <int N>
-__device__ void ket_record(const double *__restrict__ bra, int l, int p,
-                           const double *__restrict__ sml, const double *__restrict__ sms,
-                           const SysConst &sc, double *r) {
-  using D = Dim<N>;
-  const double *b = bra + (size_t)l * D::BS;
-  const double *T = sml + (size_t)p * N * N;
-  const double *Sm = sms + (size_t)p * D::N3 * D::N3;
-  double AT[N][N], Ap[N][N];
-#pragma unroll
-  for (int i = 0; i < N; ++i)
-#pragma unroll
-    for (int j = 0; j < N; ++j) {
-      double s = 0.0;
-#pragma unroll
-      for (int m = 0; m < N; ++m) s = __builtin_fma(b[D::B_A + sx(i, m)], T[m * N + j], s);
-      AT[i][j] = s;
-    }
-#pragma unroll
-  for (int i = 0; i < N; ++i)
-#pragma unroll
-    for (int j = 0; j < N; ++j) {
-      double s = 0.0;
-#pragma unroll
-      for (int m = 0; m < N; ++m) s = __builtin_fma(T[m * N + i], AT[m][j], s);
-      Ap[i][j] = s;
-    }
-#pragma unroll
-  for (int i = 0; i < N; ++i)
-#pragma unroll
-    for (int j = 0; j <= i; ++j) r[D::K_AP + sx(i, j)] = 0.5 * (Ap[i][j] + Ap[j][i]);
-  double Sp[D::N3];
-#pragma unroll
-  for (int u = 0; u < D::N3; ++u) {
-    double s = 0.0;
-#pragma unroll
-    for (int v = 0; v < D::N3; ++v) s = __builtin_fma(Sm[u * D::N3 + v], b[D::B_S + v], s);
-    Sp[u] = s;
-    r[D::K_SP + u] = s;
-  }
-  double i2 = 0.0;
-#pragma unroll
-  for (int i = 0; i < N; ++i)
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      double s = 0.0;
-#pragma unroll
-      for (int j = 0; j < N; ++j) s = __builtin_fma(Ap[i][j], Sp[3 * j + c], s);
-      r[D::K_APSP + 3 * i + c] = s;
-      i2 = __builtin_fma(Sp[3 * i + c], s, i2);
-    }
-  double ApM[N][N];
-#pragma unroll
-  for (int i = 0; i < N; ++i)
-#pragma unroll
-    for (int j = 0; j < N; ++j) {
-      double s = 0.0;
-#pragma unroll
-      for (int m = 0; m < N; ++m) s = __builtin_fma(Ap[i][m], sc.mass[m * N + j], s);
-      ApM[i][j] = s;
-      r[D::K_APM + i * N + j] = s;
-    }
-#pragma unroll
-  for (int i = 0; i < N; ++i)
-#pragma unroll
-    for (int j = 0; j <= i; ++j) {
-      double s = 0.0, s2 = 0.0;
-#pragma unroll
-      for (int m = 0; m < N; ++m) {
-        s = __builtin_fma(ApM[i][m], Ap[m][j], s);
-        s2 = __builtin_fma(ApM[j][m], Ap[m][i], s2);
-      }
-      r[D::K_NP + sx(i, j)] = 0.5 * (s + s2);
-    }
-  r[D::K_I2] = i2;
-  r[D::K_PL] = b[D::B_PK];
-  r[D::K_WC] = 1.0;
-  r[D::K_SC] = 1.0;
-}
-
+// python/gradel.py:27).  The ket record is rebuilt in registers per query
+// with unit weights.
 template <int N>
 __global__ void query_kernel(const double *__restrict__ bra, int K, int nst,
                              const double *__restrict__ sml, const double *__restrict__ sms,
@@ -520,7 +144,9 @@ __global__ void query_kernel(const double *__restrict__ bra, int K, int nst,
     return;
   }
   double kr[D::KS];
-  ket_record<N>(bra, j, p, sml, sms, sc, kr);
+  make_ket<N>(bra + (size_t)j * D::BS, sml + (size_t)p * N * N,
+              sms + (size_t)p * D::N3 * D::N3, 1.0, sc, kr);
+  kr[D::K_WC] = 1.0;
   BraReg<N> b;
   load_bra<N>(b, bra + (size_t)i * D::BS);
   RegSink<N> acc;

@@ -1,0 +1,566 @@
+// necg_batch.hip -- many independent systems (uneven n, nst and K) in one
+// evaluation: segmented preparation and finishing kernels over all systems,
+// and one persistent pair-kernel launch per particle count n whose
+// workgroups walk a cost-sorted list of (system, bra tile, ket tile, term
+// group) items.  The whole evaluation is a fixed sequence of launches with no
+// host synchronisation and no memset, so a caller can capture it in a
+// hipGraph and replay it.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <mutex>
+#include <new>
+#include <numeric>
+#include <vector>
+
+#include "../../include/necg.h"
+#include "necg_finish.hpp"
+
+using namespace necg;
+
+namespace {
+
+// Per-system record, resident on the device.
+struct BatchSys {
+  SysConst sc;
+  int n, nst, K, ts, tps, nchunk, ntiles, nacc;
+  long long x_off;                   // x / g (doubles)
+  long long bra_off, ket_off;        // bra / ket tables
+  long long hs_off;                  // H, S: K*K
+  long long hp_off;                  // element partials: ts*K*K
+  long long part_off;                // gradient partials: ntiles*nacc*K
+  long long red_off;                 // summed accumulators: nacc*K
+  long long vec_off;                 // HC, SC: K
+  long long hcp_off;                 // chunk partials: nchunk*K
+  long long sml_off, sms_off, scvv_off;
+};
+
+struct Item {  // one pair tile
+  int sys, k0, l0, l1, p0, p1, tile;
+};
+
+// Segment s with prefix[s] <= idx < prefix[s+1] (prefix has nseg+1 entries).
+__device__ __forceinline__ int seg_find(const long long *__restrict__ prefix, int nseg,
+                                        long long idx) {
+  int lo = 0, hi = nseg - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (prefix[mid] <= idx) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+#define SWITCH_N(n, call)                         \
+  switch (n) {                                    \
+    case 1: { constexpr int NN = 1; call; break; } \
+    case 2: { constexpr int NN = 2; call; break; } \
+    case 3: { constexpr int NN = 3; call; break; } \
+    case 4: { constexpr int NN = 4; call; break; } \
+    case 5: { constexpr int NN = 5; call; break; } \
+    default: { constexpr int NN = 6; call; break; } \
+  }
+
+// one lane per (system, function)
+__global__ void b_prep_bra(const BatchSys *__restrict__ sys, int nsys,
+                           const long long *__restrict__ fpre, const double *__restrict__ x,
+                           double *__restrict__ bra, int *__restrict__ flag) {
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= fpre[nsys]) return;
+  const int s = seg_find(fpre, nsys, idx);
+  const BatchSys &b = sys[s];
+  const int k = (int)(idx - fpre[s]);
+  const double *xs = x + b.x_off;
+  SWITCH_N(b.n, (make_bra<NN>(xs + (size_t)k * Dim<NN>::BLK, xs[(size_t)b.K * Dim<NN>::BLK + k],
+                              true, bra + b.bra_off + (size_t)k * Dim<NN>::BS, flag)));
+}
+
+// one lane per (system, ket, term)
+__global__ void b_prep_ket(const BatchSys *__restrict__ sys, int nsys,
+                           const long long *__restrict__ kpre, const double *__restrict__ sml,
+                           const double *__restrict__ sms, const double *__restrict__ scvv,
+                           const double *__restrict__ bra, double *__restrict__ ket) {
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= kpre[nsys]) return;
+  const int s = seg_find(kpre, nsys, idx);
+  const BatchSys &b = sys[s];
+  const int li = (int)(idx - kpre[s]);
+  const int l = li / b.nst, p = li - l * b.nst;
+  SWITCH_N(b.n, (make_ket<NN>(bra + b.bra_off + (size_t)l * Dim<NN>::BS,
+                              sml + b.sml_off + (size_t)p * NN * NN,
+                              sms + b.sms_off + (size_t)p * 9 * NN * NN, scvv[b.scvv_off + p],
+                              b.sc, ket + b.ket_off + (size_t)li * Dim<NN>::KS)));
+}
+
+// Persistent pair kernel for one particle count: each workgroup walks the
+// item list with stride gridDim.x (items sorted by decreasing cost).
+template <int N>
+__global__ void __launch_bounds__(64 * pair_waves(N), 1) b_pair(
+    const BatchSys *__restrict__ sys, const Item *__restrict__ items, int nitems,
+    const double *__restrict__ bra, const double *__restrict__ ket, double *__restrict__ part,
+    double *__restrict__ hpart, double *__restrict__ spart, int *__restrict__ flag) {
+  using D = Dim<N>;
+  __shared__ double bra_s[PairLds<N, MODE_SPLIT>::BRA];
+  __shared__ double acc_s[PairLds<N, MODE_SPLIT>::ACC];
+  for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+    const Item w = items[it];
+    const BatchSys &b = sys[w.sys];
+    const int K = b.K;
+    const int z = w.tile % b.ts;
+    pair_tile<N, MODE_SPLIT>(bra + b.bra_off, ket + b.ket_off, K, b.nst, w.k0, w.l0, w.l1, w.p0,
+                             w.p1, b.sc, 0.0, part + b.part_off + (size_t)w.tile * D::NACC * K,
+                             hpart + b.hp_off + (size_t)z * K * K,
+                             spart + b.hp_off + (size_t)z * K * K, flag, bra_s, acc_s);
+    __syncthreads();  // the next item overwrites the LDS tables
+  }
+}
+
+// one workgroup of 256 lanes per (system, row i, 256-column chunk)
+__global__ void __launch_bounds__(256) b_hs_finish(const BatchSys *__restrict__ sys, int nsys,
+                                                   const long long *__restrict__ hpre,
+                                                   const double *__restrict__ hpart,
+                                                   const double *__restrict__ spart,
+                                                   const double *__restrict__ bra,
+                                                   double *__restrict__ H, double *__restrict__ S,
+                                                   double *__restrict__ hcp,
+                                                   double *__restrict__ scp) {
+  __shared__ double red[8];
+  const long long blk = blockIdx.x;
+  const int s = seg_find(hpre, nsys, blk);
+  const BatchSys &b = sys[s];
+  const int r = (int)(blk - hpre[s]);
+  const int i = r / b.nchunk, chunk = r - i * b.nchunk;
+  int bs = 0, bc = 0;
+  SWITCH_N(b.n, (bs = Dim<NN>::BS, bc = Dim<NN>::B_C));
+  hs_row_chunk(hpart + b.hp_off, spart + b.hp_off, b.K, b.ts, bra + b.bra_off, bs, bc, i, chunk,
+               H + b.hs_off, S + b.hs_off, hcp + b.hcp_off, scp + b.hcp_off, red);
+}
+
+// one workgroup of 1024 lanes per system
+__global__ void __launch_bounds__(1024) b_quotient(const BatchSys *__restrict__ sys,
+                                                   const double *__restrict__ hcp,
+                                                   const double *__restrict__ scp,
+                                                   const double *__restrict__ bra,
+                                                   double *__restrict__ HC, double *__restrict__ SC,
+                                                   double *__restrict__ scal,
+                                                   double *__restrict__ f, int *__restrict__ flag) {
+  __shared__ double rh[1024], rs[1024];
+  const int s = blockIdx.x;
+  const BatchSys &b = sys[s];
+  int bs = 0, bc = 0;
+  SWITCH_N(b.n, (bs = Dim<NN>::BS, bc = Dim<NN>::B_C));
+  quotient_block(hcp + b.hcp_off, scp + b.hcp_off, b.nchunk, bra + b.bra_off, bs, bc, b.K,
+                 HC + b.vec_off, SC + b.vec_off, scal + 8 * (size_t)s, f ? f + s : nullptr, flag,
+                 rh, rs);
+}
+
+// one lane per (system, slot, function)
+__global__ void b_tile_reduce(const BatchSys *__restrict__ sys, int nsys,
+                              const long long *__restrict__ rpre, const double *__restrict__ part,
+                              double *__restrict__ red) {
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= rpre[nsys]) return;
+  const int s = seg_find(rpre, nsys, idx);
+  const BatchSys &b = sys[s];
+  const int r = (int)(idx - rpre[s]);
+  const int v = r / b.K, k = r - v * b.K;
+  tile_reduce_one(part + b.part_off, b.ntiles, b.K, b.nacc, v, k, red + b.red_off);
+}
+
+// one lane per (system, function)
+__global__ void b_grad_final(const BatchSys *__restrict__ sys, int nsys,
+                             const long long *__restrict__ fpre, const double *__restrict__ red,
+                             const double *__restrict__ bra, const double *__restrict__ scal,
+                             const double *__restrict__ HC, const double *__restrict__ SC,
+                             double *__restrict__ g) {
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= fpre[nsys]) return;
+  const int s = seg_find(fpre, nsys, idx);
+  const BatchSys &b = sys[s];
+  const int k = (int)(idx - fpre[s]);
+  SWITCH_N(b.n, (grad_one<NN, MODE_SPLIT>(red + b.red_off, b.K, k, bra + b.bra_off,
+                                          scal + 8 * (size_t)s, HC + b.vec_off, SC + b.vec_off,
+                                          1.0, g + b.x_off)));
+}
+
+template <int N>
+void launch_pair(int grid, hipStream_t st, const BatchSys *sys, const Item *items, int nitems,
+                 const double *bra, const double *ket, double *part, double *hpart,
+                 double *spart, int *flag) {
+  hipLaunchKernelGGL((b_pair<N>), dim3(grid), dim3(64 * pair_waves(N)), 0, st, sys, items,
+                     nitems, bra, ket, part, hpart, spart, flag);
+}
+
+int hip_rc(hipError_t e) {
+  if (e == hipSuccess) return NECG_OK;
+  if (e == hipErrorOutOfMemory) return NECG_ERR_OOM;
+  return NECG_ERR_DEVICE;
+}
+
+template <class T>
+int dalloc(T **p, size_t count) {
+  if (*p) {
+    (void)hipFree(*p);
+    *p = nullptr;
+  }
+  if (count == 0) return NECG_OK;
+  return hip_rc(hipMalloc((void **)p, count * sizeof(T)));
+}
+
+// relative cost of one ordered (pair, term): F_E + F_G of the FLOP model
+double term_cost(int n) {
+  switch (n) {
+    case 1: return 200.0;
+    case 2: return 1321.0;
+    case 3: return 3005.0;
+    case 4: return 6000.0;
+    case 5: return 9672.0;
+    default: return 15000.0;
+  }
+}
+
+int dim_bs(int n) { int r = 0; SWITCH_N(n, r = Dim<NN>::BS); return r; }
+int dim_ks(int n) { int r = 0; SWITCH_N(n, r = Dim<NN>::KS); return r; }
+int dim_blk(int n) { int r = 0; SWITCH_N(n, r = Dim<NN>::BLK); return r; }
+int dim_nacc(int n) { int r = 0; SWITCH_N(n, r = Dim<NN>::NACC); return r; }
+
+}  // namespace
+
+struct necg_batch {
+  int device = 0, nsys = 0, ncu = 0;
+  std::vector<BatchSys> hsys;
+  std::vector<long long> fpre, kpre, hpre, rpre;  // host copies of the prefixes
+  BatchSys *d_sys = nullptr;
+  long long *d_fpre = nullptr, *d_kpre = nullptr, *d_hpre = nullptr, *d_rpre = nullptr;
+  double *sml = nullptr, *sms = nullptr, *scvv = nullptr;
+  double *bra = nullptr, *ket = nullptr, *H = nullptr, *S = nullptr, *hpart = nullptr,
+         *spart = nullptr, *part = nullptr, *red = nullptr, *HC = nullptr, *SC = nullptr,
+         *hcp = nullptr, *scp = nullptr, *scal = nullptr;
+  int *flag = nullptr;
+  // pair work lists per n
+  struct Cls {
+    int n = 0;
+    Item *d_items = nullptr;
+    int nitems = 0;
+    int grid = 0;
+  };
+  std::vector<Cls> cls;
+  double item_flops = 0.0;  // target cost of one pair tile (model FLOP)
+  int grid_per_cu = 0;      // persistent workgroups per CU (0: occupancy)
+  long long nx = 0;
+  std::mutex mu;
+  std::vector<int> in_n, in_nst, in_K;
+  std::vector<std::vector<double>> in_sml, in_sms, in_scvv;
+};
+
+namespace {
+
+// Tiles every system into items of about `target` model FLOP and builds the
+// per-n work lists, heaviest items first.
+int plan(necg_batch *B) {
+  int rc;
+  const double target = B->item_flops > 0 ? B->item_flops : 6.0e7;
+  std::vector<std::vector<Item>> lists(kMaxN + 1);
+  std::vector<std::vector<double>> costs(kMaxN + 1);
+  long long part_tot = 0, hp_tot = 0;
+  for (int s = 0; s < B->nsys; ++s) {
+    BatchSys &b = B->hsys[s];
+    const int K = b.K, nst = b.nst;
+    const int nb = std::min(K, 64);
+    const double per_ket = nb * nst * term_cost(b.n);
+    int kt = (int)std::max(1.0, std::floor(target / per_ket));
+    kt = std::min(kt, K);
+    int tps = nst;
+    if (kt == 1) {  // a single ket is already above target: split the terms
+      tps = (int)std::max(1.0, std::ceil(target / (nb * term_cost(b.n))));
+      tps = std::min(tps, nst);
+    }
+    const int ts = (nst + tps - 1) / tps;
+    const int kts = (K + kt - 1) / kt;
+    b.ts = ts;
+    b.tps = tps;
+    b.ntiles = kts * ts;
+    b.part_off = part_tot;
+    part_tot += (long long)b.ntiles * b.nacc * K;
+    b.hp_off = hp_tot;
+    hp_tot += (long long)ts * K * K;
+    for (int bt = 0; bt < (K + 63) / 64; ++bt)
+      for (int y = 0; y < kts; ++y)
+        for (int z = 0; z < ts; ++z) {
+          Item it;
+          it.sys = s;
+          it.k0 = bt * 64;
+          it.l0 = y * kt;
+          it.l1 = std::min(K, it.l0 + kt);
+          it.p0 = z * tps;
+          it.p1 = std::min(nst, it.p0 + tps);
+          it.tile = y * ts + z;
+          const double c = (double)std::min(64, K - it.k0) * (it.l1 - it.l0) * (it.p1 - it.p0) *
+                           term_cost(b.n);
+          lists[b.n].push_back(it);
+          costs[b.n].push_back(c);
+        }
+  }
+  if ((rc = dalloc(&B->part, (size_t)part_tot))) return rc;
+  if ((rc = dalloc(&B->hpart, (size_t)hp_tot))) return rc;
+  if ((rc = dalloc(&B->spart, (size_t)hp_tot))) return rc;
+  for (auto &c : B->cls)
+    if (c.d_items) (void)hipFree(c.d_items);
+  B->cls.clear();
+  for (int n = 1; n <= kMaxN; ++n) {
+    if (lists[n].empty()) continue;
+    std::vector<size_t> ord(lists[n].size());
+    std::iota(ord.begin(), ord.end(), 0);
+    std::stable_sort(ord.begin(), ord.end(),
+                     [&](size_t a, size_t b) { return costs[n][a] > costs[n][b]; });
+    std::vector<Item> sorted;
+    for (size_t i : ord) sorted.push_back(lists[n][i]);
+    necg_batch::Cls c;
+    c.n = n;
+    c.nitems = (int)sorted.size();
+    if ((rc = dalloc(&c.d_items, sorted.size()))) return rc;
+    if ((rc = hip_rc(hipMemcpy(c.d_items, sorted.data(), sorted.size() * sizeof(Item),
+                               hipMemcpyHostToDevice))))
+      return rc;
+    int per_cu = B->grid_per_cu;
+    if (per_cu <= 0) {
+      int occ = 1;
+      const void *fn = nullptr;
+      SWITCH_N(n, fn = (const void *)&b_pair<NN>);
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, 64 * pair_waves(n), 0) !=
+              hipSuccess ||
+          occ < 1)
+        occ = 1;
+      per_cu = occ;
+    }
+    c.grid = std::max(1, std::min(c.nitems, per_cu * B->ncu));
+    B->cls.push_back(c);
+  }
+  if ((rc = hip_rc(hipMemcpy(B->d_sys, B->hsys.data(), B->hsys.size() * sizeof(BatchSys),
+                             hipMemcpyHostToDevice))))
+    return rc;
+  return NECG_OK;
+}
+
+struct DevGuard {
+  int prev = -1;
+  explicit DevGuard(int d) {
+    if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+    if (prev != d) (void)hipSetDevice(d);
+  }
+  ~DevGuard() {
+    if (prev >= 0) (void)hipSetDevice(prev);
+  }
+};
+
+int upload_prefix(long long **d, const std::vector<long long> &h) {
+  int rc = dalloc(d, h.size());
+  if (rc) return rc;
+  return hip_rc(hipMemcpy(*d, h.data(), h.size() * sizeof(long long), hipMemcpyHostToDevice));
+}
+
+}  // namespace
+
+extern "C" {
+
+int necg_batch_destroy(necg_batch *B) {
+  if (!B) return NECG_ERR_ARG;
+  {
+    DevGuard g(B->device);
+    double *bufs[] = {B->sml, B->sms, B->scvv, B->bra, B->ket, B->H, B->S, B->hpart, B->spart,
+                      B->part, B->red, B->HC, B->SC, B->hcp, B->scp, B->scal};
+    for (double *p : bufs)
+      if (p) (void)hipFree(p);
+    long long *pre[] = {B->d_fpre, B->d_kpre, B->d_hpre, B->d_rpre};
+    for (long long *p : pre)
+      if (p) (void)hipFree(p);
+    if (B->d_sys) (void)hipFree(B->d_sys);
+    if (B->flag) (void)hipFree(B->flag);
+    for (auto &c : B->cls)
+      if (c.d_items) (void)hipFree(c.d_items);
+  }
+  delete B;
+  return NECG_OK;
+}
+
+int necg_batch_create(const necg_system *systems, int32_t nsys, const int32_t *Ks, int device,
+                      necg_batch **out) {
+  if (!systems || !Ks || !out || nsys < 1) return NECG_ERR_ARG;
+  *out = nullptr;
+  for (int s = 0; s < nsys; ++s) {
+    const necg_system &y = systems[s];
+    if (y.n < 1 || y.n > kMaxN) return NECG_ERR_UNSUPPORTED;
+    if (y.nst < 1 || Ks[s] < 1 || !y.mass || !y.ncv || !y.scvv || !y.sml || !y.sms)
+      return NECG_ERR_ARG;
+    for (int i = 0; i < y.n; ++i)
+      for (int j = 0; j < y.n; ++j)
+        if (y.mass[i * y.n + j] != y.mass[j * y.n + i]) return NECG_ERR_ARG;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev)
+    return NECG_ERR_DEVICE;
+  necg_batch *B = new (std::nothrow) necg_batch();
+  if (!B) return NECG_ERR_OOM;
+  B->device = device;
+  B->nsys = nsys;
+  DevGuard g(device);
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) {
+    necg_batch_destroy(B);
+    return NECG_ERR_DEVICE;
+  }
+  B->ncu = prop.multiProcessorCount;
+  // host-side system records and offsets
+  std::vector<double> h_sml, h_sms, h_scvv;
+  long long xo = 0, bo = 0, ko = 0, hso = 0, ro = 0, vo = 0, hco = 0;
+  B->fpre.assign(1, 0);
+  B->kpre.assign(1, 0);
+  B->hpre.assign(1, 0);
+  B->rpre.assign(1, 0);
+  for (int s = 0; s < nsys; ++s) {
+    const necg_system &y = systems[s];
+    const int n = y.n, K = Ks[s];
+    BatchSys b{};
+    b.sc.n = n;
+    b.sc.nst = y.nst;
+    b.sc.efs = y.efs;
+    for (int i = 0; i < n * n; ++i) b.sc.mass[i] = y.mass[i];
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j <= i; ++j) b.sc.msym[sx(i, j)] = y.mass[i * n + j];
+    for (int i = 0; i <= n; ++i) b.sc.q[i] = y.ncv[i];
+    int pi = 0;
+    for (int p = 0; p <= n; ++p)
+      for (int q = p + 1; q <= n; ++q) b.sc.cpair[pi++] = y.ncv[p] * y.ncv[q];
+    b.n = n;
+    b.nst = y.nst;
+    b.K = K;
+    b.nacc = dim_nacc(n);
+    b.nchunk = (K + 255) / 256;
+    b.x_off = xo;
+    xo += (long long)K * (dim_blk(n) + 1);
+    b.bra_off = bo;
+    bo += (long long)K * dim_bs(n);
+    b.ket_off = ko;
+    ko += (long long)K * y.nst * dim_ks(n);
+    b.hs_off = hso;
+    hso += (long long)K * K;
+    b.red_off = ro;
+    ro += (long long)b.nacc * K;
+    b.vec_off = vo;
+    vo += K;
+    b.hcp_off = hco;
+    hco += (long long)b.nchunk * K;
+    b.sml_off = (long long)h_sml.size();
+    h_sml.insert(h_sml.end(), y.sml, y.sml + (size_t)y.nst * n * n);
+    b.sms_off = (long long)h_sms.size();
+    h_sms.insert(h_sms.end(), y.sms, y.sms + (size_t)y.nst * 9 * n * n);
+    b.scvv_off = (long long)h_scvv.size();
+    h_scvv.insert(h_scvv.end(), y.scvv, y.scvv + y.nst);
+    B->hsys.push_back(b);
+    B->fpre.push_back(B->fpre.back() + K);
+    B->kpre.push_back(B->kpre.back() + (long long)K * y.nst);
+    B->hpre.push_back(B->hpre.back() + (long long)K * b.nchunk);
+    B->rpre.push_back(B->rpre.back() + (long long)b.nacc * K);
+  }
+  B->nx = xo;
+  int rc = NECG_OK;
+  if ((rc = dalloc(&B->sml, h_sml.size())) || (rc = dalloc(&B->sms, h_sms.size())) ||
+      (rc = dalloc(&B->scvv, h_scvv.size())) || (rc = dalloc(&B->bra, (size_t)bo)) ||
+      (rc = dalloc(&B->ket, (size_t)ko)) || (rc = dalloc(&B->H, (size_t)hso)) ||
+      (rc = dalloc(&B->S, (size_t)hso)) || (rc = dalloc(&B->red, (size_t)ro)) ||
+      (rc = dalloc(&B->HC, (size_t)vo)) || (rc = dalloc(&B->SC, (size_t)vo)) ||
+      (rc = dalloc(&B->hcp, (size_t)hco)) || (rc = dalloc(&B->scp, (size_t)hco)) ||
+      (rc = dalloc(&B->scal, (size_t)8 * nsys)) || (rc = dalloc(&B->d_sys, (size_t)nsys)) ||
+      (rc = dalloc(&B->flag, 1)) || (rc = upload_prefix(&B->d_fpre, B->fpre)) ||
+      (rc = upload_prefix(&B->d_kpre, B->kpre)) || (rc = upload_prefix(&B->d_hpre, B->hpre)) ||
+      (rc = upload_prefix(&B->d_rpre, B->rpre))) {
+    necg_batch_destroy(B);
+    return rc;
+  }
+  if ((rc = hip_rc(hipMemcpy(B->sml, h_sml.data(), h_sml.size() * 8, hipMemcpyHostToDevice))) ||
+      (rc = hip_rc(hipMemcpy(B->sms, h_sms.data(), h_sms.size() * 8, hipMemcpyHostToDevice))) ||
+      (rc = hip_rc(hipMemcpy(B->scvv, h_scvv.data(), h_scvv.size() * 8,
+                             hipMemcpyHostToDevice))) ||
+      (rc = hip_rc(hipMemset(B->flag, 0, sizeof(int)))) || (rc = plan(B))) {
+    necg_batch_destroy(B);
+    return rc;
+  }
+  *out = B;
+  return NECG_OK;
+}
+
+int necg_batch_layout(necg_batch *B, int64_t *x_offsets) {
+  if (!B || !x_offsets) return NECG_ERR_ARG;
+  for (int s = 0; s < B->nsys; ++s) x_offsets[s] = B->hsys[s].x_off;
+  x_offsets[B->nsys] = B->nx;
+  return NECG_OK;
+}
+
+int necg_batch_set_tiling(necg_batch *B, double item_flops, int32_t grid_per_cu) {
+  if (!B || item_flops < 0 || grid_per_cu < 0) return NECG_ERR_ARG;
+  std::lock_guard<std::mutex> lk(B->mu);
+  DevGuard g(B->device);
+  B->item_flops = item_flops;
+  B->grid_per_cu = grid_per_cu;
+  return plan(B);
+}
+
+int necg_batch_outprod(necg_batch *B, const double *x, double *f, double *g, void *stream) {
+  if (!B || !x || !g) return NECG_ERR_ARG;
+  std::lock_guard<std::mutex> lk(B->mu);
+  DevGuard dg(B->device);
+  hipStream_t st = (hipStream_t)stream;
+  const int n = B->nsys;
+  const int tb = 256;
+  auto blocks = [](long long count, int t) { return (unsigned)((count + t - 1) / t); };
+  hipLaunchKernelGGL(b_prep_bra, dim3(blocks(B->fpre[n], tb)), dim3(tb), 0, st, B->d_sys, n,
+                     B->d_fpre, x, B->bra, B->flag);
+  hipLaunchKernelGGL(b_prep_ket, dim3(blocks(B->kpre[n], tb)), dim3(tb), 0, st, B->d_sys, n,
+                     B->d_kpre, B->sml, B->sms, B->scvv, B->bra, B->ket);
+  for (const auto &c : B->cls) {
+    SWITCH_N(c.n, launch_pair<NN>(c.grid, st, B->d_sys, c.d_items, c.nitems, B->bra, B->ket,
+                                  B->part, B->hpart, B->spart, B->flag));
+  }
+  hipLaunchKernelGGL(b_hs_finish, dim3((unsigned)B->hpre[n]), dim3(256), 0, st, B->d_sys, n,
+                     B->d_hpre, B->hpart, B->spart, B->bra, B->H, B->S, B->hcp, B->scp);
+  hipLaunchKernelGGL(b_quotient, dim3(n), dim3(1024), 0, st, B->d_sys, B->hcp, B->scp, B->bra,
+                     B->HC, B->SC, B->scal, f, B->flag);
+  hipLaunchKernelGGL(b_tile_reduce, dim3(blocks(B->rpre[n], tb)), dim3(tb), 0, st, B->d_sys, n,
+                     B->d_rpre, B->part, B->red);
+  hipLaunchKernelGGL(b_grad_final, dim3(blocks(B->fpre[n], tb)), dim3(tb), 0, st, B->d_sys, n,
+                     B->d_fpre, B->red, B->bra, B->scal, B->HC, B->SC, g);
+  return hip_rc(hipGetLastError());
+}
+
+int necg_batch_matrices(necg_batch *B, int32_t s, double *H, double *S, void *stream) {
+  if (!B || s < 0 || s >= B->nsys) return NECG_ERR_ARG;
+  std::lock_guard<std::mutex> lk(B->mu);
+  DevGuard dg(B->device);
+  const BatchSys &b = B->hsys[s];
+  const size_t bytes = sizeof(double) * (size_t)b.K * b.K;
+  hipStream_t st = (hipStream_t)stream;
+  if (H && hipMemcpyAsync(H, B->H + b.hs_off, bytes, hipMemcpyDeviceToDevice, st) != hipSuccess)
+    return NECG_ERR_DEVICE;
+  if (S && hipMemcpyAsync(S, B->S + b.hs_off, bytes, hipMemcpyDeviceToDevice, st) != hipSuccess)
+    return NECG_ERR_DEVICE;
+  return NECG_OK;
+}
+
+int necg_batch_status(necg_batch *B, void *stream) {
+  if (!B) return NECG_ERR_ARG;
+  std::lock_guard<std::mutex> lk(B->mu);
+  DevGuard dg(B->device);
+  if (hipStreamSynchronize((hipStream_t)stream) != hipSuccess) return NECG_ERR_DEVICE;
+  int flags = 0;
+  if (hipMemcpy(&flags, B->flag, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
+    return NECG_ERR_DEVICE;
+  if (flags && hipMemset(B->flag, 0, sizeof(int)) != hipSuccess) return NECG_ERR_DEVICE;
+  if (flags & F_BAD_INPUT) return NECG_ERR_NONFINITE;
+  if (flags & F_NOT_PD) return NECG_ERR_NOT_PD;
+  if (flags & F_CSC) return NECG_ERR_CSC;
+  if (flags & F_NONFINITE) return NECG_ERR_NONFINITE;
+  return NECG_OK;
+}
+
+}  // extern "C"

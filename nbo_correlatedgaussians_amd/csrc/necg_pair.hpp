@@ -491,6 +491,81 @@ __device__ __forceinline__ bool pair_term(const Bra &b, const double *__restrict
   return ok;
 }
 
+// latched device errors; necg_status reports the root cause first:
+// bad input, then a singular pair matrix, then C^T S C <= 0, then NaN results
+enum FlagBits { F_NOT_PD = 1, F_NONFINITE = 2, F_CSC = 4, F_BAD_INPUT = 8 };
+
+// Waves per pair-kernel workgroup: they share the 64 bra records in LDS and
+// each owns a private set of accumulator slots, so the sums are formed in a
+// fixed order (bitwise reproducible for a given launch shape).  LDS per
+// workgroup: (B_PK+1)*512 B + W*NACC*512 B = 151 KB for n = 5, W = 4.
+__host__ __device__ constexpr int pair_waves(int n) { return n <= 5 ? 4 : 2; }
+
+template <int N, int MODE>
+struct PairLds {
+  static constexpr int BRA = (Dim<N>::B_PK + 1) * 64;
+  static constexpr int NA = (MODE == MODE_ELEMENTS) ? 1 : Dim<N>::NACC * 64;
+  static constexpr int ACC = pair_waves(N) * NA;
+};
+
+// One pair tile: the 64 bras [k0, k0+64) x kets [l0, l1) x terms [p0, p1),
+// run by the pair_waves(N) waves of a workgroup.  The bra records go to LDS
+// ([field][lane]); each wave owns a set of accumulator slots ([slot][lane])
+// and takes alternate kets; ket records are wave-uniform (scalar loads).
+// Writes the element partials of bra >= ket to hp/sp ([l][k], K x K) and the
+// tile's summed accumulators to pt ([slot][k], NACC x K).
+template <int N, int MODE>
+__device__ __forceinline__ void pair_tile(const double *__restrict__ bra,
+                                          const double *__restrict__ ket, int K, int nst, int k0,
+                                          int l0, int l1, int p0, int p1, const SysConst &sc,
+                                          double eshift, double *__restrict__ pt,
+                                          double *__restrict__ hp, double *__restrict__ sp,
+                                          int *__restrict__ flag, double *bra_s, double *acc_s) {
+  using D = Dim<N>;
+  constexpr int W = pair_waves(N);
+  constexpr int NA = PairLds<N, MODE>::NA;
+  const int lane = threadIdx.x & 63;
+  // wave index: uniform across the wave, so that the ket index and every
+  // ket-record address are scalar (s_load into SGPRs)
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int k = k0 + lane;
+  const bool active = k < K;
+  const int kk = active ? k : K - 1;
+  for (int v = wv; v <= D::B_PK; v += W) bra_s[v * 64 + lane] = bra[(size_t)kk * D::BS + v];
+  if (MODE != MODE_ELEMENTS)
+    for (int v = 0; v < D::NACC; ++v) acc_s[wv * NA + v * 64 + lane] = 0.0;
+  __syncthreads();
+  const BraLds<N> b{bra_s + lane};
+  LdsSink sink{acc_s + wv * NA + lane};
+  bool ok = true;
+  for (int l = l0 + wv; l < l1; l += W) {
+    double hel = 0.0, sel = 0.0;
+    const double *kl = ket + (size_t)l * nst * D::KS;
+    for (int p = p0; p < p1; ++p) {
+      // keep the bra in LDS: re-read it for every term instead of holding
+      // 47 doubles in registers across the loop
+      __asm__ volatile("" ::: "memory");
+      ok &= pair_term<N, MODE>(b, kl + (size_t)p * D::KS, sc, eshift, hel, sel, sink, nullptr);
+    }
+    if (MODE != MODE_COMBINED && active && k >= l) {
+      const size_t o = (size_t)l * K + k;
+      hp[o] = hel;
+      sp[o] = sel;
+    }
+  }
+  if (active && !ok) atomicOr(flag, (int)F_NOT_PD);
+  __syncthreads();
+  if (MODE == MODE_ELEMENTS || !active) return;
+  double *o = pt + k;
+  for (int v = wv; v < D::NACC; v += W) {
+    if (MODE == MODE_COMBINED && v >= D::A_XS && v != D::A_SH) continue;
+    double t = 0.0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) t += acc_s[w * NA + v * 64 + lane];
+    o[(size_t)v * K] = t;
+  }
+}
+
 // vech(X L) + 1.5 s diag(1/L_ii) for symmetric X (packed) and lower L (vech
 // order, column-major): the vech(L) block of one gradient.
 template <int N>

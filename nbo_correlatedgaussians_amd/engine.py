@@ -48,6 +48,7 @@ class Engine:
         _lib.raise_for(rc, "necg_create")
         self._ctx = ctx
         self.blk = system.blk
+        self.energy_generation = 0  # bumped by every energy_t (autograd state check)
 
     # -- plumbing ----------------------------------------------------------
     def close(self):
@@ -129,6 +130,7 @@ class Engine:
         rc = self.lib.necg_energy(self._ctx, K, self._ptr(x), self._ptr(f), self._ptr(H),
                                   self._ptr(S), self._stream())
         _lib.raise_for(rc, "necg_energy")
+        self.energy_generation += 1
         return (f, H, S) if with_hs else f
 
     def vjp_t(self, x, gbar=None):
