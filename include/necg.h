@@ -146,6 +146,36 @@ int necg_batch_set_tiling(necg_batch *batch, double item_flops, int32_t grid_per
 /* As necg_status, for a batch.  Blocking. */
 int necg_batch_status(necg_batch *batch, void *stream);
 
+/* ---- serving: single-function replacement trials on resident systems ---- */
+
+typedef struct necg_pool necg_pool;
+
+/* A pool over contexts that have each completed one evaluation (necg_outprod
+ * or necg_energy) at their working K; the pool reads their resident H, S,
+ * H C, S C, C^T H C and C^T S C.  Trials are refused (NECG_ERR_ARG) if a
+ * context was re-dimensioned since.  No counterpart in the reference: it has
+ * no serving mode; a trial's row is the matel loop (python/outprod.py:49-54)
+ * restricted to one row and its energy the quotient of python/outprod.py:65-68. */
+int necg_pool_create(necg_ctx *const *ctxs, int32_t nctx, necg_pool **out);
+int necg_pool_destroy(necg_pool *pool);
+
+/* Largest K in the pool and the per-trial workspace stride (doubles). */
+int necg_pool_sizes(necg_pool *pool, int32_t *kmax, int32_t *ws_stride);
+
+/* ntr trials: trial t replaces function tfun[t] (0-based) of pool system
+ * tsys[t] by the block newp[t*pstride ..] = [vech L | s] and yields the new
+ * row of H and S (rowH/rowS[t*rstride + j], rstride >= kmax) and
+ * E[t] = C^T H' C / C^T S' C with the resident C.  The resident state is not
+ * modified.  All pointers are device buffers owned by the caller (ws:
+ * ntr*ws_stride doubles), so concurrent calls on different streams are safe
+ * while no evaluation of the same contexts runs.  Async. */
+int necg_pool_trials(necg_pool *pool, int32_t ntr, const int32_t *tsys, const int32_t *tfun,
+                     const double *newp, int32_t pstride, double *ws, double *E, double *rowH,
+                     double *rowS, int32_t rstride, void *stream);
+
+/* As necg_status, for the pool's trials.  Blocking. */
+int necg_pool_status(necg_pool *pool, void *stream);
+
 /* Human-readable text of a status code. */
 const char *necg_strerror(int code);
 

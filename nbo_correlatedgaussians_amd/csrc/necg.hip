@@ -24,7 +24,7 @@
 #include <vector>
 
 #include "../../include/necg.h"
-#include "necg_finish.hpp"
+#include "necg_ctx.hpp"
 
 using namespace necg;
 
@@ -223,28 +223,7 @@ __global__ void dhds_kernel(const double *__restrict__ bra, const double *__rest
 
 // =================================================================== host ===
 
-struct necg_ctx {
-  int device = 0;
-  int n = 0, nst = 0;
-  SysConst sc{};
-  double *d_scvv = nullptr, *d_sml = nullptr, *d_sms = nullptr;
-  int *d_flag = nullptr;
-  // per-K workspace
-  int Kcap = 0;
-  double *bra = nullptr, *ket = nullptr, *H = nullptr, *S = nullptr, *HC = nullptr,
-         *SC = nullptr, *scal = nullptr;
-  double *part = nullptr, *hpart = nullptr, *spart = nullptr;
-  double *hcp = nullptr, *scp = nullptr, *red = nullptr;
-  size_t part_cap = 0, hpart_cap = 0;
-  int req_kt = 0, req_ts = 0, req_w = 0;
-  int last_kt = 0, last_ts = 0, last_w = 0;
-  int last_K = 0;
-  // pair-kernel timing (necg_timing)
-  bool timing = false;
-  std::vector<hipEvent_t> ev;  // start/stop pairs, reused
-  int nev = 0;
-  std::mutex mu;
-};
+// necg_ctx is defined in necg_ctx.hpp
 
 namespace {
 

@@ -1,0 +1,36 @@
+// necg_ctx.hpp -- the single-system context behind the necg_* C ABI (shared
+// by the library's translation units; not part of the public header).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+#include <vector>
+
+#include "../../include/necg.h"
+#include "necg_finish.hpp"
+
+struct necg_ctx {
+  // resident state of the last evaluation on this context: bra/ket tables,
+  // H, S, HC, SC and scal = (E, 1/cSc, cHc, cSc) of x at K = last_K
+  int device = 0;
+  int n = 0, nst = 0;
+  necg::SysConst sc{};
+  double *d_scvv = nullptr, *d_sml = nullptr, *d_sms = nullptr;
+  int *d_flag = nullptr;
+  // per-K workspace
+  int Kcap = 0;
+  double *bra = nullptr, *ket = nullptr, *H = nullptr, *S = nullptr, *HC = nullptr,
+         *SC = nullptr, *scal = nullptr;
+  double *part = nullptr, *hpart = nullptr, *spart = nullptr;
+  double *hcp = nullptr, *scp = nullptr, *red = nullptr;
+  size_t part_cap = 0, hpart_cap = 0;
+  int req_kt = 0, req_ts = 0, req_w = 0;
+  int last_kt = 0, last_ts = 0, last_w = 0;
+  int last_K = 0;
+  // pair-kernel timing (necg_timing)
+  bool timing = false;
+  std::vector<hipEvent_t> ev;  // start/stop pairs, reused
+  int nev = 0;
+  std::mutex mu;
+};
