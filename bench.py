@@ -58,6 +58,69 @@ def cpu_baseline(sysd, K, x, rows, threads):
     }
 
 
+def batched_leg(reps=5):
+    """64 independent systems (He/H2/LiH, K log-uniform in [32, 1024], seeds
+    100..163) evaluated together (autotuned tiles, HIP-graph replay) against
+    the plain per-system path (one outprod per system, back to back)."""
+    import torch
+    from nbo_correlatedgaussians_amd.batch import Batch, BatchGraph, autotune, mixed_systems
+    from nbo_correlatedgaussians_amd.engine import Engine
+    systems, Ks, xs = mixed_systems()
+    b = Batch(systems, Ks, "cuda:0")
+    xa = b.pack(xs)
+    best, _ = autotune(b, xa, reps=2)
+    G = BatchGraph(b, xa)
+    for _ in range(2):
+        G.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        G.replay()
+    torch.cuda.synchronize()
+    tb = (time.perf_counter() - t0) / reps
+    b.check("batched leg")
+    engs = [Engine(s, "cuda:0") for s in systems]
+    xds = [torch.as_tensor(x, device="cuda:0") for x in xs]
+    for e, x in zip(engs, xds):
+        e.outprod_t(x)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        for e, x in zip(engs, xds):
+            e.outprod_t(x)
+    torch.cuda.synchronize()
+    tp = (time.perf_counter() - t0) / reps
+    pairs = b.pairs
+    return {"workload": "64 systems He/H2/LiH, K log-uniform [32,1024], seeds 100..163; "
+                        "energy+gradient of all",
+            "pairs": pairs, "ms": tb * 1e3, "pairs_per_s": pairs / tb,
+            "plain_ms": tp * 1e3, "plain_pairs_per_s": pairs / tp, "speedup": tp / tb,
+            "tiling": {"item_flops": best[0], "grid_per_cu": best[1]}}
+
+
+def served_leg(nreq, rate):
+    """32 resident systems (He/H2/LiH, K log-uniform in [32, 4096], seeds
+    200..231); nreq requests with Poisson arrivals at `rate`/s, 70 % single-
+    function replacement trials, 30 % full energy+gradient evaluations."""
+    from nbo_correlatedgaussians_amd.batch import mixed_systems
+    from nbo_correlatedgaussians_amd.serve import Server, latency_stats, poisson_trace
+    systems, Ks, xs = mixed_systems(count=32, seed0=200, kmax=4096)
+    srv = Server(systems, xs, "cuda:0")
+    st = latency_stats(srv.run(poisson_trace(systems, srv.Ks, nreq, rate=rate, seed=7)))
+    sat = latency_stats(srv.run(poisson_trace(systems, srv.Ks, nreq, rate=1e9, seed=7)))
+    tpl, ppl = srv.run_plain(poisson_trace(systems, srv.Ks, nreq, rate=1e9, seed=7))
+    out = {"workload": f"32 resident systems He/H2/LiH, K log-uniform [32,4096], seeds "
+                       f"200..231; {nreq} requests, Poisson arrivals at {rate:g}/s, 70% "
+                       f"single-function replacement trials, 30% full energy+gradient",
+           "rate_per_s": rate}
+    out.update({k: v for k, v in st.items() if k != "requests"})
+    out["saturated_pairs_per_s"] = sat["pairs_per_s"]
+    out["saturated_p99_ms"] = sat["p99_ms"]
+    out["plain_pairs_per_s"] = ppl / tpl
+    out["speedup_vs_plain"] = sat["pairs_per_s"] / (ppl / tpl)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -67,6 +130,10 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cpu-rows", type=int, default=48)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=10000)
+    ap.add_argument("--serve-rate", type=float, default=1000.0)
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -136,6 +203,12 @@ def main():
                      "tiling": {"ket_tile": kt, "term_split": ts, "waves": w}},
         "energy": float(f.item()),
     }
+    if not args.no_batch:
+        out["batched"] = batched_leg()
+        print(f"bench: batched leg done", file=sys.stderr, flush=True)
+    if not args.no_serve:
+        out["served"] = served_leg(args.serve_requests, args.serve_rate)
+        print(f"bench: served leg done", file=sys.stderr, flush=True)
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(sysd, K, x, args.cpu_rows, threads)

@@ -81,6 +81,12 @@ int necg_energy_vjp(necg_ctx *ctx, int32_t K, const double *x, const double *gba
  * iv has K*blk entries (no C).  Device.  Async. */
 int necg_matrices(necg_ctx *ctx, int32_t K, const double *iv, double *H, double *S, void *stream);
 
+/* Kinetic and potential matrices (K*K row-major, symmetric):
+ * T = sum_P scvv tk, V = sum_P scvv ert (python/necg.py:43-57; the Stark
+ * term enters H only).  x is the full parameter vector.  Device.  Async. */
+int necg_kinetic_potential(necg_ctx *ctx, int32_t K, const double *x, double *T, double *V,
+                           void *stream);
+
 /* Per-term elements: for query t, (stf, i, j) = q[3t..3t+2] (0-based term,
  * bra, ket) gives out[4t..4t+3] = (ov, tk, ert, zht).  Replaces
  * matel(stf, ii, jj) (python/matel.py:28, 1-based there).  Device.  Async. */

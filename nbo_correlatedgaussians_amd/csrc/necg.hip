@@ -78,7 +78,6 @@ __global__ void __launch_bounds__(64 * pair_waves(N), NECG_MIN_WAVES) pair_kerne
     int tps, SysConst sc, const double *__restrict__ scal, double *__restrict__ part,
     double *__restrict__ hpart, double *__restrict__ spart, int *__restrict__ flag) {
   using D = Dim<N>;
-  __shared__ double bra_s[PairLds<N, MODE>::BRA];
   __shared__ double acc_s[PairLds<N, MODE>::ACC];
   const int l0 = blockIdx.y * kt;
   const int p0 = blockIdx.z * tps;
@@ -86,7 +85,7 @@ __global__ void __launch_bounds__(64 * pair_waves(N), NECG_MIN_WAVES) pair_kerne
   pair_tile<N, MODE>(bra, ket, K, nst, blockIdx.x * 64, l0, min(K, l0 + kt), p0,
                      min(nst, p0 + tps), sc, (MODE == MODE_COMBINED) ? scal[0] : 0.0,
                      part + tile * D::NACC * K, hpart + (size_t)blockIdx.z * K * K,
-                     spart + (size_t)blockIdx.z * K * K, flag, bra_s, acc_s);
+                     spart + (size_t)blockIdx.z * K * K, flag, acc_s);
 }
 
 // Full symmetric H and S and the per-chunk partials of H C and S C.
@@ -364,7 +363,7 @@ int run_pairs(necg_ctx *c, int K, hipStream_t st, int &ntiles, int &ts_out) {
   ntiles = kts * ts;
   ts_out = ts;
   int rc;
-  if (MODE != MODE_ELEMENTS) {
+  if (mode_has_grad(MODE)) {
     if ((rc = ensure_part(c, (size_t)ntiles * Dim<N>::NACC * K))) return rc;
   }
   if (MODE != MODE_COMBINED) {
@@ -393,10 +392,13 @@ int run_pairs(necg_ctx *c, int K, hipStream_t st, int &ntiles, int &ts_out) {
   return hip_status(hipGetLastError());
 }
 
-int finish_hs(necg_ctx *c, int K, int ts, hipStream_t st) {
+// Full symmetric matrices from the element partials, into H and S (the
+// context's own by default).
+int finish_hs(necg_ctx *c, int K, int ts, hipStream_t st, double *H = nullptr,
+              double *S = nullptr) {
   const int bs = bra_stride(c->n), bc = bra_c_off(c->n);
   hipLaunchKernelGGL(hs_finish, dim3((K + 255) / 256, K), dim3(256), 0, st, c->hpart, c->spart,
-                     K, ts, c->bra, bs, bc, c->H, c->S, c->hcp, c->scp);
+                     K, ts, c->bra, bs, bc, H ? H : c->H, S ? S : c->S, c->hcp, c->scp);
   return hip_status(hipGetLastError());
 }
 
@@ -460,6 +462,14 @@ int do_matrices(necg_ctx *c, int K, const double *iv, double *H, double *S, hipS
   if (H) HIP_TRY(hipMemcpyAsync(H, c->H, sizeof(double) * K * K, hipMemcpyDeviceToDevice, st));
   if (S) HIP_TRY(hipMemcpyAsync(S, c->S, sizeof(double) * K * K, hipMemcpyDeviceToDevice, st));
   return NECG_OK;
+}
+
+template <int N>
+int do_tv(necg_ctx *c, int K, const double *x, double *T, double *V, hipStream_t st) {
+  int rc, ntiles, ts;
+  if ((rc = run_prep<N>(c, K, x, 1, st))) return rc;
+  if ((rc = run_pairs<N, MODE_TV>(c, K, st, ntiles, ts))) return rc;
+  return finish_hs(c, K, ts, st, T, V);
 }
 
 template <int N>
@@ -627,6 +637,16 @@ int necg_matrices(necg_ctx *c, int32_t K, const double *iv, double *H, double *S
   int rc = ensure_K(c, K);
   if (rc) return rc;
   DISPATCH_N(c->n, (do_matrices<NN>(c, K, iv, H, S, (hipStream_t)stream)));
+}
+
+int necg_kinetic_potential(necg_ctx *c, int32_t K, const double *x, double *T, double *V,
+                           void *stream) {
+  if (!c || K < 1 || !x || !T || !V) return NECG_ERR_ARG;
+  std::lock_guard<std::mutex> lk(c->mu);
+  DeviceGuard dg(c->device);
+  int rc = ensure_K(c, K);
+  if (rc) return rc;
+  DISPATCH_N(c->n, (do_tv<NN>(c, K, x, T, V, (hipStream_t)stream)));
 }
 
 int necg_matel(necg_ctx *c, int32_t K, const double *iv, int32_t nq, const int32_t *q,

@@ -101,7 +101,6 @@ __global__ void __launch_bounds__(64 * pair_waves(N), 1) b_pair(
     const double *__restrict__ bra, const double *__restrict__ ket, double *__restrict__ part,
     double *__restrict__ hpart, double *__restrict__ spart, int *__restrict__ flag) {
   using D = Dim<N>;
-  __shared__ double bra_s[PairLds<N, MODE_SPLIT>::BRA];
   __shared__ double acc_s[PairLds<N, MODE_SPLIT>::ACC];
   for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
     const Item w = items[it];
@@ -111,7 +110,7 @@ __global__ void __launch_bounds__(64 * pair_waves(N), 1) b_pair(
     pair_tile<N, MODE_SPLIT>(bra + b.bra_off, ket + b.ket_off, K, b.nst, w.k0, w.l0, w.l1, w.p0,
                              w.p1, b.sc, 0.0, part + b.part_off + (size_t)w.tile * D::NACC * K,
                              hpart + b.hp_off + (size_t)z * K * K,
-                             spart + b.hp_off + (size_t)z * K * K, flag, bra_s, acc_s);
+                             spart + b.hp_off + (size_t)z * K * K, flag, acc_s);
     __syncthreads();  // the next item overwrites the LDS tables
   }
 }

@@ -49,21 +49,6 @@ constexpr double kC1 = 0.564691197e-04, kC2 = 0.758433197e-03, kC3 = 0.769838037
 
 constexpr int kMaxN = 6;
 
-// Scheduling fence between the phases of one pair term.  The fully unrolled
-// term is one basic block; without fences the machine scheduler interleaves
-// independent phases (notably the Coulomb pairs) for ILP and the live set
-// grows far beyond the register file.
-#ifndef NECG_FENCE_MASK
-#define NECG_FENCE_MASK 0
-#endif
-#ifndef NECG_NO_FENCES
-#define NECG_FENCE() do { __asm__ volatile("" ::: "memory"); __builtin_amdgcn_sched_barrier(NECG_FENCE_MASK); } while (0)
-#else
-#define NECG_FENCE() ((void)0)
-#endif
-#ifndef NECG_PAIR_FENCE_EVERY
-#define NECG_PAIR_FENCE_EVERY 1
-#endif
 
 __host__ __device__ constexpr int sx(int i, int j) {
   return i >= j ? i * (i + 1) / 2 + j : j * (j + 1) / 2 + i;
@@ -159,9 +144,7 @@ __device__ __forceinline__ void boys_f0(double z, double &f, double &fp) {
   fp = big ? -0.5 * fa * (v * v) : 0.5 * __builtin_fma(Q, dP, -P * dQ) * v;
 }
 
-// Bra-side data of one basis function.  BraReg holds it in registers (query
-// kernels); BraLds reads it from an LDS table laid out [field][64 lanes] (pair
-// kernels, where the waves of a workgroup share 64 bras).
+// Bra-side data of one basis function, held in registers.
 template <int N>
 struct BraReg {
   double a[Dim<N>::B_PK + 1];
@@ -178,15 +161,6 @@ __device__ __forceinline__ void load_bra(BraReg<N> &b, const double *__restrict_
   for (int t = 0; t <= Dim<N>::B_PK; ++t) b.a[t] = rec[t];
 }
 
-template <int N>
-struct BraLds {
-  const double *p;  // &table[0 * 64 + lane]
-  __device__ __forceinline__ double A(int t) const { return p[64 * (Dim<N>::B_A + t)]; }
-  __device__ __forceinline__ double S(int t) const { return p[64 * (Dim<N>::B_S + t)]; }
-  __device__ __forceinline__ double AS(int t) const { return p[64 * (Dim<N>::B_AS + t)]; }
-  __device__ __forceinline__ double i1() const { return p[64 * Dim<N>::B_I1]; }
-  __device__ __forceinline__ double pk() const { return p[64 * Dim<N>::B_PK]; }
-};
 
 // Accumulator sinks.  Slot layout (Dim<N>::A_*): X_H | Y_H | X_S | Y_S | sH | sS.
 // MODE 0 keeps the H and S parts apart (E not yet known); MODE 1 folds them
@@ -215,7 +189,15 @@ struct Parts {
   double ov, tk, ert, zht;
 };
 
-enum { MODE_SPLIT = 0, MODE_COMBINED = 1, MODE_ELEMENTS = 2 };
+// MODE_SPLIT: elements + gradient with H and S parts apart; MODE_COMBINED:
+// gradient with the energy folded in; MODE_ELEMENTS: H and S only;
+// MODE_TV: the kinetic and potential matrices (python/necg.py:43-50) in place
+// of H and S.
+enum { MODE_SPLIT = 0, MODE_COMBINED = 1, MODE_ELEMENTS = 2, MODE_TV = 3 };
+
+__host__ __device__ constexpr bool mode_has_grad(int mode) {
+  return mode == MODE_SPLIT || mode == MODE_COMBINED;
+}
 
 // One (bra, ket-under-term) contribution.  kd points at the ket-table record.
 // Adds sc*ov*alpha to hel and sc*ov to sel; for the gradient modes adds the
@@ -281,7 +263,6 @@ __device__ __forceinline__ bool pair_term(const Bra &b, const double *__restrict
       for (int m = i; m < N; ++m) t = __builtin_fma(Ri[sx(m, i)], Ri[sx(m, j)], t);
       Bi[sx(i, j)] = t;
     }
-  NECG_FENCE();
   double det32 = 1.0;  // det(B)^-3/2
 #pragma unroll
   for (int j = 0; j < N; ++j) det32 *= rd[j] * rd[j] * rd[j];
@@ -302,7 +283,6 @@ __device__ __forceinline__ bool pair_term(const Bra &b, const double *__restrict
       i3 = __builtin_fma(Em[3 * i + x], t, i3);
     }
   const double ov = b.pk() * pl * det32 * exp(i3 - b.i1() - i2);
-  NECG_FENCE();
 
   // Kinetic (python/matel.py:144-155).  a1 = A dif1; the gradient needs
   // only u = 8 A'M a1 from it, formed here so that dif1, a1 and A'M are dead
@@ -330,7 +310,7 @@ __device__ __forceinline__ bool pair_term(const Bra &b, const double *__restrict
 #pragma unroll
         for (int j = 0; j < N; ++j) {
           t = __builtin_fma(sc_.msym[sx(i, j)], a1[3 * j + x], t);
-          if (MODE != MODE_ELEMENTS) v = __builtin_fma(ApM[i * N + j], a1[3 * j + x], v);
+          if (mode_has_grad(MODE)) v = __builtin_fma(ApM[i * N + j], a1[3 * j + x], v);
         }
         i3k = __builtin_fma(-a1[3 * i + x], t, i3k);
         u8[3 * i + x] = 8.0 * v;
@@ -348,7 +328,6 @@ __device__ __forceinline__ bool pair_term(const Bra &b, const double *__restrict
     }
   T2 *= 3.0;
   const double tfac = 2.0 * i3k + T2;
-  NECG_FENCE();
 
   // Coulomb (python/matel.py:162-197; gradients python/gradel.py:266-337).
   double csum = 0.0;  // sum_c c f r0
@@ -358,6 +337,8 @@ __device__ __forceinline__ bool pair_term(const Bra &b, const double *__restrict
     La[t] = 0.0;
     Lb[t] = 0.0;
   }
+  (void)La;
+  (void)Lb;
   {
     int pi = 0;
 #pragma unroll
@@ -385,7 +366,7 @@ __device__ __forceinline__ bool pair_term(const Bra &b, const double *__restrict
         const double c = sc_.cpair[pi];
         const double cr0 = c * ri * kInvSqrtPi;  // c / sqrt(pi i2p)
         csum = __builtin_fma(cr0, f, csum);
-        if (MODE != MODE_ELEMENTS) {
+        if (mode_has_grad(MODE)) {
           const double cb = 4.0 * fp * cr0 * inv2;
           const double ca = __builtin_fma(cb, z, 2.0 * f * cr0 * inv2);
           if (p == 0) {
@@ -400,7 +381,6 @@ __device__ __forceinline__ bool pair_term(const Bra &b, const double *__restrict
             Lb[sx(a, bq)] -= cb;
           }
         }
-        if ((pi + 1) % NECG_PAIR_FENCE_EVERY == 0) NECG_FENCE();
       }
   }
   // Stark (python/matel.py:205-208)
@@ -408,15 +388,21 @@ __device__ __forceinline__ bool pair_term(const Bra &b, const double *__restrict
 #pragma unroll
   for (int i = 0; i < N; ++i) zsum = __builtin_fma(sc_.q[i + 1], st[3 * i + 2], zsum);
   const double alpha = 2.0 * tfac + 2.0 * csum - sc_.efs * zsum;
-  hel = __builtin_fma(scv * ov, alpha, hel);
-  sel = __builtin_fma(scv, ov, sel);
+  if (MODE == MODE_TV) {
+    // Kinetic += scvv tk, Potential += scvv ert (the Stark term is in H only)
+    hel = __builtin_fma(scv * ov, 2.0 * tfac, hel);
+    sel = __builtin_fma(scv * ov, 2.0 * csum, sel);
+  } else {
+    hel = __builtin_fma(scv * ov, alpha, hel);
+    sel = __builtin_fma(scv, ov, sel);
+  }
   if (parts) {
     parts->ov = ov;
     parts->tk = 2.0 * ov * tfac;
     parts->ert = 2.0 * ov * csum;
     parts->zht = ov * zsum * sc_.efs;
   }
-  if (MODE == MODE_ELEMENTS) return ok;
+  if (!mode_has_grad(MODE)) return ok;
 
   // Gradient.  F^T = 4 A'M (A dif1 - A' dif2) + Lb st - efs q e_z^T
   //                = 8 A'M a1 + Lb st - efs q e_z^T;  Phi = B^-1 F^T.
@@ -431,7 +417,6 @@ __device__ __forceinline__ bool pair_term(const Bra &b, const double *__restrict
       if (x == 2) t = __builtin_fma(-sc_.efs, sc_.q[i + 1], t);
       Ft[3 * i + x] = t;
     }
-  NECG_FENCE();
   double Phi[3 * N];
 #pragma unroll
   for (int i = 0; i < N; ++i)
@@ -442,7 +427,6 @@ __device__ __forceinline__ bool pair_term(const Bra &b, const double *__restrict
       for (int m = 0; m < N; ++m) t = __builtin_fma(Bi[sx(i, m)], Ft[3 * m + x], t);
       Phi[3 * i + x] = t;
     }
-  NECG_FENCE();
   // G = B^-1 (12 A'MA' + La) B^-1 (symmetric)
   double Tm[N * N];
 #pragma unroll
@@ -455,7 +439,6 @@ __device__ __forceinline__ bool pair_term(const Bra &b, const double *__restrict
         t = __builtin_fma(__builtin_fma(12.0, Np[sx(i, m)], La[sx(i, m)]), Bi[sx(m, j)], t);
       Tm[i * N + j] = t;
     }
-  NECG_FENCE();
   double d1[3 * N];
 #pragma unroll
   for (int t = 0; t < 3 * N; ++t) d1[t] = st[t] - b.S(t);
@@ -495,23 +478,23 @@ __device__ __forceinline__ bool pair_term(const Bra &b, const double *__restrict
 // bad input, then a singular pair matrix, then C^T S C <= 0, then NaN results
 enum FlagBits { F_NOT_PD = 1, F_NONFINITE = 2, F_CSC = 4, F_BAD_INPUT = 8 };
 
-// Waves per pair-kernel workgroup: they share the 64 bra records in LDS and
-// each owns a private set of accumulator slots, so the sums are formed in a
+// Waves per pair-kernel workgroup: they work on the same 64 bras and each
+// owns a private set of accumulator slots in LDS, so the sums are formed in a
 // fixed order (bitwise reproducible for a given launch shape).  LDS per
-// workgroup: (B_PK+1)*512 B + W*NACC*512 B = 151 KB for n = 5, W = 4.
+// workgroup: W*NACC*512 B = 124 KB for n = 5, W = 4.
 __host__ __device__ constexpr int pair_waves(int n) { return n <= 5 ? 4 : 2; }
 
 template <int N, int MODE>
 struct PairLds {
-  static constexpr int BRA = (Dim<N>::B_PK + 1) * 64;
-  static constexpr int NA = (MODE == MODE_ELEMENTS) ? 1 : Dim<N>::NACC * 64;
+  static constexpr int NA = mode_has_grad(MODE) ? Dim<N>::NACC * 64 : 1;
   static constexpr int ACC = pair_waves(N) * NA;
 };
 
 // One pair tile: the 64 bras [k0, k0+64) x kets [l0, l1) x terms [p0, p1),
-// run by the pair_waves(N) waves of a workgroup.  The bra records go to LDS
-// ([field][lane]); each wave owns a set of accumulator slots ([slot][lane])
-// and takes alternate kets; ket records are wave-uniform (scalar loads).
+// run by the pair_waves(N) waves of a workgroup.  Each lane holds its bra
+// record in registers; each wave owns a set of accumulator slots in LDS
+// ([slot][lane]) and takes alternate kets; ket records are wave-uniform
+// (scalar loads).
 // Writes the element partials of bra >= ket to hp/sp ([l][k], K x K) and the
 // tile's summed accumulators to pt ([slot][k], NACC x K).
 template <int N, int MODE>
@@ -520,7 +503,7 @@ __device__ __forceinline__ void pair_tile(const double *__restrict__ bra,
                                           int l0, int l1, int p0, int p1, const SysConst &sc,
                                           double eshift, double *__restrict__ pt,
                                           double *__restrict__ hp, double *__restrict__ sp,
-                                          int *__restrict__ flag, double *bra_s, double *acc_s) {
+                                          int *__restrict__ flag, double *acc_s) {
   using D = Dim<N>;
   constexpr int W = pair_waves(N);
   constexpr int NA = PairLds<N, MODE>::NA;
@@ -531,22 +514,22 @@ __device__ __forceinline__ void pair_tile(const double *__restrict__ bra,
   const int k = k0 + lane;
   const bool active = k < K;
   const int kk = active ? k : K - 1;
-  for (int v = wv; v <= D::B_PK; v += W) bra_s[v * 64 + lane] = bra[(size_t)kk * D::BS + v];
-  if (MODE != MODE_ELEMENTS)
+  if (mode_has_grad(MODE))
     for (int v = 0; v < D::NACC; ++v) acc_s[wv * NA + v * 64 + lane] = 0.0;
   __syncthreads();
-  const BraLds<N> b{bra_s + lane};
+  // The bra record is held in registers for the whole tile.  (Re-reading it
+  // from an LDS table for every term, interleaved with the LDS accumulator
+  // adds and the scalar ket loads, gave wrong element values in the gradient
+  // modes on gfx950; that layout is not used.)
+  BraReg<N> b;
+  load_bra<N>(b, bra + (size_t)kk * D::BS);
   LdsSink sink{acc_s + wv * NA + lane};
   bool ok = true;
   for (int l = l0 + wv; l < l1; l += W) {
     double hel = 0.0, sel = 0.0;
     const double *kl = ket + (size_t)l * nst * D::KS;
-    for (int p = p0; p < p1; ++p) {
-      // keep the bra in LDS: re-read it for every term instead of holding
-      // 47 doubles in registers across the loop
-      __asm__ volatile("" ::: "memory");
+    for (int p = p0; p < p1; ++p)
       ok &= pair_term<N, MODE>(b, kl + (size_t)p * D::KS, sc, eshift, hel, sel, sink, nullptr);
-    }
     if (MODE != MODE_COMBINED && active && k >= l) {
       const size_t o = (size_t)l * K + k;
       hp[o] = hel;
@@ -555,7 +538,7 @@ __device__ __forceinline__ void pair_tile(const double *__restrict__ bra,
   }
   if (active && !ok) atomicOr(flag, (int)F_NOT_PD);
   __syncthreads();
-  if (MODE == MODE_ELEMENTS || !active) return;
+  if (!mode_has_grad(MODE) || !active) return;
   double *o = pt + k;
   for (int v = wv; v < D::NACC; v += W) {
     if (MODE == MODE_COMBINED && v >= D::A_XS && v != D::A_SH) continue;

@@ -154,6 +154,17 @@ class Engine:
         _lib.raise_for(rc, "necg_matrices")
         return H, S
 
+    def kinetic_potential_t(self, x):
+        """Kinetic and potential matrices (python/necg.py:36-58) for x."""
+        x = self._dev(x)
+        K = self.K_of(x.numel())
+        T = self._empty(K, K)
+        V = self._empty(K, K)
+        rc = self.lib.necg_kinetic_potential(self._ctx, K, self._ptr(x), self._ptr(T),
+                                             self._ptr(V), self._stream())
+        _lib.raise_for(rc, "necg_kinetic_potential")
+        return T, V
+
     # -- synchronous NumPy API ---------------------------------------------
     @staticmethod
     def _host_x(x):

@@ -283,6 +283,25 @@ class Server:
             e.check("serving")
         return reqs
 
+    def run_plain(self, requests):
+        """The same requests one at a time, each launched alone and waited
+        for (no batching, no overlap): the per-request GPU path that the
+        scheduler is compared with.  Returns (seconds, pairs)."""
+        torch = _torch()
+        t0 = time.perf_counter()
+        pairs = 0
+        for r in requests:
+            K = self.Ks[r.sys]
+            if r.kind == "trial":
+                L = self._launch_trials(0, [Request(r.rid, "trial", r.sys, 0.0, r.func, r.newp)])
+                pairs += K
+            else:
+                L = self._launch_full(0, Request(r.rid, "full", r.sys, 0.0))
+                pairs += K * (K + 1) // 2
+            L.event.synchronize()
+        torch.cuda.synchronize(self.device)
+        return time.perf_counter() - t0, pairs
+
     def trial_alone(self, r: Request):
         """One trial launched by itself (reference for the in-mix result)."""
         torch = _torch()

@@ -1,0 +1,89 @@
+"""The steps around the hot path: generalised eigenpair on the device (against
+scipy.linalg.eigh), kinetic/potential matrices and energetics (against the
+oracle's per-term kinetic and Coulomb values), and the optimiser driven by the
+GPU outprod (against the same optimiser driven by the oracle)."""
+
+import numpy as np
+import pytest
+from scipy.linalg import eigh
+
+import nbo_correlatedgaussians_amd as necg
+from conftest import golden, system_by_name
+
+pytestmark = pytest.mark.gpu
+
+
+def test_lowest_eigenpair_vs_scipy():
+    from nbo_correlatedgaussians_amd.driver import lowest_eigenpair
+    d = golden("lih_oldquit")
+    lam, v = lowest_eigenpair(d["H"], d["S"])
+    w, V = eigh(d["H"], d["S"])
+    assert abs(lam - w[0]) <= 1e-9 * max(1.0, abs(w[0]))
+    v = v.cpu().numpy()
+    v0 = V[:, 0] * np.sign(V[:, 0] @ d["S"] @ v)
+    assert np.max(np.abs(v - v0)) <= 1e-7 * np.max(np.abs(v0))
+
+
+def test_kinetic_potential_and_energetics(oracle_lib):
+    from nbo_correlatedgaussians_amd.driver import energetics
+    from nbo_correlatedgaussians_amd.engine import Engine
+    s = necg.lih_system()
+    K = 8
+    iv, C = necg.generate_parameters(s, K, 31)
+    x = necg.pack_x(iv, C)
+    eng = Engine(s, "cuda:0")
+    T, V = (t.cpu().numpy() for t in eng.kinetic_potential_t(x))
+    o = oracle_lib.Oracle(s)
+    To = np.zeros((K, K))
+    Vo = np.zeros((K, K))
+    Zo = np.zeros((K, K))
+    for i in range(K):
+        for j in range(i + 1):
+            for p in range(s.nst):
+                ov, tk, ert, zht = o.matel(iv, p, i, j)
+                To[i, j] += s.scvv[p] * tk
+                Vo[i, j] += s.scvv[p] * ert
+                Zo[i, j] += s.scvv[p] * zht
+            To[j, i], Vo[j, i], Zo[j, i] = To[i, j], Vo[i, j], Zo[i, j]
+    assert np.max(np.abs(T - To)) <= 1e-12 * np.max(np.abs(To))
+    assert np.max(np.abs(V - Vo)) <= 1e-12 * np.max(np.abs(Vo))
+    en = energetics(eng, x)
+    E, g = eng.outprod(x)
+    assert abs(en["energy"] - E) <= 1e-13 * abs(E)
+    cSc = C @ (o.matrices(iv, K)[1]) @ C
+    assert abs(en["kinetic"] - C @ To @ C / cSc) <= 1e-12 * abs(en["kinetic"])
+    assert abs(en["potential"] - C @ Vo @ C / cSc) <= 1e-12 * abs(en["potential"])
+    assert en["virial"] == -2.0 * en["kinetic"] / en["potential"]
+
+
+@pytest.mark.parametrize("method", ["L-BFGS-B", "TN"])
+def test_optimiser_trajectory_matches_oracle(oracle_lib, method):
+    """Config 0 (LiH K = 12, seed 0): the optimiser driven by the GPU
+    outprod follows the oracle-driven trajectory.  Truncated Newton forms
+    Hessian-vector products from gradient differences over a ~sqrt(eps) step,
+    which amplifies 1e-12 gradient differences, so only its first outer
+    iteration is compared tightly; L-BFGS-B is compared over two."""
+    from nbo_correlatedgaussians_amd.driver import optimize
+    from nbo_correlatedgaussians_amd.engine import Engine
+    d = golden("lih_k12_s0")
+    s = system_by_name(d["system"])
+    x0 = d["x"]
+    eng = Engine(s, "cuda:0")
+    o = oracle_lib.Oracle(s)
+    rg = optimize(eng.outprod, x0, method=method, maxiter=4, siv=12 * s.blk, M=12)
+    ro = optimize(o.outprod, x0, method=method, maxiter=4, siv=12 * s.blk, M=12)
+    assert rg.nit == ro.nit >= 1
+    assert rg.energies[0] < float(d["E"])
+    ncmp = 2 if method == "L-BFGS-B" else 1
+    for a, b in zip(rg.energies[:ncmp], ro.energies[:ncmp]):
+        assert abs(a - b) <= 1e-8 * max(1.0, abs(b))
+    assert rg.energies[-1] < rg.energies[0]
+
+
+def test_driver_run_small():
+    from nbo_correlatedgaussians_amd.driver import run
+    s = necg.he_system()
+    iv, _ = necg.generate_parameters(s, 10, 41)
+    out = run(s, iv, device="cuda:0", method="L-BFGS-B", maxiter=5)
+    assert out["final"]["energy"] <= out["initial"]["energy"] + 1e-12
+    assert abs(out["initial"]["energy"] - out["eigenvalue"]) <= 1e-8 * max(1.0, abs(out["eigenvalue"]))

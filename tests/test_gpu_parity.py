@@ -205,3 +205,24 @@ def test_error_paths():
     # the latch is cleared: a good call afterwards succeeds
     E, g = eng.outprod(necg.pack_x(iv, C))
     assert math.isfinite(E)
+
+
+@pytest.mark.parametrize("case", ["lih_oldquit", "lih_k32_s2"])
+def test_tilings_agree_and_repeat(case):
+    """Every launch shape (ket tile, term groups) gives the golden energy and
+    gradient, and repeated calls are bitwise identical."""
+    from nbo_correlatedgaussians_amd.engine import Engine
+    d = golden(case)
+    eng = Engine(system_by_name(d["system"]), "cuda:0")
+    for kt, ts in ((0, 0), (1, 24), (4, 6), (16, 1), (1, 1), (3, 5)):
+        eng.set_tiling(kt, ts, 0)
+        runs = [eng.outprod(d["x"]) for _ in range(3)]
+        for E, g in runs:
+            assert close_E(E, float(d["E"])), (kt, ts, E)
+            assert close_g(g, d["g"]), (kt, ts)
+        assert all(E == runs[0][0] and np.array_equal(g, runs[0][1]) for E, g in runs), (kt, ts)
+        Ee, H, S = eng.energy(d["x"], with_hs=True)
+        assert close_E(Ee, float(d["E"]))
+        gx = eng.vjp_t(d["x"]).cpu().numpy()
+        eng.check()
+        assert close_g(gx, d["g"]), (kt, ts)
