@@ -105,7 +105,7 @@ def served_leg(nreq, rate):
     from nbo_correlatedgaussians_amd.batch import mixed_systems
     from nbo_correlatedgaussians_amd.serve import Server, latency_stats, poisson_trace
     systems, Ks, xs = mixed_systems(count=32, seed0=200, kmax=4096)
-    srv = Server(systems, xs, "cuda:0")
+    srv = Server(systems, xs, "cuda:0", tune=True)
     st = latency_stats(srv.run(poisson_trace(systems, srv.Ks, nreq, rate=rate, seed=7)))
     sat = latency_stats(srv.run(poisson_trace(systems, srv.Ks, nreq, rate=1e9, seed=7)))
     tpl, ppl = srv.run_plain(poisson_trace(systems, srv.Ks, nreq, rate=1e9, seed=7))
@@ -118,7 +118,44 @@ def served_leg(nreq, rate):
     out["saturated_p99_ms"] = sat["p99_ms"]
     out["plain_pairs_per_s"] = ppl / tpl
     out["speedup_vs_plain"] = sat["pairs_per_s"] / (ppl / tpl)
+    out["tiling"] = {"item_flops": srv.batch.tiling[0]}
     return out
+
+
+def traffic_probe(K, seed):
+    """For the rocprofv3 counter passes: a 256 MiB device copy (the
+    calibration of FETCH_SIZE / WRITE_SIZE) followed by two outprod calls."""
+    import torch
+    import nbo_correlatedgaussians_amd as necg
+    from nbo_correlatedgaussians_amd.engine import Engine
+    a = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda:0")
+    b = torch.empty_like(a)
+    for _ in range(2):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    s = necg.lih_system()
+    iv, C = necg.generate_parameters(s, K, seed)
+    eng = Engine(s, "cuda:0")
+    xd = torch.as_tensor(necg.pack_x(iv, C), device="cuda:0")
+    for _ in range(2):
+        eng.outprod_t(xd)
+    torch.cuda.synchronize()
+    eng.check("traffic probe")
+    print("traffic probe done", flush=True)
+
+
+def recorded_traffic():
+    """HBM bytes per pair-kernel launch measured by the counter passes
+    (profiles/pair_kernel_lih_k2048_traffic.md), or None."""
+    path = os.path.join(ROOT, "profiles", "pair_kernel_lih_k2048_traffic.md")
+    try:
+        with open(path, encoding="utf-8") as fh:
+            for line in fh:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
 
 
 def main():
@@ -134,7 +171,11 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=10000)
     ap.add_argument("--serve-rate", type=float, default=1000.0)
+    ap.add_argument("--traffic-probe", action="store_true")
     args = ap.parse_args()
+    if args.traffic_probe:
+        traffic_probe(args.K, args.seed)
+        return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(f"bench.py: this build measures one GPU only (asked for {args.gpus} GPUs, "
@@ -198,7 +239,9 @@ def main():
                                                  "peaks are both 78.6 TFLOP/s)",
                      "kernel": "pair_kernel<5,MODE_SPLIT>",
                      "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "frac": achieved / PEAK_FP64_TFLOPS,
+                     "traffic": recorded_traffic() if (K == 2048 and args.seed == 1) else None,
+                     "traffic_source": "profiles/pair_kernel_lih_k2048_traffic.md",
                      "flop_per_launch": fl, "pair_kernel_ms": pair_ms,
                      "tiling": {"ket_tile": kt, "term_split": ts, "waves": w}},
         "energy": float(f.item()),

@@ -141,6 +141,13 @@ int necg_batch_layout(necg_batch *batch, int64_t *x_offsets);
  * hipGraph.  x, f, g: device.  Async. */
 int necg_batch_outprod(necg_batch *batch, const double *x, double *f, double *g, void *stream);
 
+/* As necg_batch_outprod for the systems s with active[s] != 0 only (active:
+ * device, nsys entries); the others are skipped and keep their results and
+ * resident state.  Evaluations with disjoint active sets may run concurrently
+ * on different streams.  Async. */
+int necg_batch_outprod_masked(necg_batch *batch, const int32_t *active, const double *x,
+                              double *f, double *g, void *stream);
+
 /* H and S (K_s x K_s) of system s from the last evaluation.  Device.  Async. */
 int necg_batch_matrices(necg_batch *batch, int32_t s, double *H, double *S, void *stream);
 
@@ -178,6 +185,10 @@ int necg_pool_sizes(necg_pool *pool, int32_t *kmax, int32_t *ws_stride);
 int necg_pool_trials(necg_pool *pool, int32_t ntr, const int32_t *tsys, const int32_t *tfun,
                      const double *newp, int32_t pstride, double *ws, double *E, double *rowH,
                      double *rowS, int32_t rstride, void *stream);
+
+/* A pool over the resident state of every system of a batch (after one full
+ * evaluation of the batch).  The batch must outlive the pool. */
+int necg_batch_pool(necg_batch *batch, necg_pool **out);
 
 /* As necg_status, for the pool's trials.  Blocking. */
 int necg_pool_status(necg_pool *pool, void *stream);

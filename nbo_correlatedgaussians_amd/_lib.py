@@ -28,7 +28,8 @@ EXPORTS = ("necg_create", "necg_destroy", "necg_outprod", "necg_energy",
            "necg_dhds", "necg_status", "necg_set_tiling", "necg_get_tiling",
            "necg_timing", "necg_strerror", "necg_batch_create", "necg_batch_destroy",
            "necg_batch_layout", "necg_batch_outprod", "necg_batch_matrices",
-           "necg_batch_set_tiling", "necg_batch_status", "necg_pool_create",
+           "necg_batch_set_tiling", "necg_batch_status", "necg_batch_outprod_masked",
+           "necg_batch_pool", "necg_pool_create",
            "necg_pool_destroy", "necg_pool_sizes", "necg_pool_trials", "necg_pool_status")
 
 
@@ -103,6 +104,8 @@ def load() -> ct.CDLL:
         "necg_batch_matrices": [P, I, P, P, P],
         "necg_batch_set_tiling": [P, D, I],
         "necg_batch_status": [P, P],
+        "necg_batch_outprod_masked": [P, P, P, P, P, P],
+        "necg_batch_pool": [P, ct.POINTER(P)],
         "necg_pool_create": [P, I, ct.POINTER(P)],
         "necg_pool_destroy": [P],
         "necg_pool_sizes": [P, ct.POINTER(I), ct.POINTER(I)],

@@ -111,19 +111,36 @@ class Batch:
     def check(self, where: str = ""):
         _lib.raise_for(self.lib.necg_batch_status(self._h, self._stream()), where)
 
-    def outprod_t(self, x, f=None, g=None):
+    def outprod_t(self, x, f=None, g=None, active=None):
         """All energies f (nsys,) and gradients g (batch layout) for a device
-        x in the batch layout.  Asynchronous."""
+        x in the batch layout.  With ``active`` (device int32, one flag per
+        system) only the flagged systems are evaluated; the entries of f and
+        the segments of g of the others are left as they were.  Asynchronous."""
         torch = _torch()
         if x.numel() != self.nx or x.dtype != torch.float64 or not x.is_contiguous():
             raise ValueError("x must be a contiguous float64 tensor in the batch layout")
         f = torch.empty(len(self), dtype=torch.float64, device=self.device) if f is None else f
         g = torch.empty(self.nx, dtype=torch.float64, device=self.device) if g is None else g
-        rc = self.lib.necg_batch_outprod(self._h, ct.c_void_p(x.data_ptr()),
-                                         ct.c_void_p(f.data_ptr()), ct.c_void_p(g.data_ptr()),
-                                         self._stream())
+        if active is None:
+            rc = self.lib.necg_batch_outprod(self._h, ct.c_void_p(x.data_ptr()),
+                                             ct.c_void_p(f.data_ptr()), ct.c_void_p(g.data_ptr()),
+                                             self._stream())
+        else:
+            if (active.numel() != len(self) or active.dtype != torch.int32
+                    or active.device != self.device):
+                raise ValueError("active must be an int32 device tensor with one flag per system")
+            rc = self.lib.necg_batch_outprod_masked(
+                self._h, ct.c_void_p(active.data_ptr()), ct.c_void_p(x.data_ptr()),
+                ct.c_void_p(f.data_ptr()), ct.c_void_p(g.data_ptr()), self._stream())
         _lib.raise_for(rc, "necg_batch_outprod")
         return f, g
+
+    def pool(self):
+        """Handle of a trial pool over this batch's resident state (valid
+        after one full evaluation; the batch must outlive it)."""
+        h = ct.c_void_p()
+        _lib.raise_for(self.lib.necg_batch_pool(self._h, ct.byref(h)), "necg_batch_pool")
+        return h
 
     def outprod(self, xs):
         """[(E_s, g_s)] for a list of per-system x vectors (synchronous)."""

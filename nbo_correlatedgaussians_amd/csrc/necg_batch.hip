@@ -17,6 +17,7 @@
 
 #include "../../include/necg.h"
 #include "necg_finish.hpp"
+#include "necg_pool_impl.hpp"
 
 using namespace necg;
 
@@ -63,12 +64,16 @@ __device__ __forceinline__ int seg_find(const long long *__restrict__ prefix, in
   }
 
 // one lane per (system, function)
+// active (nullable): per-system flags; systems with active[s] == 0 are
+// skipped by every kernel of the evaluation and keep their previous state.
 __global__ void b_prep_bra(const BatchSys *__restrict__ sys, int nsys,
-                           const long long *__restrict__ fpre, const double *__restrict__ x,
-                           double *__restrict__ bra, int *__restrict__ flag) {
+                           const long long *__restrict__ fpre, const int32_t *__restrict__ active,
+                           const double *__restrict__ x, double *__restrict__ bra,
+                           int *__restrict__ flag) {
   const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= fpre[nsys]) return;
   const int s = seg_find(fpre, nsys, idx);
+  if (active && !active[s]) return;
   const BatchSys &b = sys[s];
   const int k = (int)(idx - fpre[s]);
   const double *xs = x + b.x_off;
@@ -78,12 +83,14 @@ __global__ void b_prep_bra(const BatchSys *__restrict__ sys, int nsys,
 
 // one lane per (system, ket, term)
 __global__ void b_prep_ket(const BatchSys *__restrict__ sys, int nsys,
-                           const long long *__restrict__ kpre, const double *__restrict__ sml,
-                           const double *__restrict__ sms, const double *__restrict__ scvv,
-                           const double *__restrict__ bra, double *__restrict__ ket) {
+                           const long long *__restrict__ kpre, const int32_t *__restrict__ active,
+                           const double *__restrict__ sml, const double *__restrict__ sms,
+                           const double *__restrict__ scvv, const double *__restrict__ bra,
+                           double *__restrict__ ket) {
   const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= kpre[nsys]) return;
   const int s = seg_find(kpre, nsys, idx);
+  if (active && !active[s]) return;
   const BatchSys &b = sys[s];
   const int li = (int)(idx - kpre[s]);
   const int l = li / b.nst, p = li - l * b.nst;
@@ -98,12 +105,14 @@ __global__ void b_prep_ket(const BatchSys *__restrict__ sys, int nsys,
 template <int N>
 __global__ void __launch_bounds__(64 * pair_waves(N), 1) b_pair(
     const BatchSys *__restrict__ sys, const Item *__restrict__ items, int nitems,
-    const double *__restrict__ bra, const double *__restrict__ ket, double *__restrict__ part,
-    double *__restrict__ hpart, double *__restrict__ spart, int *__restrict__ flag) {
+    const int32_t *__restrict__ active, const double *__restrict__ bra,
+    const double *__restrict__ ket, double *__restrict__ part, double *__restrict__ hpart,
+    double *__restrict__ spart, int *__restrict__ flag) {
   using D = Dim<N>;
   __shared__ double acc_s[PairLds<N, MODE_SPLIT>::ACC];
   for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
     const Item w = items[it];
+    if (active && !active[w.sys]) continue;  // uniform over the workgroup
     const BatchSys &b = sys[w.sys];
     const int K = b.K;
     const int z = w.tile % b.ts;
@@ -118,6 +127,7 @@ __global__ void __launch_bounds__(64 * pair_waves(N), 1) b_pair(
 // one workgroup of 256 lanes per (system, row i, 256-column chunk)
 __global__ void __launch_bounds__(256) b_hs_finish(const BatchSys *__restrict__ sys, int nsys,
                                                    const long long *__restrict__ hpre,
+                                                   const int32_t *__restrict__ active,
                                                    const double *__restrict__ hpart,
                                                    const double *__restrict__ spart,
                                                    const double *__restrict__ bra,
@@ -127,6 +137,7 @@ __global__ void __launch_bounds__(256) b_hs_finish(const BatchSys *__restrict__ 
   __shared__ double red[8];
   const long long blk = blockIdx.x;
   const int s = seg_find(hpre, nsys, blk);
+  if (active && !active[s]) return;  // uniform over the workgroup
   const BatchSys &b = sys[s];
   const int r = (int)(blk - hpre[s]);
   const int i = r / b.nchunk, chunk = r - i * b.nchunk;
@@ -138,6 +149,7 @@ __global__ void __launch_bounds__(256) b_hs_finish(const BatchSys *__restrict__ 
 
 // one workgroup of 1024 lanes per system
 __global__ void __launch_bounds__(1024) b_quotient(const BatchSys *__restrict__ sys,
+                                                   const int32_t *__restrict__ active,
                                                    const double *__restrict__ hcp,
                                                    const double *__restrict__ scp,
                                                    const double *__restrict__ bra,
@@ -146,6 +158,7 @@ __global__ void __launch_bounds__(1024) b_quotient(const BatchSys *__restrict__ 
                                                    double *__restrict__ f, int *__restrict__ flag) {
   __shared__ double rh[1024], rs[1024];
   const int s = blockIdx.x;
+  if (active && !active[s]) return;
   const BatchSys &b = sys[s];
   int bs = 0, bc = 0;
   SWITCH_N(b.n, (bs = Dim<NN>::BS, bc = Dim<NN>::B_C));
@@ -156,11 +169,13 @@ __global__ void __launch_bounds__(1024) b_quotient(const BatchSys *__restrict__ 
 
 // one lane per (system, slot, function)
 __global__ void b_tile_reduce(const BatchSys *__restrict__ sys, int nsys,
-                              const long long *__restrict__ rpre, const double *__restrict__ part,
+                              const long long *__restrict__ rpre,
+                              const int32_t *__restrict__ active, const double *__restrict__ part,
                               double *__restrict__ red) {
   const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= rpre[nsys]) return;
   const int s = seg_find(rpre, nsys, idx);
+  if (active && !active[s]) return;
   const BatchSys &b = sys[s];
   const int r = (int)(idx - rpre[s]);
   const int v = r / b.K, k = r - v * b.K;
@@ -169,13 +184,15 @@ __global__ void b_tile_reduce(const BatchSys *__restrict__ sys, int nsys,
 
 // one lane per (system, function)
 __global__ void b_grad_final(const BatchSys *__restrict__ sys, int nsys,
-                             const long long *__restrict__ fpre, const double *__restrict__ red,
+                             const long long *__restrict__ fpre,
+                             const int32_t *__restrict__ active, const double *__restrict__ red,
                              const double *__restrict__ bra, const double *__restrict__ scal,
                              const double *__restrict__ HC, const double *__restrict__ SC,
                              double *__restrict__ g) {
   const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= fpre[nsys]) return;
   const int s = seg_find(fpre, nsys, idx);
+  if (active && !active[s]) return;
   const BatchSys &b = sys[s];
   const int k = (int)(idx - fpre[s]);
   SWITCH_N(b.n, (grad_one<NN, MODE_SPLIT>(red + b.red_off, b.K, k, bra + b.bra_off,
@@ -185,10 +202,10 @@ __global__ void b_grad_final(const BatchSys *__restrict__ sys, int nsys,
 
 template <int N>
 void launch_pair(int grid, hipStream_t st, const BatchSys *sys, const Item *items, int nitems,
-                 const double *bra, const double *ket, double *part, double *hpart,
-                 double *spart, int *flag) {
+                 const int32_t *active, const double *bra, const double *ket, double *part,
+                 double *hpart, double *spart, int *flag) {
   hipLaunchKernelGGL((b_pair<N>), dim3(grid), dim3(64 * pair_waves(N)), 0, st, sys, items,
-                     nitems, bra, ket, part, hpart, spart, flag);
+                     nitems, active, bra, ket, part, hpart, spart, flag);
 }
 
 int hip_rc(hipError_t e) {
@@ -505,7 +522,8 @@ int necg_batch_set_tiling(necg_batch *B, double item_flops, int32_t grid_per_cu)
   return plan(B);
 }
 
-int necg_batch_outprod(necg_batch *B, const double *x, double *f, double *g, void *stream) {
+int necg_batch_outprod_masked(necg_batch *B, const int32_t *active, const double *x, double *f,
+                              double *g, void *stream) {
   if (!B || !x || !g) return NECG_ERR_ARG;
   std::lock_guard<std::mutex> lk(B->mu);
   DevGuard dg(B->device);
@@ -514,22 +532,49 @@ int necg_batch_outprod(necg_batch *B, const double *x, double *f, double *g, voi
   const int tb = 256;
   auto blocks = [](long long count, int t) { return (unsigned)((count + t - 1) / t); };
   hipLaunchKernelGGL(b_prep_bra, dim3(blocks(B->fpre[n], tb)), dim3(tb), 0, st, B->d_sys, n,
-                     B->d_fpre, x, B->bra, B->flag);
+                     B->d_fpre, active, x, B->bra, B->flag);
   hipLaunchKernelGGL(b_prep_ket, dim3(blocks(B->kpre[n], tb)), dim3(tb), 0, st, B->d_sys, n,
-                     B->d_kpre, B->sml, B->sms, B->scvv, B->bra, B->ket);
+                     B->d_kpre, active, B->sml, B->sms, B->scvv, B->bra, B->ket);
   for (const auto &c : B->cls) {
-    SWITCH_N(c.n, launch_pair<NN>(c.grid, st, B->d_sys, c.d_items, c.nitems, B->bra, B->ket,
-                                  B->part, B->hpart, B->spart, B->flag));
+    SWITCH_N(c.n, launch_pair<NN>(c.grid, st, B->d_sys, c.d_items, c.nitems, active, B->bra,
+                                  B->ket, B->part, B->hpart, B->spart, B->flag));
   }
   hipLaunchKernelGGL(b_hs_finish, dim3((unsigned)B->hpre[n]), dim3(256), 0, st, B->d_sys, n,
-                     B->d_hpre, B->hpart, B->spart, B->bra, B->H, B->S, B->hcp, B->scp);
-  hipLaunchKernelGGL(b_quotient, dim3(n), dim3(1024), 0, st, B->d_sys, B->hcp, B->scp, B->bra,
-                     B->HC, B->SC, B->scal, f, B->flag);
+                     B->d_hpre, active, B->hpart, B->spart, B->bra, B->H, B->S, B->hcp, B->scp);
+  hipLaunchKernelGGL(b_quotient, dim3(n), dim3(1024), 0, st, B->d_sys, active, B->hcp, B->scp,
+                     B->bra, B->HC, B->SC, B->scal, f, B->flag);
   hipLaunchKernelGGL(b_tile_reduce, dim3(blocks(B->rpre[n], tb)), dim3(tb), 0, st, B->d_sys, n,
-                     B->d_rpre, B->part, B->red);
+                     B->d_rpre, active, B->part, B->red);
   hipLaunchKernelGGL(b_grad_final, dim3(blocks(B->fpre[n], tb)), dim3(tb), 0, st, B->d_sys, n,
-                     B->d_fpre, B->red, B->bra, B->scal, B->HC, B->SC, g);
+                     B->d_fpre, active, B->red, B->bra, B->scal, B->HC, B->SC, g);
   return hip_rc(hipGetLastError());
+}
+
+int necg_batch_outprod(necg_batch *B, const double *x, double *f, double *g, void *stream) {
+  return necg_batch_outprod_masked(B, nullptr, x, f, g, stream);
+}
+
+int necg_batch_pool(necg_batch *B, necg_pool **out) {
+  if (!B || !out) return NECG_ERR_ARG;
+  std::vector<PoolSys> recs;
+  for (const BatchSys &b : B->hsys) {
+    PoolSys r{};
+    r.sc = b.sc;
+    r.n = b.n;
+    r.nst = b.nst;
+    r.K = b.K;
+    r.bra = B->bra + b.bra_off;
+    r.H = B->H + b.hs_off;
+    r.S = B->S + b.hs_off;
+    r.HC = B->HC + b.vec_off;
+    r.SC = B->SC + b.vec_off;
+    r.scal = B->scal + 8 * (size_t)(&b - B->hsys.data());
+    r.sml = B->sml + b.sml_off;
+    r.sms = B->sms + b.sms_off;
+    r.scvv = B->scvv + b.scvv_off;
+    recs.push_back(r);
+  }
+  return pool_create_from(B->device, recs, out);
 }
 
 int necg_batch_matrices(necg_batch *B, int32_t s, double *H, double *S, void *stream) {

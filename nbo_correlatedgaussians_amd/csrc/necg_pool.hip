@@ -17,16 +17,11 @@
 #include <vector>
 
 #include "necg_ctx.hpp"
+#include "necg_pool_impl.hpp"
 
 using namespace necg;
 
 namespace {
-
-struct PoolSys {
-  SysConst sc;
-  int n, nst, K;
-  const double *bra, *H, *S, *HC, *SC, *scal, *sml, *sms, *scvv;
-};
 
 #define POOL_SWITCH_N(n, call)                    \
   switch (n) {                                    \
@@ -143,7 +138,7 @@ int pool_ws_stride(const std::vector<PoolSys> &v) {
 
 struct necg_pool {
   int device = 0;
-  std::vector<necg_ctx *> ctx;
+  std::vector<necg_ctx *> ctx;  // empty for a batch-backed pool
   std::vector<PoolSys> hsys;
   PoolSys *d_sys = nullptr;
   int *flag = nullptr;
@@ -177,6 +172,35 @@ bool snapshot(necg_pool *P, std::vector<PoolSys> &out) {
 
 }  // namespace
 
+int necg::pool_create_from(int device, const std::vector<PoolSys> &recs, necg_pool **out) {
+  if (recs.empty() || !out) return NECG_ERR_ARG;
+  necg_pool *P = new (std::nothrow) necg_pool();
+  if (!P) return NECG_ERR_OOM;
+  P->device = device;
+  P->hsys = recs;
+  for (const PoolSys &b : P->hsys) P->kmax = std::max(P->kmax, b.K);
+  P->wstride = pool_ws_stride(P->hsys);
+  int prev = -1;
+  (void)hipGetDevice(&prev);
+  (void)hipSetDevice(device);
+  int rc = NECG_OK;
+  if (hipMalloc((void **)&P->d_sys, sizeof(PoolSys) * P->hsys.size()) != hipSuccess ||
+      hipMalloc((void **)&P->flag, sizeof(int)) != hipSuccess ||
+      hipMemcpy(P->d_sys, P->hsys.data(), sizeof(PoolSys) * P->hsys.size(),
+                hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemset(P->flag, 0, sizeof(int)) != hipSuccess)
+    rc = NECG_ERR_DEVICE;
+  if (prev >= 0) (void)hipSetDevice(prev);
+  if (rc) {
+    if (P->d_sys) (void)hipFree(P->d_sys);
+    if (P->flag) (void)hipFree(P->flag);
+    delete P;
+    return rc;
+  }
+  *out = P;
+  return NECG_OK;
+}
+
 extern "C" {
 
 int necg_pool_destroy(necg_pool *P) {
@@ -194,39 +218,16 @@ int necg_pool_destroy(necg_pool *P) {
 int necg_pool_create(necg_ctx *const *ctxs, int32_t nctx, necg_pool **out) {
   if (!ctxs || nctx < 1 || !out) return NECG_ERR_ARG;
   *out = nullptr;
-  necg_pool *P = new (std::nothrow) necg_pool();
-  if (!P) return NECG_ERR_OOM;
-  P->device = ctxs[0]->device;
+  necg_pool probe;
   for (int i = 0; i < nctx; ++i) {
-    if (!ctxs[i] || ctxs[i]->device != P->device) {
-      delete P;
-      return NECG_ERR_ARG;
-    }
-    P->ctx.push_back(ctxs[i]);
+    if (!ctxs[i] || ctxs[i]->device != ctxs[0]->device) return NECG_ERR_ARG;
+    probe.ctx.push_back(ctxs[i]);
   }
-  if (!snapshot(P, P->hsys)) {  // every context needs one evaluation first
-    delete P;
-    return NECG_ERR_ARG;
-  }
-  for (const PoolSys &b : P->hsys) P->kmax = std::max(P->kmax, b.K);
-  P->wstride = pool_ws_stride(P->hsys);
-  int prev = -1;
-  (void)hipGetDevice(&prev);
-  (void)hipSetDevice(P->device);
-  int rc = NECG_OK;
-  if (hipMalloc((void **)&P->d_sys, sizeof(PoolSys) * P->hsys.size()) != hipSuccess ||
-      hipMalloc((void **)&P->flag, sizeof(int)) != hipSuccess ||
-      hipMemcpy(P->d_sys, P->hsys.data(), sizeof(PoolSys) * P->hsys.size(),
-                hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemset(P->flag, 0, sizeof(int)) != hipSuccess)
-    rc = NECG_ERR_DEVICE;
-  if (prev >= 0) (void)hipSetDevice(prev);
-  if (rc) {
-    necg_pool_destroy(P);
-    return rc;
-  }
-  *out = P;
-  return NECG_OK;
+  std::vector<PoolSys> recs;
+  if (!snapshot(&probe, recs)) return NECG_ERR_ARG;  // every context needs one evaluation first
+  int rc = pool_create_from(ctxs[0]->device, recs, out);
+  if (rc == NECG_OK) (*out)->ctx = probe.ctx;
+  return rc;
 }
 
 int necg_pool_sizes(necg_pool *P, int32_t *kmax, int32_t *ws_stride) {
@@ -243,12 +244,15 @@ int necg_pool_trials(necg_pool *P, int32_t ntr, const int32_t *tsys, const int32
       rstride < P->kmax || pstride < 1)
     return NECG_ERR_ARG;
   if (ntr == 0) return NECG_OK;
-  // the contexts must still hold the state the pool was made from
-  std::vector<PoolSys> now;
-  if (!snapshot(P, now)) return NECG_ERR_ARG;
-  for (size_t i = 0; i < now.size(); ++i)
-    if (now[i].bra != P->hsys[i].bra || now[i].H != P->hsys[i].H || now[i].K != P->hsys[i].K)
-      return NECG_ERR_ARG;
+  // a context-backed pool: the contexts must still hold the state the pool
+  // was made from
+  if (!P->ctx.empty()) {
+    std::vector<PoolSys> now;
+    if (!snapshot(P, now)) return NECG_ERR_ARG;
+    for (size_t i = 0; i < now.size(); ++i)
+      if (now[i].bra != P->hsys[i].bra || now[i].H != P->hsys[i].H || now[i].K != P->hsys[i].K)
+        return NECG_ERR_ARG;
+  }
   int prev = -1;
   (void)hipGetDevice(&prev);
   if (prev != P->device) (void)hipSetDevice(P->device);

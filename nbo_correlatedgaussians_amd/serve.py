@@ -10,12 +10,15 @@ request kinds arrive over time:
 * ``full``: energy and analytic gradient of system s at its resident x
   (``outprod``, python/outprod.py:20).
 
-The scheduler does continuous batching: on every pass it collects the trials
-that have arrived, admits as many as the free memory budget allows into one
-batched trial launch on a free trial stream, and starts full evaluations on
-free evaluation streams; launches on different streams overlap on the device
+The scheduler does continuous batching: on every pass it collects the
+requests that have arrived, admits as many trials as the free memory budget
+allows into one batched trial launch on a free trial stream, and admits the
+waiting full evaluations (one per system) into one masked batch evaluation on
+a free evaluation stream; launches on different streams overlap on the device
 and complete independently, and a new pass admits whatever arrived meanwhile.
-A system with a full evaluation in flight takes no new work until it ends.
+A system with a full evaluation in flight takes no new work until it ends, a
+full evaluation waits for its system's trials in flight, and a trial that
+arrived after a waiting full evaluation of its system waits behind it.
 Latency is measured from a request's arrival to its results being on the
 host.
 """
@@ -70,37 +73,46 @@ class _Launch:
 
 
 class Server:
-    """Resident systems on one GPU with continuous batching of requests."""
+    """Resident systems on one GPU with continuous batching of requests.
+
+    All systems live in one ``Batch`` (their parameters, bra/ket tables, H, S,
+    H C, S C and quotients resident in HBM); trials run through a pool over
+    that resident state, and the full evaluations admitted in one scheduling
+    pass run as one masked batch evaluation (only their systems active)."""
 
     def __init__(self, systems, xs, device=None, mem_budget: int = 4 << 30,
-                 max_trials: int = 512, trial_streams: int = 2, eval_streams: int = 2):
+                 max_trials: int = 512, trial_streams: int = 2, eval_streams: int = 2,
+                 item_flops: float = 0.0, tune: bool = False):
         torch = _torch()
+        from .batch import Batch, autotune
         self.device = torch.device(device if device is not None else "cuda:0")
         self.systems = list(systems)
-        self.engines = []
-        self.x = []
-        for s, x in zip(systems, xs):
-            eng = Engine(s, self.device)
-            xd = torch.as_tensor(np.asarray(x, dtype=np.float64), device=self.device)
-            eng.outprod_t(xd)           # resident state of x
-            self.engines.append(eng)
-            self.x.append(xd)
+        self.xs = [np.asarray(x, dtype=np.float64) for x in xs]
+        Ks = []
+        for s, x in zip(self.systems, self.xs):
+            if x.size % (s.blk + 1):
+                raise ValueError("x length is not a multiple of (blk + 1)")
+            Ks.append(x.size // (s.blk + 1))
+        self.Ks = Ks
+        self.batch = Batch(self.systems, Ks, self.device)
+        if item_flops:
+            self.batch.set_tiling(item_flops, 0)
+        self.xb = torch.as_tensor(self.batch.pack(self.xs), device=self.device)
+        if tune:                                # pair-tile size timed on these systems
+            autotune(self.batch, self.xb.cpu().numpy(), reps=2)
+        self.batch.outprod_t(self.xb)           # resident state of every system
         torch.cuda.synchronize(self.device)
-        for e in self.engines:
-            e.check("initial evaluation")
-        self.Ks = [e.K_of(x.numel()) for e, x in zip(self.engines, self.x)]
+        self.batch.check("initial evaluation")
+        self.offsets = self.batch.offsets
         self.lib = _lib.load()
-        ctxs = (ct.c_void_p * len(self.engines))(*[e._ctx.value for e in self.engines])
-        h = ct.c_void_p()
-        _lib.raise_for(self.lib.necg_pool_create(ctxs, len(self.engines), ct.byref(h)),
-                       "necg_pool_create")
-        self._pool = h
+        self._pool = self.batch.pool()
         kmax, wst = ct.c_int32(), ct.c_int32()
-        self.lib.necg_pool_sizes(h, ct.byref(kmax), ct.byref(wst))
+        self.lib.necg_pool_sizes(self._pool, ct.byref(kmax), ct.byref(wst))
         self.kmax, self.wstride = kmax.value, wst.value
         self.pmax = max(s.blk for s in systems)
         self.max_trials = max_trials
         self.mem_budget = mem_budget
+        nsys = len(self.systems)
         f64 = dict(dtype=torch.float64, device=self.device)
         i32 = dict(dtype=torch.int32, device=self.device)
         self._tbuf = []
@@ -114,24 +126,26 @@ class Server:
                 rowS=torch.empty(T * self.kmax, **f64),
                 h_in=torch.empty(T * (2 + self.pmax), dtype=torch.float64).pin_memory(),
                 h_E=torch.empty(T, dtype=torch.float64).pin_memory()))
-        xmax = max(x.numel() for x in self.x)
         self._ebuf = []
         for _ in range(eval_streams):
             self._ebuf.append(dict(
                 stream=torch.cuda.Stream(device=self.device),
-                f=torch.empty(1, **f64), g=torch.empty(xmax, **f64),
-                h_f=torch.empty(1, dtype=torch.float64).pin_memory(),
-                h_g=torch.empty(xmax, dtype=torch.float64).pin_memory()))
+                active=torch.zeros(nsys, **i32), f=torch.empty(nsys, **f64),
+                g=torch.empty(self.batch.nx, **f64),
+                h_active=torch.zeros(nsys, dtype=torch.int32).pin_memory(),
+                h_f=torch.empty(nsys, dtype=torch.float64).pin_memory(),
+                h_g=torch.empty(self.batch.nx, dtype=torch.float64).pin_memory()))
+        self._engines = None
         # bytes a request holds while in flight (admission budget)
         self.trial_bytes = 8 * (self.pmax + self.wstride + 1 + 2 * self.kmax)
-        self.full_bytes = [self._eval_bytes(e, K) for e, K in zip(self.engines, self.Ks)]
+        self.full_bytes = [self._eval_bytes(s, K) for s, K in zip(self.systems, self.Ks)]
 
     @staticmethod
-    def _eval_bytes(eng, K):
+    def _eval_bytes(system, K):
         """Device bytes one evaluation of this system works through: H, S and
         the element partials (4 K^2), the ket table (K nst KS), and the
         gradient partials (about K/16 tiles of NACC x K)."""
-        n, nst = eng.system.n, eng.system.nst
+        n, nst = system.n, system.nst
         nacc = n * (n + 1) + 6 * n + 2
         ks = n * (n + 1) + 6 * n + n * n + 6
         return 8 * (4 * K * K + K * nst * ks + max(1, K // 16) * nacc * K)
@@ -175,18 +189,25 @@ class Server:
             ev.record(b["stream"])
         return _Launch("trial", slot, ev, reqs, self.trial_bytes * n, {r.sys for r in reqs})
 
-    def _launch_full(self, slot, r):
+    def _launch_full(self, slot, reqs):
+        """One masked batch evaluation of the requests' systems (distinct)."""
         torch = _torch()
         b = self._ebuf[slot]
-        eng, x = self.engines[r.sys], self.x[r.sys]
-        nx = x.numel()
+        ha = b["h_active"].numpy()
+        ha[:] = 0
+        for r in reqs:
+            ha[r.sys] = 1
         with torch.cuda.stream(b["stream"]):
-            eng.outprod_t(x, b["f"], b["g"][:nx])
+            b["active"].copy_(b["h_active"], non_blocking=True)
+            self.batch.outprod_t(self.xb, b["f"], b["g"], active=b["active"])
             b["h_f"].copy_(b["f"], non_blocking=True)
-            b["h_g"][:nx].copy_(b["g"][:nx], non_blocking=True)
+            for r in reqs:
+                lo, hi = int(self.offsets[r.sys]), int(self.offsets[r.sys + 1])
+                b["h_g"][lo:hi].copy_(b["g"][lo:hi], non_blocking=True)
             ev = torch.cuda.Event()
             ev.record(b["stream"])
-        return _Launch("full", slot, ev, [r], self.full_bytes[r.sys], {r.sys})
+        nbytes = sum(self.full_bytes[r.sys] for r in reqs)
+        return _Launch("full", slot, ev, list(reqs), nbytes, {r.sys for r in reqs})
 
     # -- the loop ---------------------------------------------------------
     def run(self, requests, on_pass=None):
@@ -230,22 +251,28 @@ class Server:
                     free_t.append(L.slot)
                 else:
                     b = self._ebuf[L.slot]
-                    r = L.reqs[0]
-                    r.E = float(b["h_f"][0])
-                    r.g = b["h_g"][:self.x[r.sys].numel()].numpy().copy()
-                    r.done = tdone
-                    full_busy.discard(r.sys)
+                    fh, gh = b["h_f"].numpy(), b["h_g"].numpy()
+                    for r in L.reqs:
+                        r.E = float(fh[r.sys])
+                        r.g = gh[self.offsets[r.sys]:self.offsets[r.sys + 1]].copy()
+                        r.done = tdone
+                        full_busy.discard(r.sys)
                     free_e.append(L.slot)
                 used -= L.nbytes
                 finished += len(L.reqs)
             inflight = still
+            # a trial that arrived after a waiting full evaluation of its
+            # system waits behind it (per-system arrival order)
+            first_full = {}
+            for r in ready_f:
+                first_full.setdefault(r.sys, r.arrival)
             # admit trials: one batched launch per free trial stream
             while free_t and ready_t:
                 take = []
                 keep = deque()
                 while ready_t and len(take) < self.max_trials:
                     r = ready_t.popleft()
-                    if r.sys in full_busy:          # wait for that evaluation
+                    if r.sys in full_busy or r.arrival > first_full.get(r.sys, math.inf):
                         keep.append(r)
                         continue
                     if used + self.trial_bytes * (len(take) + 1) > self.mem_budget:
@@ -260,51 +287,83 @@ class Server:
                 L = self._launch_trials(free_t.pop(), take)
                 used += L.nbytes
                 inflight.append(L)
-            # admit full evaluations
-            skipped = deque()
+            # admit full evaluations: one masked batch launch per free stream
             while free_e and ready_f:
-                r = ready_f.popleft()
-                if (r.sys in full_busy or trial_busy.get(r.sys, 0) > 0 or
-                        used + self.full_bytes[r.sys] > self.mem_budget):
-                    skipped.append(r)
-                    continue
-                full_busy.add(r.sys)
-                L = self._launch_full(free_e.pop(), r)
+                take, keep, chosen, nb = [], deque(), set(), 0
+                while ready_f:
+                    r = ready_f.popleft()
+                    if (r.sys in full_busy or r.sys in chosen or trial_busy.get(r.sys, 0) > 0
+                            or used + nb + self.full_bytes[r.sys] > self.mem_budget):
+                        keep.append(r)
+                        continue
+                    take.append(r)
+                    chosen.add(r.sys)
+                    nb += self.full_bytes[r.sys]
+                ready_f.extendleft(reversed(keep))
+                if not take:
+                    break
+                full_busy |= chosen
+                L = self._launch_full(free_e.pop(), take)
                 used += L.nbytes
                 inflight.append(L)
-            ready_f.extendleft(reversed(skipped))
             if on_pass:
                 on_pass(len(inflight), len(ready_t) + len(ready_f))
             if not inflight and ptr < total and not ready_t and not ready_f:
                 wait = reqs[ptr].arrival - (time.perf_counter() - t0)
                 if wait > 0:
                     time.sleep(min(wait, 1e-3))
-        for e in self.engines:
-            e.check("serving")
+        self.batch.check("serving")
+        _lib.raise_for(self.lib.necg_pool_status(self._pool, ct.c_void_p(
+            self._tbuf[0]["stream"].cuda_stream)), "serving trials")
         return reqs
+
+    def full_alone(self, s: int):
+        """(E, g) of system s from a masked evaluation with only s active
+        (reference for the in-mix result)."""
+        L = self._launch_full(0, [Request(0, "full", s, 0.0)])
+        L.event.synchronize()
+        self.batch.check("full evaluation")
+        b = self._ebuf[0]
+        return float(b["h_f"][s]), b["h_g"].numpy()[self.offsets[s]:self.offsets[s + 1]].copy()
+
+    def engines(self):
+        """Per-system engines (the plain per-system path), built on first use."""
+        if self._engines is None:
+            self._engines = [Engine(s, self.device) for s in self.systems]
+        return self._engines
 
     def run_plain(self, requests):
         """The same requests one at a time, each launched alone and waited
-        for (no batching, no overlap): the per-request GPU path that the
-        scheduler is compared with.  Returns (seconds, pairs)."""
+        for (no batching, no overlap): trials one by one through the pool,
+        full evaluations through each system's own engine (``outprod``).
+        Returns (seconds, pairs)."""
         torch = _torch()
+        engines = self.engines()
+        xd = [torch.as_tensor(x, device=self.device) for x in self.xs]
+        f = torch.empty(1, dtype=torch.float64, device=self.device)
+        g = torch.empty(max(x.numel() for x in xd), dtype=torch.float64, device=self.device)
+        torch.cuda.synchronize(self.device)
         t0 = time.perf_counter()
         pairs = 0
         for r in requests:
             K = self.Ks[r.sys]
             if r.kind == "trial":
                 L = self._launch_trials(0, [Request(r.rid, "trial", r.sys, 0.0, r.func, r.newp)])
+                L.event.synchronize()
                 pairs += K
             else:
-                L = self._launch_full(0, Request(r.rid, "full", r.sys, 0.0))
+                nx = xd[r.sys].numel()
+                engines[r.sys].outprod_t(xd[r.sys], f, g[:nx])
+                E, gh = float(f.item()), g[:nx].cpu()
                 pairs += K * (K + 1) // 2
-            L.event.synchronize()
         torch.cuda.synchronize(self.device)
-        return time.perf_counter() - t0, pairs
+        dt = time.perf_counter() - t0
+        for e in engines:
+            e.check("plain serving")
+        return dt, pairs
 
     def trial_alone(self, r: Request):
         """One trial launched by itself (reference for the in-mix result)."""
-        torch = _torch()
         q = Request(r.rid, "trial", r.sys, 0.0, r.func, r.newp)
         L = self._launch_trials(0, [q])
         L.event.synchronize()

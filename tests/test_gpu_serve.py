@@ -1,7 +1,8 @@
 """Serving: single-function replacement trials against resident systems and
 full evaluations under continuous batching.  A trial's row and energy match a
 full recomputation with the replaced function; every result is bitwise the
-same alone and in the mix; a small memory budget still serves everything."""
+same alone and in the mix (full evaluations: a masked batch evaluation with
+only that system active), and matches the per-system engine; a small memory budget still serves everything."""
 
 import numpy as np
 import pytest
@@ -53,16 +54,26 @@ def test_mix_equals_alone_and_budget():
     assert all(r.done is not None for r in out)
     st = latency_stats(out)
     assert st["p99_ms"] >= st["p50_ms"] > 0
-    full_ref = {}
+    full_alone, full_eng = {}, {}
     for si, (s, x) in enumerate(sx):
-        full_ref[si] = Engine(s, "cuda:0").outprod(x)
+        full_alone[si] = srv.full_alone(si)
+        full_eng[si] = Engine(s, "cuda:0").outprod(x)
+    nfull = 0
     for r in out:
         if r.kind == "trial":
             E, _ = srv.trial_alone(r)
             assert r.E == E, r.rid
         else:
-            Ef, gf = full_ref[r.sys]
+            nfull += 1
+            Ef, gf = full_alone[r.sys]
             assert r.E == Ef and np.array_equal(r.g, gf), r.rid
+    assert nfull > 0
+    # the masked batch evaluation agrees with the per-system engine
+    for si in full_alone:
+        Ef, gf = full_alone[si]
+        Ee, ge = full_eng[si]
+        assert abs(Ef - Ee) <= 1e-12 * max(1.0, abs(Ee))
+        assert np.max(np.abs(gf - ge)) <= 1e-9 * max(1.0, np.max(np.abs(ge)))
     # a budget of two trials' workspace still serves every request
     small = Server(systems, xs, "cuda:0", mem_budget=2 * srv.trial_bytes + max(srv.full_bytes))
     out2 = small.run(poisson_trace(systems, small.Ks, 60, rate=5000.0, seed=2))
