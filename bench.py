@@ -122,6 +122,27 @@ def served_leg(nreq, rate):
     return out
 
 
+def properties_leg(eng, xd, K, reps=5):
+    """The driver's dipole pass (both sides symmetrised: nst^2 term pairs per
+    basis pair) on the headline basis (LiH K = 2048, seed 1)."""
+    import torch
+    out = eng.dipole_t(xd)
+    eng.check("dipole warm-up")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        eng.dipole_t(xd, out)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    eng.check("dipole")
+    o = out.cpu().numpy()
+    nst = eng.system.nst
+    pairs = K * (K + 1) // 2
+    return {"workload": f"dipole pass, LiH K={K}: {pairs} pairs x {nst}^2 term pairs",
+            "ms": dt * 1e3, "pairs_per_s": pairs / dt, "term_pairs_per_s": pairs * nst * nst / dt,
+            "dipole_au": [float(v) for v in o[-3:]]}
+
+
 def traffic_probe(K, seed):
     """For the rocprofv3 counter passes: a 256 MiB device copy (the
     calibration of FETCH_SIZE / WRITE_SIZE) followed by two outprod calls."""
@@ -169,6 +190,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-props", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=10000)
     ap.add_argument("--serve-rate", type=float, default=1000.0)
     ap.add_argument("--traffic-probe", action="store_true")
@@ -246,6 +268,9 @@ def main():
                      "tiling": {"ket_tile": kt, "term_split": ts, "waves": w}},
         "energy": float(f.item()),
     }
+    if not args.no_props:
+        out["properties"] = properties_leg(eng, xd, K)
+        print(f"bench: properties leg done", file=sys.stderr, flush=True)
     if not args.no_batch:
         out["batched"] = batched_leg()
         print(f"bench: batched leg done", file=sys.stderr, flush=True)

@@ -84,6 +84,15 @@ int necg_matrices(necg_ctx *ctx, int32_t K, const double *iv, double *H, double 
 /* Kinetic and potential matrices (K*K row-major, symmetric):
  * T = sum_P scvv tk, V = sum_P scvv ert (python/necg.py:43-57; the Stark
  * term enters H only).  x is the full parameter vector.  Device.  Async. */
+/* The dipole pass of the driver (python/necg.py:200-232 over python/rrms.py:21-120;
+ * source/necg.f90:387-446 over source/rrms.f90): both bra and ket symmetrised,
+ * weights scvv[stf] scvv[stfe], contracted with C.  out (device, 3n + 4):
+ * out[0] = C^T s C; out[1 + 3 i + a] = <r_i,a> (the reference's d3 of
+ * pseudo-particle i, component a); out[3n + 1 + a] = sum_i ncv[i+1] <r_i,a>
+ * (d7, d8, d9; times 2.541765 for Debye).  x is the full parameter vector.
+ * Async. */
+int necg_dipole(necg_ctx *ctx, int32_t K, const double *x, double *out, void *stream);
+
 int necg_kinetic_potential(necg_ctx *ctx, int32_t K, const double *x, double *T, double *V,
                            void *stream);
 

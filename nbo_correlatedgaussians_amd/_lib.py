@@ -23,7 +23,7 @@ NECG_ERR_CSC = 6
 NECG_ERR_NONFINITE = 7
 
 EXPORTS = ("necg_create", "necg_destroy", "necg_outprod", "necg_energy",
-           "necg_kinetic_potential",
+           "necg_kinetic_potential", "necg_dipole",
            "necg_energy_vjp", "necg_matrices", "necg_matel", "necg_gradel",
            "necg_dhds", "necg_status", "necg_set_tiling", "necg_get_tiling",
            "necg_timing", "necg_strerror", "necg_batch_create", "necg_batch_destroy",
@@ -90,6 +90,7 @@ def load() -> ct.CDLL:
         "necg_energy_vjp": [P, I, P, P, P, P],
         "necg_matrices": [P, I, P, P, P, P],
         "necg_kinetic_potential": [P, I, P, P, P, P],
+        "necg_dipole": [P, I, P, P, P],
         "necg_matel": [P, I, P, I, P, P, P],
         "necg_gradel": [P, I, P, I, P, P, P, P],
         "necg_dhds": [P, I, P, P, P, P],

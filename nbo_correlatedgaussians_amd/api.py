@@ -3,7 +3,8 @@
 The reference keeps its state in module globals filled by ``read_input``
 (python/read_input.py:16, python/shared.py) and exposes ``outprod(x)``
 (python/outprod.py:20), ``matel(stf, ii, jj)`` (python/matel.py:28) and
-``gradel(stf, ii, jj)`` (python/gradel.py:27) with 1-based indices.  This
+``gradel(stf, ii, jj)`` (python/gradel.py:27) with 1-based indices, and runs
+the dipole pass of its driver (``dipole()``, python/necg.py:200-232).  This
 module mirrors that: after ``read_input(path)`` (or ``use(system, iv, C)``),
 ``outprod`` can be handed to ``scipy.optimize.minimize(..., jac=True)``
 unchanged.  The arithmetic runs on the GPU through ``libnecg_hip.so``.
@@ -92,3 +93,13 @@ def gradel(stf: int, ii: int, jj: int):
     """(tgvl, tgvls) of term stf between bra ii and ket jj, 1-based."""
     a, b = engine().gradel(iv, M, [(stf - 1, ii - 1, jj - 1)])
     return a[0], b[0]
+
+
+DEBYE = 2.541765   # a.u. of dipole moment in Debye (python/necg.py:234)
+
+
+def dipole():
+    """(d7, d8, d9) of the current state in a.u. and in Debye, with the
+    centres <r_i> (n, 3) of the pseudo-particles (python/necg.py:200-244)."""
+    css, rbar, d = engine().dipole(x0())
+    return {"dipole_au": d, "dipole_debye": d * DEBYE, "centres": rbar, "cSc": css}

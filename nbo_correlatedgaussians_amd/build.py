@@ -11,7 +11,8 @@ import subprocess
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SRCS = [os.path.join(HERE, "csrc", f) for f in ("necg.hip", "necg_batch.hip", "necg_pool.hip")]
+SRCS = [os.path.join(HERE, "csrc", f) for f in ("necg.hip", "necg_batch.hip", "necg_pool.hip",
+                                                   "necg_props.hip")]
 OUT = os.path.join(HERE, "libnecg_hip.so")
 ARCH = os.environ.get("NECG_OFFLOAD_ARCH", "gfx950")
 

@@ -25,6 +25,9 @@ struct necg_ctx {
   double *part = nullptr, *hpart = nullptr, *spart = nullptr;
   double *hcp = nullptr, *scp = nullptr, *red = nullptr;
   size_t part_cap = 0, hpart_cap = 0;
+  // properties pass: (function, term) records and per-wave partial rows
+  double *prop_rec = nullptr, *prop_part = nullptr;
+  size_t prop_rec_cap = 0, prop_part_cap = 0;
   int req_kt = 0, req_ts = 0, req_w = 0;
   int last_kt = 0, last_ts = 0, last_w = 0;
   int last_K = 0;

@@ -12,8 +12,10 @@
   run with the LMQN constants of source/necg.f90:274-280 through scipy's TNC
   (maxCGit = 20, eta = 0.21, stepmx = 7, accuracy = 2.22e-16,
   maxfun = 150 (siv + M)), stopped after a given number of outer iterations.
+* ``properties``: the dipole pass (python/necg.py:200-244): centres of the
+  pseudo-particles and the dipole moment in a.u. and Debye.
 * ``run``: the whole sequence for one input (matrices, eigenvector start,
-  energetics, optimisation, final energetics).
+  energetics, optimisation, final energetics and dipole).
 """
 
 from __future__ import annotations
@@ -64,6 +66,15 @@ def energetics(engine: Engine, x):
     kin = float(C @ (T @ C)) / cSc
     pot = float(C @ (V @ C)) / cSc
     return {"energy": E, "kinetic": kin, "potential": pot, "virial": -2.0 * kin / pot}
+
+
+DEBYE = 2.541765   # python/necg.py:234
+
+
+def properties(engine: Engine, x):
+    """Dipole pass for x: {'dipole_au', 'dipole_debye', 'centres', 'cSc'}."""
+    css, rbar, d = engine.dipole(x)
+    return {"dipole_au": d, "dipole_debye": d * DEBYE, "centres": rbar, "cSc": css}
 
 
 class _Stop(Exception):
@@ -157,5 +168,6 @@ def run(system: System, iv, C=None, device=None, method: str = "L-BFGS-B", maxit
                            "message": res.message, "energies": res.energies,
                            "seconds": res.seconds}
     out["final"] = energetics(eng, res.x)
+    out["final"].update(properties(eng, res.x))
     out["x"] = res.x
     return out

@@ -165,6 +165,17 @@ class Engine:
         _lib.raise_for(rc, "necg_kinetic_potential")
         return T, V
 
+    def dipole_t(self, x, out=None):
+        """The driver's dipole pass (python/necg.py:200-232) for x: a device
+        vector [C^T s C, <r_i,a> (3n, particle-major), dipole (3)]."""
+        x = self._dev(x)
+        K = self.K_of(x.numel())
+        n = self.system.n
+        out = self._empty(3 * n + 4) if out is None else out
+        rc = self.lib.necg_dipole(self._ctx, K, self._ptr(x), self._ptr(out), self._stream())
+        _lib.raise_for(rc, "necg_dipole")
+        return out
+
     # -- synchronous NumPy API ---------------------------------------------
     @staticmethod
     def _host_x(x):
@@ -191,6 +202,16 @@ class Engine:
             f, H, S = out
             return float(f.item()), H.cpu().numpy(), S.cpu().numpy()
         return float(out.item())
+
+    def dipole(self, x):
+        """(C^T s C, <r> as (n, 3), dipole (3,)) in atomic units; the
+        reference prints the dipole also times 2.541765 (Debye)."""
+        x = self._host_x(x)
+        out = self.dipole_t(x)
+        self.check("dipole")
+        o = out.cpu().numpy()
+        n = self.system.n
+        return float(o[0]), o[1:1 + 3 * n].reshape(n, 3).copy(), o[1 + 3 * n:].copy()
 
     def matrices(self, iv, K: int):
         iv = self._host_x(iv)

@@ -14,6 +14,7 @@
  *   gradel            python/gradel.py:27-392
  *   outprod           python/outprod.py:20-76
  *   make_grad         python/make_grad.py:18-90
+ *   rrms, dipole      python/rrms.py:21-120, python/necg.py:200-232
  * in "n-space": every 3n x 3n operator kron(X, I3) of the reference acts on a
  * particle-major 3n vector exactly as the n x n matrix X acts on the n x 3
  * matrix holding that vector (row i = xyz of pseudo-particle i), so X is used
@@ -25,7 +26,11 @@
  * Pinning (what checks this file besides the reference's own numbers): the
  * golden vectors written from the reference, central finite differences of
  * matel against gradel, the hydrogen-atom closed form, the Gaussian product
- * theorem for overlaps, and F0(0) = 1 / erf form of the Boys function; see
+ * theorem for overlaps, and F0(0) = 1 / erf form of the Boys function; rrms
+ * and the dipole by the Gaussian product theorem (one particle: overlap and
+ * centre in closed form), a single basis function (the centre is the shift),
+ * the overlap of matel for an identity bra term, and the covariance of the
+ * centres under one common translation of every shift; see
  * tests/test_oracle.py.
  */
 #include <math.h>
@@ -623,4 +628,122 @@ int necg_oracle_matel_rows(const osys *s, int K, const double *iv, int row0, int
     }
   }
   return bad;
+}
+
+/* rrms (python/rrms.py:21-120, source/rrms.f90:1-250) for bra ii under term
+ * stfe and ket jj under term stf (0-based): the overlap ov and ov * st[c] for
+ * every component c of st = (A_k' + A_l')^-1 (A_k' s_k' + A_l' s_l') (the
+ * reference returns one component, n1; all 3n are returned here).  Both bra
+ * and ket are transformed: A' = sml^T A sml, s' = sms s.  Returns 0 if the
+ * pair matrix is singular. */
+int necg_oracle_rrms(const osys *s, const double *iv, int stfe, int stf, int ii, int jj,
+                     double *ov, double *tk) {
+  const int n = s->n, svh = n * (n + 1) / 2, skp = 3 * n, blk = svh + skp;
+  double lk[NMAX * NMAX] = {0}, ell[NMAX * NMAX] = {0}, lkt[NMAX * NMAX] = {0}, ellt[NMAX * NMAX] = {0};
+  int kk = ii * blk, ll = jj * blk;
+  for (int i = 0; i < n; ++i)
+    for (int j = i; j < n; ++j) {
+      lk[j * n + i] = iv[kk++];
+      ell[j * n + i] = iv[ll++];
+    }
+  double ak0[NMAX * NMAX], al0[NMAX * NMAX], ak[NMAX * NMAX], al[NMAX * NMAX];
+  double tmp[NMAX * NMAX], smt[NMAX * NMAX];
+  tr_(n, n, lk, lkt);
+  tr_(n, n, ell, ellt);
+  mm(n, n, n, lk, lkt, ak0);
+  mm(n, n, n, ell, ellt, al0);
+  const double *pl = s->sml + (size_t)stf * n * n, *pk = s->sml + (size_t)stfe * n * n;
+  mm(n, n, n, al0, pl, tmp);       /* ket: sml^T (al sml) */
+  tr_(n, n, pl, smt);
+  mm(n, n, n, smt, tmp, al);
+  mm(n, n, n, ak0, pk, tmp);       /* bra: sml^T (ak sml) */
+  tr_(n, n, pk, smt);
+  mm(n, n, n, smt, tmp, ak);
+  double st1[SKMAX], st2[SKMAX];
+  const double *s1 = iv + ii * blk + svh, *s2 = iv + jj * blk + svh;
+  const double *vl = s->sms + (size_t)stf * skp * skp, *vk = s->sms + (size_t)stfe * skp * skp;
+  for (int r = 0; r < skp; ++r) {
+    double a = 0.0, b = 0.0;
+    for (int c = 0; c < skp; ++c) {
+      a += vl[r * skp + c] * s2[c];
+      b += vk[r * skp + c] * s1[c];
+    }
+    st2[r] = a;
+    st1[r] = b;
+  }
+  double akl[NMAX * NMAX], akli[NMAX * NMAX], d;
+  for (int i = 0; i < n * n; ++i) akl[i] = ak[i] + al[i];
+  if (!inv_det(n, akl, akli, &d)) return 0;
+  const double dett = fabs(d);
+  double pk_ = 1.0, pl_ = 1.0;
+  for (int i = 0; i < n; ++i) { pk_ *= lk[i * n + i]; pl_ *= ell[i * n + i]; }
+  const double detak = sqrt(fabs(pk_ * pk_)), detal = sqrt(fabs(pl_ * pl_));
+  double iv1[SKMAX], iv2[SKMAX], ee[SKMAX], st[SKMAX];
+  mm(n, n, 3, ak, st1, iv1);
+  mm(n, n, 3, al, st2, iv2);
+  const double i1 = dot(skp, st1, iv1), i2 = dot(skp, st2, iv2);
+  for (int i = 0; i < skp; ++i) ee[i] = iv1[i] + iv2[i];
+  mm(n, n, 3, akli, ee, st);
+  const double i3 = dot(skp, ee, st);
+  const double o = exp(-(i1 + i2) + i3) * sqrt(pow(8.0, n)) * sqrt(pow(detak * detal / dett, 3));
+  *ov = o;
+  for (int c = 0; c < skp; ++c) tk[c] = st[c] * o;
+  return 1;
+}
+
+/* The dipole pass of the driver (python/necg.py:200-232, source/necg.f90:387-446):
+ * for every component c = 3 i + xyz, h_c and s summed over all term pairs
+ * (stfe, stf) with weight scvv[stf] scvv[stfe] on the lower triangle (bra
+ * j >= ket i), mirrored, then rbar[c] = C^T h_c C / C^T s C and
+ * dip[xyz] = sum_i ncv[i + 1] rbar[3 i + xyz].  *cSc = C^T s C.
+ * Returns 0 on success, 1 for a singular pair matrix, 2 for C^T s C == 0. */
+int necg_oracle_dipole(const osys *s, int K, const double *x, double *cSc, double *rbar,
+                       double *dip, int nthreads) {
+  const int n = s->n, skp = 3 * n, blk = n * (n + 1) / 2 + skp, nst = s->nst;
+  const double *C = x + (size_t)K * blk;
+  double *h = (double *)calloc((size_t)K * K * skp, sizeof(double));
+  double *sd = (double *)calloc((size_t)K * K, sizeof(double));
+  int bad = 0;
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : bad)
+#endif
+  for (int j = 0; j < K; ++j) {
+    for (int i = 0; i <= j; ++i) {
+      double hs[SKMAX] = {0}, ss = 0.0, tk[SKMAX], ov;
+      for (int stfe = 0; stfe < nst; ++stfe)
+        for (int stf = 0; stf < nst; ++stf) {
+          if (!necg_oracle_rrms(s, x, stfe, stf, j, i, &ov, tk)) { bad++; continue; }
+          const double w = s->scvv[stf] * s->scvv[stfe];
+          for (int c = 0; c < skp; ++c) hs[c] += w * tk[c];
+          ss += w * ov;
+        }
+      for (int c = 0; c < skp; ++c) {
+        h[((size_t)c * K + j) * K + i] = hs[c];
+        h[((size_t)c * K + i) * K + j] = hs[c];
+      }
+      sd[(size_t)j * K + i] = ss;
+      sd[(size_t)i * K + j] = ss;
+    }
+  }
+  int rc = 0;
+  if (bad) { rc = 1; goto done; }
+  double d2 = 0.0;
+  for (int i = 0; i < K; ++i)
+    for (int j = 0; j < K; ++j) d2 += C[i] * sd[(size_t)i * K + j] * C[j];
+  if (d2 == 0.0) { rc = 2; goto done; }
+  *cSc = d2;
+  for (int a = 0; a < 3; ++a) dip[a] = 0.0;
+  for (int c = 0; c < skp; ++c) {
+    double d1 = 0.0;
+    const double *hc = h + (size_t)c * K * K;
+    for (int i = 0; i < K; ++i)
+      for (int j = 0; j < K; ++j) d1 += C[i] * hc[(size_t)i * K + j] * C[j];
+    rbar[c] = d1 / d2;
+    dip[c % 3] += s->ncv[c / 3 + 1] * rbar[c];
+  }
+done:
+  free(h);
+  free(sd);
+  return rc;
 }

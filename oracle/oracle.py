@@ -51,6 +51,8 @@ def _load():
         _lib.necg_oracle_grada_rows.argtypes = [P, ct.c_int, P, ct.c_double, ct.c_double,
                                                 ct.c_int, ct.c_int, P, ct.c_int]
         _lib.necg_oracle_pair_sums.argtypes = [P, P, ct.c_int, P, P, P, P, P]
+        _lib.necg_oracle_rrms.argtypes = [P, P, ct.c_int, ct.c_int, ct.c_int, ct.c_int, P, P]
+        _lib.necg_oracle_dipole.argtypes = [P, ct.c_int, P, P, P, P, ct.c_int]
         _lib.necg_oracle_matel_rows.argtypes = [P, ct.c_int, P, ct.c_int, ct.c_int, P, P,
                                                 ct.c_int]
     return _lib
@@ -159,3 +161,28 @@ class Oracle:
                                               _p(dh), _p(ds)):
             raise np.linalg.LinAlgError("singular pair matrix")
         return H, S, dh, ds
+
+    def rrms(self, iv, stfe, stf, i, j):
+        """(ov, ov * st) of bra i under term stfe and ket j under term stf,
+        0-based (python/rrms.py:21-120, all 3n components of st)."""
+        iv = np.ascontiguousarray(iv, dtype=np.float64)
+        ov = np.zeros(1)
+        tk = np.zeros(3 * self.sys.n)
+        if not self.lib.necg_oracle_rrms(self._po, _p(iv), stfe, stf, i, j, _p(ov), _p(tk)):
+            raise np.linalg.LinAlgError("singular pair matrix")
+        return float(ov[0]), tk
+
+    def dipole(self, x, nthreads=0):
+        """(C^T s C, rbar (n, 3), dipole (3,)) of the driver's dipole pass
+        (python/necg.py:200-232)."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        K = x.size // (self.sys.blk + 1)
+        css = np.zeros(1)
+        rbar = np.zeros(3 * self.sys.n)
+        dip = np.zeros(3)
+        rc = self.lib.necg_oracle_dipole(self._po, K, _p(x), _p(css), _p(rbar), _p(dip), nthreads)
+        if rc == 1:
+            raise np.linalg.LinAlgError("singular pair matrix")
+        if rc == 2:
+            raise ZeroDivisionError("C^T s C == 0")
+        return float(css[0]), rbar.reshape(self.sys.n, 3), dip

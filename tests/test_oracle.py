@@ -181,3 +181,79 @@ def test_oracle_invariants(O):
     E4, g4 = o.outprod(necg.pack_x(iv, V[:, 0]))
     assert abs(E4 - w[0]) <= 1e-10 * max(1.0, abs(w[0]))
     assert np.max(np.abs(g4[K * s.blk:])) <= 1e-8 * np.max(np.abs(g4))
+
+
+# ------------------------------------------------------------ rrms / dipole
+
+def _identity_only(s):
+    """The same particles with only the identity term (no symmetrisation)."""
+    n = s.n
+    return necg.System(n=n, nst=1, efs=0.0, mass=s.mass, ncv=s.ncv, scvv_raw=np.ones(1),
+                       sml=np.eye(n)[None], sms=np.eye(3 * n)[None], name=s.name + "-id")
+
+
+def test_rrms_gaussian_product_theorem(O):
+    """One particle, two Gaussians: overlap (4ab/(a+b)^2)^(3/4) exp(-mu d^2)
+    and centre P = (aA + bB)/(a+b); the dipole pass of two functions is the
+    C-weighted mean of the centres A, B and P."""
+    a, b = 0.8, 2.3
+    A, B = np.array([0.1, -0.2, 0.3]), np.array([-0.4, 0.25, 0.05])
+    o, s, iv = _one_electron(O, [a, b], [A, B])
+    mu = a * b / (a + b)
+    Sab = (4 * a * b / (a + b) ** 2) ** 0.75 * math.exp(-mu * float(np.sum((A - B) ** 2)))
+    P = (a * A + b * B) / (a + b)
+    ov, tk = o.rrms(iv, 0, 0, 1, 0)
+    assert abs(ov - Sab) <= 1e-15
+    assert np.max(np.abs(tk - Sab * P)) <= 1e-15
+    ov, tk = o.rrms(iv, 0, 0, 0, 0)
+    assert abs(ov - 1.0) <= 1e-15 and np.max(np.abs(tk - A)) <= 1e-15
+    C = np.array([0.7, -1.3])
+    css, rbar, dip = o.dipole(np.concatenate([iv, C]))
+    den = C[0] ** 2 + C[1] ** 2 + 2 * C[0] * C[1] * Sab
+    num = C[0] ** 2 * A + C[1] ** 2 * B + 2 * C[0] * C[1] * Sab * P
+    assert abs(css - den) <= 1e-14 * den
+    assert np.max(np.abs(rbar[0] - num / den)) <= 1e-14
+    assert np.max(np.abs(dip - s.ncv[1] * num / den)) <= 1e-14
+
+
+@pytest.mark.parametrize("name", ["He", "H2", "LiH"])
+def test_rrms_single_function_centre(O, name):
+    """Without symmetrisation one basis function's centre is its shift and its
+    norm is 1 (times C^2)."""
+    s = _identity_only(system_by_name(name))
+    iv, C = necg.generate_parameters(s, 1, 11)
+    css, rbar, dip = O(s).dipole(necg.pack_x(iv, C))
+    assert abs(css - C[0] ** 2) <= 1e-13 * C[0] ** 2
+    np.testing.assert_allclose(rbar.reshape(-1), iv[s.svh:], rtol=0, atol=1e-13)
+    np.testing.assert_allclose(dip, s.ncv[1:] @ rbar, rtol=0, atol=1e-13)
+
+
+def test_rrms_overlap_is_matel_overlap(O):
+    """With the identity as bra term, rrms's overlap is matel's (which the
+    reference's golden vectors pin)."""
+    s = necg.lih_system()
+    o = O(s)
+    ident = [p for p in range(s.nst) if np.array_equal(s.sml[p], np.eye(s.n))]
+    assert ident
+    iv, _ = necg.generate_parameters(s, 4, 12)
+    for (i, j) in ((3, 1), (2, 2), (0, 3)):
+        for q in (0, 5, 17, 23):
+            ov, _ = o.rrms(iv, ident[0], q, i, j)
+            assert abs(ov - o.matel(iv, q, i, j)[0]) <= 1e-14 * max(1.0, abs(ov))
+
+
+@pytest.mark.parametrize("name,K", [("LiH", 4), ("H2", 6)])
+def test_dipole_translation_covariance(O, name, K):
+    """Moving every shift by one vector t moves every centre by t and leaves
+    C^T s C unchanged (the symmetry terms permute particles)."""
+    s = system_by_name(name)
+    o = O(s)
+    iv, C = necg.generate_parameters(s, K, 13)
+    css, rbar, dip = o.dipole(necg.pack_x(iv, C))
+    t = np.array([0.05, -0.11, 0.07])
+    iv2 = iv.reshape(K, s.blk).copy()
+    iv2[:, s.svh:] += np.tile(t, s.n)
+    css2, rbar2, dip2 = o.dipole(necg.pack_x(iv2.reshape(-1), C))
+    assert abs(css2 - css) <= 1e-12 * abs(css)
+    np.testing.assert_allclose(rbar2 - rbar, np.tile(t, (s.n, 1)), rtol=0, atol=1e-12)
+    np.testing.assert_allclose(dip2 - dip, t * s.ncv[1:].sum(), rtol=0, atol=1e-11)
