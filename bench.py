@@ -122,6 +122,31 @@ def served_leg(nreq, rate):
     return out
 
 
+def outprod_leg(eng, xd, K, reps=5):
+    """The same evaluation through the scipy drop-in route: one outprod call
+    (fused energy + gradient, E and g written to device buffers)."""
+    import torch
+    f = torch.empty(1, dtype=torch.float64, device=xd.device)
+    g = torch.empty_like(xd)
+    eng.outprod_t(xd, f, g)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        eng.outprod_t(xd, f, g)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    eng.check("outprod")
+    xh = xd.cpu().numpy()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        eng.outprod(xh)          # host x in, host (E, g) out: PCIe both ways
+    th = (time.perf_counter() - t0) / reps
+    pairs = K * (K + 1) // 2
+    return {"workload": f"LiH K={K}: outprod(x) -> (E, g)", "ms": dt * 1e3,
+            "pairs_per_s": pairs / dt, "host_buffers_ms": th * 1e3,
+            "host_buffers_pairs_per_s": pairs / th, "energy": float(f.item())}
+
+
 def properties_leg(eng, xd, K, reps=5):
     """The driver's dipole pass (both sides symmetrised: nst^2 term pairs per
     basis pair) on the headline basis (LiH K = 2048, seed 1)."""
@@ -191,6 +216,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-props", action="store_true")
+    ap.add_argument("--path", choices=("autograd", "outprod"), default="autograd",
+                    help="timed step: E = ecg_energy(x); E.backward(), or outprod(x)")
     ap.add_argument("--serve-requests", type=int, default=10000)
     ap.add_argument("--serve-rate", type=float, default=1000.0)
     ap.add_argument("--traffic-probe", action="store_true")
@@ -219,16 +246,28 @@ def main():
     xd = torch.as_tensor(x, device="cuda:0")
     f = torch.empty(1, dtype=torch.float64, device="cuda:0")
     g = torch.empty(x.size, dtype=torch.float64, device="cuda:0")
-    for _ in range(args.warmup):
+    from nbo_correlatedgaussians_amd.autograd import ecg_energy
+    xr = xd.clone().requires_grad_(True)
+
+    def step():
+        if args.path == "autograd":      # forward: pair kernel + quotient; backward: adjoint
+            xr.grad = None
+            E = ecg_energy(xr, eng)
+            E.backward()
+            return E
         eng.outprod_t(xd, f, g)
+        return f
+
+    for _ in range(args.warmup):
+        E = step()
     eng.check("warmup")
-    print(f"bench: warmup done, E = {f.item():.12e}", file=sys.stderr, flush=True)
+    print(f"bench: warmup done, E = {E.item():.12e}", file=sys.stderr, flush=True)
 
     eng.timing(1)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        eng.outprod_t(xd, f, g)
+        E = step()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     eng.check("timed steps")
@@ -255,7 +294,9 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": f"LiH n=5 nst=24 K={K} seed={args.seed}: one energy+analytic "
-                               f"gradient (outprod) per step",
+                               f"gradient per step, "
+                               + ("E = ecg_energy(x) (torch.autograd.Function) + E.backward()"
+                                  if args.path == "autograd" else "outprod(x)"),
                    "K": K, "pairs": pairs, "parallelism": "single GPU"},
         "roofline": {"bound": "mfma", "engine": "FP64 VALU (MI355X FP64 vector and matrix "
                                                  "peaks are both 78.6 TFLOP/s)",
@@ -266,8 +307,11 @@ def main():
                      "traffic_source": "profiles/pair_kernel_lih_k2048_traffic.md",
                      "flop_per_launch": fl, "pair_kernel_ms": pair_ms,
                      "tiling": {"ket_tile": kt, "term_split": ts, "waves": w}},
-        "energy": float(f.item()),
+        "energy": float(E.item()),
     }
+    if args.path == "autograd":
+        out["outprod"] = outprod_leg(eng, xd, K)
+        print(f"bench: outprod leg done", file=sys.stderr, flush=True)
     if not args.no_props:
         out["properties"] = properties_leg(eng, xd, K)
         print(f"bench: properties leg done", file=sys.stderr, flush=True)

@@ -34,7 +34,8 @@ enum necg_status_code {
   NECG_ERR_OOM = 4,         /* device allocation failed                        */
   NECG_ERR_NOT_PD = 5,      /* A_k + A_l' not positive definite / L_ii == 0    */
   NECG_ERR_CSC = 6,         /* C^T S C <= 0 (reference: ZeroDivisionError)     */
-  NECG_ERR_NONFINITE = 7    /* NaN or Inf in x or in a result                  */
+  NECG_ERR_NONFINITE = 7,   /* NaN or Inf in x or in a result                  */
+  NECG_ERR_STALE = 8        /* necg_energy_bwd without its necg_energy_fwd     */
 };
 
 /* The physics block of an input file: the module globals filled by
@@ -69,6 +70,22 @@ int necg_outprod(necg_ctx *ctx, int32_t K, const double *x, double *f, double *g
  * NULL (kept inside the context).  x, f, H, S: device.  Async. */
 int necg_energy(necg_ctx *ctx, int32_t K, const double *x, double *f, double *H, double *S,
                 void *stream);
+
+/* Forward with a tape: *f = E as necg_energy, running the gradient pair
+ * kernel of outprod (python/outprod.py:49-68 and the gradel pair loop of
+ * python/make_grad.py:34-55) and keeping its per-function partials on the
+ * context for necg_energy_bwd; *tape (host) receives the tape's id.  Any other
+ * evaluation on the context drops the tape.  x, f: device.  Async. */
+int necg_energy_fwd(necg_ctx *ctx, int32_t K, const double *x, double *f, int64_t *tape,
+                    void *stream);
+
+/* Backward of necg_energy_fwd: gx = gbar * dE/dx from the tape (the
+ * contraction of make_grad, python/make_grad.py:57-90: GRADA from the pair
+ * partials, GRADC from H C and S C).  NECG_ERR_STALE unless the last
+ * evaluation on the context was the necg_energy_fwd that returned `tape`, at
+ * this K.  gbar: device scalar (NULL = 1); gx: device.  Async. */
+int necg_energy_bwd(necg_ctx *ctx, int32_t K, int64_t tape, const double *gbar, double *gx,
+                    void *stream);
 
 /* Backward half: gx = gbar * dE/dx for the x of the last necg_energy call on
  * this context (the make_grad contraction, python/make_grad.py:37-90, as an

@@ -21,9 +21,10 @@ NECG_ERR_OOM = 4
 NECG_ERR_NOT_PD = 5
 NECG_ERR_CSC = 6
 NECG_ERR_NONFINITE = 7
+NECG_ERR_STALE = 8
 
 EXPORTS = ("necg_create", "necg_destroy", "necg_outprod", "necg_energy",
-           "necg_kinetic_potential", "necg_dipole",
+           "necg_kinetic_potential", "necg_dipole", "necg_energy_fwd", "necg_energy_bwd",
            "necg_energy_vjp", "necg_matrices", "necg_matel", "necg_gradel",
            "necg_dhds", "necg_status", "necg_set_tiling", "necg_get_tiling",
            "necg_timing", "necg_strerror", "necg_batch_create", "necg_batch_destroy",
@@ -91,6 +92,8 @@ def load() -> ct.CDLL:
         "necg_matrices": [P, I, P, P, P, P],
         "necg_kinetic_potential": [P, I, P, P, P, P],
         "necg_dipole": [P, I, P, P, P],
+        "necg_energy_fwd": [P, I, P, P, ct.POINTER(ct.c_int64), P],
+        "necg_energy_bwd": [P, I, ct.c_int64, P, P, P],
         "necg_matel": [P, I, P, I, P, P, P],
         "necg_gradel": [P, I, P, I, P, P, P, P],
         "necg_dhds": [P, I, P, P, P, P],

@@ -1,12 +1,17 @@
 """The ECG energy as a differentiable torch function.
 
-forward(x) -> E runs the element pair kernel (the matel loop,
-python/outprod.py:49-60) and the on-device quotient E = C^T H C / C^T S C
-(python/outprod.py:65-68).  backward(gE) runs the hand-written adjoint of
-gradel/make_grad: with the energy known, one pair kernel accumulates
-sum_kl C_k C_l (dH_kl - E dS_kl) per function (python/make_grad.py:37-90),
-i.e. the vector-Jacobian product with H-bar = C C^T / C^T S C and
-S-bar = -E C C^T / C^T S C, scaled by gE.
+forward(x) -> E.  When x needs a gradient, the forward runs the gradient pair
+kernel of outprod (the matel loop and the gradel pair terms of make_grad,
+python/outprod.py:49-68, python/make_grad.py:34-55) and the on-device quotient
+E = C^T H C / C^T S C, and keeps the per-function pair partials on the engine
+as its tape; otherwise it runs the element-only pair kernel.
+backward(gE) is the hand-written adjoint contraction of make_grad
+(python/make_grad.py:57-90): GRADA from the tape's partials and GRADC from
+H C and S C, scaled by gE.  If another evaluation ran on the engine between
+the two (the tape is gone), backward recomputes E, H C, S C for its own x
+and runs the adjoint pair kernel, which accumulates
+sum_kl C_k C_l (dH_kl - E dS_kl) per function: the vector-Jacobian product
+with H-bar = C C^T / C^T S C and S-bar = -E C C^T / C^T S C.
 """
 
 from __future__ import annotations
@@ -21,9 +26,12 @@ class ECGEnergy(torch.autograd.Function):
     def forward(ctx, x, engine: Engine):
         if x.dtype != torch.float64:
             raise TypeError("the ECG energy is FP64 throughout: x must be float64")
-        f = engine.energy_t(x)
+        ctx.tape = 0
+        if ctx.needs_input_grad[0]:
+            f, ctx.tape = engine.energy_fwd_t(x)
+        else:
+            f = engine.energy_t(x)
         ctx.engine = engine
-        ctx.generation = engine.energy_generation
         ctx.save_for_backward(x)
         return f.reshape(())
 
@@ -31,11 +39,11 @@ class ECGEnergy(torch.autograd.Function):
     def backward(ctx, gE):
         (x,) = ctx.saved_tensors
         eng = ctx.engine
-        if eng.energy_generation != ctx.generation:
-            # another forward ran on this engine since: restore its state (E,
-            # H C, S C) for this x before the adjoint pass
+        gb = gE.reshape(1).to(torch.float64)
+        gx = eng.energy_bwd_t(x, ctx.tape, gb) if ctx.tape else None
+        if gx is None:
             eng.energy_t(x)
-        gx = eng.vjp_t(x, gE.reshape(1).to(torch.float64))
+            gx = eng.vjp_t(x, gb)
         return gx.to(x.device), None
 
 

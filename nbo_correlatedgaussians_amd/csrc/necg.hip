@@ -430,6 +430,20 @@ int do_outprod(necg_ctx *c, int K, const double *x, double *f, double *g, hipStr
   return finish_grad<N, MODE_SPLIT>(c, K, ntiles, nullptr, g, st);
 }
 
+// Energy with the gradient's pair partials kept for a later backward
+// (necg_energy_bwd): the split pair kernel, H/S and the quotient.
+template <int N>
+int do_energy_fwd(necg_ctx *c, int K, const double *x, double *f, hipStream_t st) {
+  int rc, ntiles, ts;
+  if ((rc = run_prep<N>(c, K, x, 1, st))) return rc;
+  if ((rc = run_pairs<N, MODE_SPLIT>(c, K, st, ntiles, ts))) return rc;
+  if ((rc = finish_hs(c, K, ts, st))) return rc;
+  if ((rc = finish_quotient(c, K, f, st))) return rc;
+  c->tape_K = K;
+  c->tape_ntiles = ntiles;
+  return NECG_OK;
+}
+
 template <int N>
 int do_energy(necg_ctx *c, int K, const double *x, double *f, double *H, double *S,
               hipStream_t st) {
@@ -530,6 +544,7 @@ const char *necg_strerror(int code) {
     case NECG_ERR_NOT_PD: return "pair matrix not positive definite (zero Cholesky diagonal?)";
     case NECG_ERR_CSC: return "C^T S C <= 0";
     case NECG_ERR_NONFINITE: return "NaN or Inf in input or result";
+    case NECG_ERR_STALE: return "no forward tape for this backward (another evaluation ran since)";
     default: return "unknown status";
   }
 }
@@ -604,6 +619,7 @@ int necg_destroy(necg_ctx *c) {
 int necg_outprod(necg_ctx *c, int32_t K, const double *x, double *f, double *g, void *stream) {
   if (!c || K < 1 || !x || !g) return NECG_ERR_ARG;
   std::lock_guard<std::mutex> lk(c->mu);
+  c->tape_K = 0;
   DeviceGuard dg(c->device);
   int rc = ensure_K(c, K);
   if (rc) return rc;
@@ -615,6 +631,7 @@ int necg_energy(necg_ctx *c, int32_t K, const double *x, double *f, double *H, d
                 void *stream) {
   if (!c || K < 1 || !x) return NECG_ERR_ARG;
   std::lock_guard<std::mutex> lk(c->mu);
+  c->tape_K = 0;
   DeviceGuard dg(c->device);
   int rc = ensure_K(c, K);
   if (rc) return rc;
@@ -622,10 +639,36 @@ int necg_energy(necg_ctx *c, int32_t K, const double *x, double *f, double *H, d
   DISPATCH_N(c->n, (do_energy<NN>(c, K, x, f, H, S, (hipStream_t)stream)));
 }
 
+int necg_energy_fwd(necg_ctx *c, int32_t K, const double *x, double *f, int64_t *tape,
+                    void *stream) {
+  if (!c || K < 1 || !x || !tape) return NECG_ERR_ARG;
+  std::lock_guard<std::mutex> lk(c->mu);
+  c->tape_K = 0;
+  *tape = 0;
+  DeviceGuard dg(c->device);
+  int rc = ensure_K(c, K);
+  if (rc) return rc;
+  c->last_K = K;
+  auto run = [&]() -> int { DISPATCH_N(c->n, (do_energy_fwd<NN>(c, K, x, f, (hipStream_t)stream))); };
+  if ((rc = run()) == NECG_OK) *tape = c->tape_id = ++c->tape_seq;
+  return rc;
+}
+
+int necg_energy_bwd(necg_ctx *c, int32_t K, int64_t tape, const double *gbar, double *gx,
+                    void *stream) {
+  if (!c || K < 1 || !gx) return NECG_ERR_ARG;
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (c->tape_K != K || tape == 0 || tape != c->tape_id) return NECG_ERR_STALE;
+  DeviceGuard dg(c->device);
+  DISPATCH_N(c->n, (finish_grad<NN, MODE_SPLIT>(c, K, c->tape_ntiles, gbar, gx,
+                                                (hipStream_t)stream)));
+}
+
 int necg_energy_vjp(necg_ctx *c, int32_t K, const double *x, const double *gbar, double *gx,
                     void *stream) {
   if (!c || K < 1 || !x || !gx) return NECG_ERR_ARG;
   std::lock_guard<std::mutex> lk(c->mu);
+  c->tape_K = 0;
   if (c->last_K != K || K > c->Kcap) return NECG_ERR_ARG;  // needs necg_energy first
   DeviceGuard dg(c->device);
   DISPATCH_N(c->n, (do_vjp<NN>(c, K, x, gbar, gx, (hipStream_t)stream)));
@@ -634,6 +677,7 @@ int necg_energy_vjp(necg_ctx *c, int32_t K, const double *x, const double *gbar,
 int necg_matrices(necg_ctx *c, int32_t K, const double *iv, double *H, double *S, void *stream) {
   if (!c || K < 1 || !iv) return NECG_ERR_ARG;
   std::lock_guard<std::mutex> lk(c->mu);
+  c->tape_K = 0;
   DeviceGuard dg(c->device);
   int rc = ensure_K(c, K);
   if (rc) return rc;
@@ -644,6 +688,7 @@ int necg_kinetic_potential(necg_ctx *c, int32_t K, const double *x, double *T, d
                            void *stream) {
   if (!c || K < 1 || !x || !T || !V) return NECG_ERR_ARG;
   std::lock_guard<std::mutex> lk(c->mu);
+  c->tape_K = 0;
   DeviceGuard dg(c->device);
   int rc = ensure_K(c, K);
   if (rc) return rc;
@@ -654,6 +699,7 @@ int necg_matel(necg_ctx *c, int32_t K, const double *iv, int32_t nq, const int32
                double *out, void *stream) {
   if (!c || K < 1 || !iv || nq < 0 || (nq > 0 && (!q || !out))) return NECG_ERR_ARG;
   std::lock_guard<std::mutex> lk(c->mu);
+  c->tape_K = 0;
   DeviceGuard dg(c->device);
   int rc = ensure_K(c, K);
   if (rc) return rc;
@@ -664,6 +710,7 @@ int necg_gradel(necg_ctx *c, int32_t K, const double *iv, int32_t nq, const int3
                 double *tgvl, double *tgvls, void *stream) {
   if (!c || K < 1 || !iv || nq < 0 || (nq > 0 && (!q || !tgvl || !tgvls))) return NECG_ERR_ARG;
   std::lock_guard<std::mutex> lk(c->mu);
+  c->tape_K = 0;
   DeviceGuard dg(c->device);
   int rc = ensure_K(c, K);
   if (rc) return rc;
@@ -674,6 +721,7 @@ int necg_gradel(necg_ctx *c, int32_t K, const double *iv, int32_t nq, const int3
 int necg_dhds(necg_ctx *c, int32_t K, const double *iv, double *DH, double *DS, void *stream) {
   if (!c || K < 1 || !iv || !DH || !DS) return NECG_ERR_ARG;
   std::lock_guard<std::mutex> lk(c->mu);
+  c->tape_K = 0;
   DeviceGuard dg(c->device);
   int rc = ensure_K(c, K);
   if (rc) return rc;

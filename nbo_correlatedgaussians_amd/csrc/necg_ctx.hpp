@@ -31,6 +31,10 @@ struct necg_ctx {
   int req_kt = 0, req_ts = 0, req_w = 0;
   int last_kt = 0, last_ts = 0, last_w = 0;
   int last_K = 0;
+  // forward tape of necg_energy_fwd: the pair partials of x at K = tape_K
+  // (0: none; any other evaluation on the context clears it), named tape_id
+  int tape_K = 0, tape_ntiles = 0;
+  long long tape_id = 0, tape_seq = 0;
   // pair-kernel timing (necg_timing)
   bool timing = false;
   std::vector<hipEvent_t> ev;  // start/stop pairs, reused

@@ -48,7 +48,6 @@ class Engine:
         _lib.raise_for(rc, "necg_create")
         self._ctx = ctx
         self.blk = system.blk
-        self.energy_generation = 0  # bumped by every energy_t (autograd state check)
 
     # -- plumbing ----------------------------------------------------------
     def close(self):
@@ -130,8 +129,33 @@ class Engine:
         rc = self.lib.necg_energy(self._ctx, K, self._ptr(x), self._ptr(f), self._ptr(H),
                                   self._ptr(S), self._stream())
         _lib.raise_for(rc, "necg_energy")
-        self.energy_generation += 1
         return (f, H, S) if with_hs else f
+
+    def energy_fwd_t(self, x):
+        """(E, tape) for x: the gradient's pair partials stay on the context
+        as the tape that ``energy_bwd_t`` consumes."""
+        x = self._dev(x)
+        K = self.K_of(x.numel())
+        f = self._empty(1)
+        tape = ct.c_int64(0)
+        rc = self.lib.necg_energy_fwd(self._ctx, K, self._ptr(x), self._ptr(f), ct.byref(tape),
+                                      self._stream())
+        _lib.raise_for(rc, "necg_energy_fwd")
+        return f, tape.value
+
+    def energy_bwd_t(self, x, tape: int, gbar=None):
+        """gbar * dE/dx from ``tape`` (of ``energy_fwd_t`` at x), or None if
+        another evaluation ran on this engine since that forward."""
+        x = self._dev(x)
+        K = self.K_of(x.numel())
+        gx = self._empty(x.numel())
+        gb = None if gbar is None else self._dev(gbar).reshape(1)
+        rc = self.lib.necg_energy_bwd(self._ctx, K, int(tape), self._ptr(gb), self._ptr(gx),
+                                      self._stream())
+        if rc == _lib.NECG_ERR_STALE:
+            return None
+        _lib.raise_for(rc, "necg_energy_bwd")
+        return gx
 
     def vjp_t(self, x, gbar=None):
         x = self._dev(x)

@@ -64,3 +64,28 @@ def test_gradcheck_small():
     xr = x.clone().requires_grad_(True)
     assert torch.autograd.gradcheck(lambda t: ecg_energy(t, eng), (xr,), eps=1e-6, atol=1e-6,
                                     rtol=1e-4)
+
+
+def test_tape_backward_is_outprod_gradient():
+    """With gE = 1 the taped backward runs the same contraction as outprod:
+    the gradient is bitwise equal; a forward without grad gives the same E
+    (element-only kernel) to 1e-13; another evaluation drops the tape."""
+    import torch
+    from nbo_correlatedgaussians_amd.autograd import ecg_energy
+    s, eng, x = _setup("LiH", 200, 7)
+    xr = x.clone().requires_grad_(True)
+    E = ecg_energy(xr, eng)
+    E.backward()
+    Ef, gf = eng.outprod(x.cpu().numpy())
+    assert E.item() == Ef
+    np.testing.assert_array_equal(xr.grad.cpu().numpy(), gf)
+    with torch.no_grad():
+        E0 = ecg_energy(x, eng)
+    assert abs(E0.item() - Ef) <= 1e-13 * max(1.0, abs(Ef))
+    _, t1 = eng.energy_fwd_t(x)
+    _, t2 = eng.energy_fwd_t(x)
+    assert t2 != t1 and eng.energy_bwd_t(x, t1) is None     # a later forward replaced it
+    assert eng.energy_bwd_t(x, t2) is not None
+    eng.outprod_t(x)
+    assert eng.energy_bwd_t(x, t2) is None                  # another evaluation dropped it
+    eng.check()
