@@ -161,10 +161,14 @@ def properties_leg(eng, xd, K, reps=5):
     dt = (time.perf_counter() - t0) / reps
     eng.check("dipole")
     o = out.cpu().numpy()
+    from nbo_correlatedgaussians_amd.flops import dipole_flops_per_eval
     nst = eng.system.nst
     pairs = K * (K + 1) // 2
+    fl = dipole_flops_per_eval(eng.system.n, nst, K)
     return {"workload": f"dipole pass, LiH K={K}: {pairs} pairs x {nst}^2 term pairs",
             "ms": dt * 1e3, "pairs_per_s": pairs / dt, "term_pairs_per_s": pairs * nst * nst / dt,
+            "model_flop": fl, "achieved_tflops": fl / dt / 1e12,
+            "frac_fp64_peak": fl / dt / 1e12 / PEAK_FP64_TFLOPS,
             "dipole_au": [float(v) for v in o[-3:]]}
 
 

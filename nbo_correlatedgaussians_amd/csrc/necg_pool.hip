@@ -1,6 +1,7 @@
 // necg_pool.hip -- single-function replacement trials against resident
 // systems (serving).  A pool references the resident state of several
-// contexts (bra tables, H, S, HC, SC, cHc, cSc of their last evaluation).  A
+// contexts, or of every system of a batch (bra tables, H, S, HC, SC, cHc, cSc
+// of their last evaluation).  A
 // trial replaces function k of system s by new parameters: its bra and ket
 // records are built, the new row of H and S is formed (bra j of the resident
 // table against the new ket, the diagonal from the new bra; the matel loop of

@@ -20,3 +20,20 @@ def flops_per_eval(n: int, nst: int, K: int) -> float:
 
 def pairs_per_eval(K: int) -> int:
     return K * (K + 1) // 2
+
+
+def dipole_flops_per_term_pair(n: int) -> int:
+    """One (bra term, ket term) pair of the dipole pass in the L D L^T form,
+    counted with the rules above: M and e (n(n+1)/2 + 3n adds), L D L^T with
+    reciprocal pivots, forward and backward solves for three right-hand
+    sides, e.st, the weight (2 subtractions, exp, sqrt, 5 multiplications) and
+    the 1 + 3n accumulations.  LiH (n = 5): 295."""
+    ns = n * (n + 1) // 2
+    ldl = sum(2 * j + 2 + (n - 1 - j) * (2 * j + 1) for j in range(n))
+    fwd = 3 * n * (n - 1)
+    bwd = 3 * (n + n * (n - 1))
+    return ns + 3 * n + ldl + fwd + bwd + 6 * n + 9 + 1 + 6 * n
+
+
+def dipole_flops_per_eval(n: int, nst: int, K: int) -> float:
+    return float(dipole_flops_per_term_pair(n) * nst * nst * (K * (K + 1) // 2))
