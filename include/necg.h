@@ -212,6 +212,23 @@ int necg_pool_trials(necg_pool *pool, int32_t ntr, const int32_t *tsys, const in
                      const double *newp, int32_t pstride, double *ws, double *E, double *rowH,
                      double *rowS, int32_t rstride, void *stream);
 
+/* Resident factors of pool system s (device, owned by the caller, valid while
+ * the pool is used): V (K x K row-major) with V^T S V = I, V^T H V = diag(lam)
+ * and lam ascending -- the generalised eigenproblem that initialises C in the
+ * driver (python/necg.py:129-134, scipy.linalg.eigh; DSYGV at
+ * source/necg.f90:175).  V = lam = NULL removes them.  Blocking. */
+int necg_pool_set_factors(necg_pool *pool, int32_t s, const double *V, const double *lam);
+
+/* After necg_pool_trials on the same stream (same tsys/tfun/rowH/rowS/E):
+ * Eeig[t] = the lowest eigenvalue of the trial's pencil (H', S'), i.e. the
+ * energy with C re-optimised for the replaced basis, from the resident factors
+ * in O(K^2) per trial.  NaN where the system has no factors (and the
+ * bad-input status is latched) or where S' is not positive definite by a
+ * margin (smallest eigenvalue of S' relative to S below 1e-10; no status).  bg: device workspace of ntr * 2 * kmax doubles.  Async. */
+int necg_pool_eig(necg_pool *pool, int32_t ntr, const int32_t *tsys, const int32_t *tfun,
+                  const double *rowH, const double *rowS, int32_t rstride, const double *E,
+                  double *bg, double *Eeig, void *stream);
+
 /* A pool over the resident state of every system of a batch (after one full
  * evaluation of the batch).  The batch must outlive the pool. */
 int necg_batch_pool(necg_batch *batch, necg_pool **out);

@@ -31,7 +31,8 @@ EXPORTS = ("necg_create", "necg_destroy", "necg_outprod", "necg_energy",
            "necg_batch_layout", "necg_batch_outprod", "necg_batch_matrices",
            "necg_batch_set_tiling", "necg_batch_status", "necg_batch_outprod_masked",
            "necg_batch_pool", "necg_pool_create",
-           "necg_pool_destroy", "necg_pool_sizes", "necg_pool_trials", "necg_pool_status")
+           "necg_pool_destroy", "necg_pool_sizes", "necg_pool_trials", "necg_pool_status",
+           "necg_pool_set_factors", "necg_pool_eig")
 
 
 class NecgError(RuntimeError):
@@ -114,6 +115,8 @@ def load() -> ct.CDLL:
         "necg_pool_destroy": [P],
         "necg_pool_sizes": [P, ct.POINTER(I), ct.POINTER(I)],
         "necg_pool_trials": [P, I, P, P, P, I, P, P, P, P, I, P],
+        "necg_pool_set_factors": [P, I, P, P],
+        "necg_pool_eig": [P, I, P, P, P, P, I, P, P, P, P],
         "necg_pool_status": [P, P],
     }
     for name, args in sig.items():

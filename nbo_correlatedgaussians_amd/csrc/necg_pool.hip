@@ -34,16 +34,34 @@ namespace {
     default: { constexpr int NN = 6; call; break; } \
   }
 
+// The record of trial t's system and the replaced function k, or null when
+// tsys[t] or tfun[t] is out of range (t_prep latches the bad-input status).
+__device__ __forceinline__ const PoolSys *trial_sys(const PoolSys *__restrict__ sys, int nsys,
+                                                    const int32_t *__restrict__ tsys,
+                                                    const int32_t *__restrict__ tfun, int t,
+                                                    int &k) {
+  const int s = tsys[t];
+  if (s < 0 || s >= nsys) return nullptr;
+  k = tfun[t];
+  if (k < 0 || k >= sys[s].K) return nullptr;
+  return sys + s;
+}
+
 // Workgroup per trial: the new bra record, then its ket record under every term.
 __global__ void __launch_bounds__(256) t_prep(const PoolSys *__restrict__ sys,
                                               const int32_t *__restrict__ tsys,
-                                              const int32_t *__restrict__ tfun,
+                                              const int32_t *__restrict__ tfun, int nsys,
                                               const double *__restrict__ newp, int pstride,
                                               double *__restrict__ ws, long long wstride,
                                               int *__restrict__ flag) {
   const int t = blockIdx.x;
-  const PoolSys &b = sys[tsys[t]];
-  const int k = tfun[t];
+  int k = 0;
+  const PoolSys *bp = trial_sys(sys, nsys, tsys, tfun, t, k);
+  if (!bp) {
+    if (threadIdx.x == 0) atomicOr(flag, (int)F_BAD_INPUT);
+    return;
+  }
+  const PoolSys &b = *bp;
   double *w = ws + (size_t)t * wstride;
   if (threadIdx.x == 0) {
     POOL_SWITCH_N(b.n, (make_bra<NN>(newp + (size_t)t * pstride,
@@ -78,15 +96,17 @@ __device__ __forceinline__ void trial_row(const PoolSys &b, int k, const double 
 // Grid (ceil(Kmax/64), ntr), 64 lanes: lane j = bra j of the new row.
 __global__ void __launch_bounds__(64) t_rows(const PoolSys *__restrict__ sys,
                                              const int32_t *__restrict__ tsys,
-                                             const int32_t *__restrict__ tfun,
+                                             const int32_t *__restrict__ tfun, int nsys,
                                              const double *__restrict__ ws, long long wstride,
                                              double *__restrict__ rowH, double *__restrict__ rowS,
                                              int rstride, int *__restrict__ flag) {
   const int t = blockIdx.y;
-  const PoolSys &b = sys[tsys[t]];
+  int k = 0;
+  const PoolSys *bp = trial_sys(sys, nsys, tsys, tfun, t, k);
+  if (!bp) return;
+  const PoolSys &b = *bp;
   const int j = blockIdx.x * 64 + threadIdx.x;
   if (j >= b.K) return;
-  const int k = tfun[t];
   double *rh = rowH + (size_t)t * rstride + j, *rs = rowS + (size_t)t * rstride + j;
   POOL_SWITCH_N(b.n, trial_row<NN>(b, k, ws + (size_t)t * wstride, j, rh, rs, flag));
 }
@@ -94,14 +114,19 @@ __global__ void __launch_bounds__(64) t_rows(const PoolSys *__restrict__ sys,
 // Workgroup of 256 per trial: the updated quotient.
 __global__ void __launch_bounds__(256) t_energy(const PoolSys *__restrict__ sys,
                                                 const int32_t *__restrict__ tsys,
-                                                const int32_t *__restrict__ tfun,
+                                                const int32_t *__restrict__ tfun, int nsys,
                                                 const double *__restrict__ rowH,
                                                 const double *__restrict__ rowS, int rstride,
                                                 double *__restrict__ E, int *__restrict__ flag) {
   __shared__ double red[8];
   const int t = blockIdx.x;
-  const PoolSys &b = sys[tsys[t]];
-  const int k = tfun[t];
+  int k = 0;
+  const PoolSys *bp = trial_sys(sys, nsys, tsys, tfun, t, k);
+  if (!bp) {
+    if (threadIdx.x == 0) E[t] = __builtin_nan("");
+    return;
+  }
+  const PoolSys &b = *bp;
   int bs = 0, bc = 0;
   POOL_SWITCH_N(b.n, (bs = Dim<NN>::BS, bc = Dim<NN>::B_C));
   const double *rh = rowH + (size_t)t * rstride, *rs = rowS + (size_t)t * rstride;
@@ -123,6 +148,223 @@ __global__ void __launch_bounds__(256) t_energy(const PoolSys *__restrict__ sys,
     E[t] = cHc / cSc;
     if (!(cSc > 0.0)) atomicOr(flag, (int)F_CSC);
   }
+}
+
+// Sum of four values over a 256-lane workgroup in a fixed order; every lane
+// receives the totals.
+__device__ __forceinline__ void block_sum4(double v[4], double *red) {
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    double a = v[q];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) a += __shfl_down(a, o, 64);
+    if ((threadIdx.x & 63) == 0) red[(threadIdx.x >> 6) * 4 + q] = a;
+  }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    const int q = threadIdx.x;
+    red[16 + q] = (red[q] + red[4 + q]) + (red[8 + q] + red[12 + q]);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < 4; ++q) v[q] = red[16 + q];
+  __syncthreads();
+}
+
+constexpr int kProjSlices = 8;  // i-range slices of the projection (partial rows)
+
+// Grid (ceil(Kmax/256), ntr, kProjSlices), 256 lanes.  For trial t (function
+// k of system s replaced): the changes of row/column k, w = h' - H[:, k] and
+// v = s' - S[:, k] with their k-th entries halved (so that
+// H' = H + e_k w^T + w e_k^T), projected on the resident factors:
+// beta = V^T w, gamma = V^T v, as kProjSlices partial sums over i-ranges.
+// bg[t]: for each slice, beta (kmax) then gamma (kmax).
+__global__ void __launch_bounds__(256) t_proj(const PoolSys *__restrict__ sys,
+                                              const int32_t *__restrict__ tsys,
+                                              const int32_t *__restrict__ tfun, int nsys,
+                                              const double *__restrict__ rowH,
+                                              const double *__restrict__ rowS, int rstride,
+                                              double *__restrict__ bg, int kmax) {
+  const int t = blockIdx.y;
+  int k = 0;
+  const PoolSys *bp = trial_sys(sys, nsys, tsys, tfun, t, k);
+  if (!bp) return;
+  const PoolSys &b = *bp;
+  const int K = b.K;
+  if (blockIdx.x * 256 >= K || !b.V) return;  // uniform over the workgroup
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  const int per = (K + kProjSlices - 1) / kProjSlices;
+  const int i0 = blockIdx.z * per, i1 = min(K, i0 + per);
+  const double *rh = rowH + (size_t)t * rstride, *rs = rowS + (size_t)t * rstride;
+  const double *hk = b.H + (size_t)k * K, *sk = b.S + (size_t)k * K;
+  double be = 0.0, ga = 0.0;
+  if (j < K) {
+    for (int i = i0; i < i1; ++i) {
+      double w = rh[i] - hk[i], v = rs[i] - sk[i];
+      if (i == k) {
+        w *= 0.5;
+        v *= 0.5;
+      }
+      const double vij = b.V[(size_t)i * K + j];
+      be = __builtin_fma(vij, w, be);
+      ga = __builtin_fma(vij, v, ga);
+    }
+    double *o = bg + ((size_t)t * kProjSlices + blockIdx.z) * 2 * kmax;
+    o[j] = be;
+    o[kmax + j] = ga;
+  }
+}
+
+// beta_j and gamma_j of trial t from the projection's partial rows.
+__device__ __forceinline__ void proj_sum(const double *__restrict__ bg, int kmax, int j,
+                                         double &be, double &ga) {
+  be = 0.0;
+  ga = 0.0;
+#pragma unroll
+  for (int z = 0; z < kProjSlices; ++z) {
+    be += bg[(size_t)z * 2 * kmax + j];
+    ga += bg[(size_t)z * 2 * kmax + kmax + j];
+  }
+}
+
+// The count of eigenvalues of the trial pencil (H', S') below x.  In the
+// resident factor basis, V^T (H' - x S') V = D + a b^T + b a^T with
+// D = diag(lam - x), a = V^T e_k (row k of V), b = beta - x gamma.  By
+// Sylvester's law (S' positive definite) the count is the negative inertia of
+// that matrix, and by Haynsworth's inertia additivity on the bordered matrix
+// [[D, U], [U^T, -C]] with U = [a b], C = [[0, 1], [1, 0]] (C^-1 = C):
+//   nu(D + U C U^T) = nu(D) + nu(N) - nu(-C),  N = -C - U^T D^-1 U,  nu(-C) = 1.
+// det(N) changes sign at every eigenvalue between two poles lam_j, which the
+// root finder uses.
+struct TrialCount {
+  int count;    // eigenvalues of (H', S') below x
+  int below;    // resident eigenvalues lam_j below x
+  double detN;  // det N(x)
+};
+
+__device__ TrialCount trial_count(const double *__restrict__ lam, const double *__restrict__ a,
+                                  const double *__restrict__ beta,
+                                  const double *__restrict__ gam, int K, double x,
+                                  double *red) {
+  double v[4] = {0.0, 0.0, 0.0, 0.0};  // below, aa, ab, bb
+  for (int j = threadIdx.x; j < K; j += 256) {
+    double d = lam[j] - x;
+    v[0] += d < 0.0 ? 1.0 : 0.0;
+    if (d == 0.0) d = 1e-300;
+    const double r = 1.0 / d;
+    const double bj = beta[j] - x * gam[j];
+    v[1] = __builtin_fma(a[j] * a[j], r, v[1]);
+    v[2] = __builtin_fma(a[j] * bj, r, v[2]);
+    v[3] = __builtin_fma(bj * bj, r, v[3]);
+  }
+  block_sum4(v, red);
+  const double p = -v[1], q = -1.0 - v[2], rr = -v[3];
+  TrialCount c;
+  c.detN = p * rr - q * q;
+  const int nuN = c.detN < 0.0 ? 1 : ((p + rr) < 0.0 ? 2 : 0);
+  c.below = (int)v[0];
+  c.count = c.below + nuN - 1;
+  return c;
+}
+
+// One workgroup (256) per trial: the lowest eigenvalue mu of the trial pencil.
+// Bracket: mu <= lam_2 (a rank-2 change moves the lowest eigenvalue at most
+// past lam_2: nu(D) = 2 just above it) and mu <= E, the trial's Rayleigh
+// quotient; below, step down until the count is 0.  Then bisection while a
+// pole lam_j lies inside the bracket, and Illinois regula falsi on det N once
+// none does, down to adjacent doubles.  NaN if the system has no resident
+// factors or S' is not positive definite by a margin of 1e-10 (relative to S).
+__global__ void __launch_bounds__(256) t_eig(const PoolSys *__restrict__ sys,
+                                             const int32_t *__restrict__ tsys,
+                                             const int32_t *__restrict__ tfun, int nsys,
+                                             const double *__restrict__ bg, int kmax,
+                                             const double *__restrict__ E,
+                                             double *__restrict__ bgs,
+                                             double *__restrict__ Eeig, int *__restrict__ flag) {
+  __shared__ double red[20];
+  const int t = blockIdx.x;
+  int k = 0;
+  const PoolSys *bp = trial_sys(sys, nsys, tsys, tfun, t, k);
+  if (!bp) {
+    if (threadIdx.x == 0) Eeig[t] = __builtin_nan("");
+    return;
+  }
+  const PoolSys &b = *bp;
+  const int K = b.K;
+  if (!b.V || !b.lam) {
+    if (threadIdx.x == 0) {
+      Eeig[t] = __builtin_nan("");
+      atomicOr(flag, (int)F_BAD_INPUT);
+    }
+    return;
+  }
+  // beta, gamma from the partial rows, into this trial's summed rows
+  double *beta = bgs + (size_t)t * 2 * kmax, *gam = beta + kmax;
+  const double *part = bg + (size_t)t * kProjSlices * 2 * kmax;
+  const double *a = b.V + (size_t)k * K;
+  // V^T S' V = I + a g^T + g a^T has the smallest eigenvalue 1 + a.g - |a| |g|
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int j = threadIdx.x; j < K; j += 256) {
+    double be, ga;
+    proj_sum(part, kmax, j, be, ga);
+    beta[j] = be;
+    gam[j] = ga;
+    v[1] = __builtin_fma(a[j], a[j], v[1]);
+    v[2] = __builtin_fma(a[j], ga, v[2]);
+    v[3] = __builtin_fma(ga, ga, v[3]);
+  }
+  block_sum4(v, red);  // also orders the beta/gamma stores before the reads below
+  const double smin = 1.0 + v[2] - sqrt(v[1]) * sqrt(v[3]);
+  double hi = E[t];
+  if (!(smin > 1e-10) || !isfinite(hi)) {
+    if (threadIdx.x == 0) Eeig[t] = __builtin_nan("");
+    return;
+  }
+  const double scale = fmax(1.0, fabs(hi));
+  if (K >= 2) hi = fmin(hi, b.lam[1]);
+  double step = 1e-12 * scale;
+  hi += step;
+  TrialCount ch = trial_count(b.lam, a, beta, gam, K, hi, red);
+  for (int it = 0; ch.count < 1 && it < 64; ++it) {
+    step *= 2.0;
+    hi += step;
+    ch = trial_count(b.lam, a, beta, gam, K, hi, red);
+  }
+  step = 1e-3 * scale;
+  double lo = fmin(hi, b.lam[0]) - step;
+  TrialCount cl = trial_count(b.lam, a, beta, gam, K, lo, red);
+  for (int it = 0; cl.count >= 1 && it < 200; ++it) {
+    step *= 2.0;
+    lo -= step;
+    cl = trial_count(b.lam, a, beta, gam, K, lo, red);
+  }
+  int side = 0;  // last end moved: -1 lo, +1 hi (Illinois)
+  double flo = cl.detN, fhi = ch.detN;
+  for (int it = 0; it < 200; ++it) {
+    const double mid = 0.5 * (lo + hi);
+    if (!(mid > lo && mid < hi)) break;
+    double x = mid;
+    if (cl.below == ch.below && ((flo < 0.0) != (fhi < 0.0))) {
+      const double xf = hi - fhi * (hi - lo) / (fhi - flo);
+      if (xf > lo && xf < hi) x = xf;
+    }
+    const TrialCount cx = trial_count(b.lam, a, beta, gam, K, x, red);
+    if (cx.count >= 1) {
+      hi = x;
+      ch = cx;
+      fhi = cx.detN;
+      if (side == 1) flo *= 0.5;
+      side = 1;
+    } else {
+      lo = x;
+      cl = cx;
+      flo = cx.detN;
+      if (side == -1) fhi *= 0.5;
+      side = -1;
+    }
+    if (cx.detN == 0.0) break;
+  }
+  if (threadIdx.x == 0) Eeig[t] = hi;
 }
 
 int pool_ws_stride(const std::vector<PoolSys> &v) {
@@ -258,12 +500,48 @@ int necg_pool_trials(necg_pool *P, int32_t ntr, const int32_t *tsys, const int32
   (void)hipGetDevice(&prev);
   if (prev != P->device) (void)hipSetDevice(P->device);
   hipStream_t st = (hipStream_t)stream;
-  hipLaunchKernelGGL(t_prep, dim3(ntr), dim3(256), 0, st, P->d_sys, tsys, tfun, newp, pstride, ws,
-                     (long long)P->wstride, P->flag);
+  const int nsys = (int)P->hsys.size();
+  hipLaunchKernelGGL(t_prep, dim3(ntr), dim3(256), 0, st, P->d_sys, tsys, tfun, nsys, newp,
+                     pstride, ws, (long long)P->wstride, P->flag);
   hipLaunchKernelGGL(t_rows, dim3((P->kmax + 63) / 64, ntr), dim3(64), 0, st, P->d_sys, tsys, tfun,
-                     ws, (long long)P->wstride, rowH, rowS, rstride, P->flag);
-  hipLaunchKernelGGL(t_energy, dim3(ntr), dim3(256), 0, st, P->d_sys, tsys, tfun, rowH, rowS,
-                     rstride, E, P->flag);
+                     nsys, ws, (long long)P->wstride, rowH, rowS, rstride, P->flag);
+  hipLaunchKernelGGL(t_energy, dim3(ntr), dim3(256), 0, st, P->d_sys, tsys, tfun, nsys, rowH,
+                     rowS, rstride, E, P->flag);
+  const int rc = hipGetLastError() == hipSuccess ? NECG_OK : NECG_ERR_DEVICE;
+  if (prev >= 0 && prev != P->device) (void)hipSetDevice(prev);
+  return rc;
+}
+
+int necg_pool_set_factors(necg_pool *P, int32_t s, const double *V, const double *lam) {
+  if (!P || s < 0 || s >= (int)P->hsys.size() || (!V) != (!lam)) return NECG_ERR_ARG;
+  P->hsys[s].V = V;
+  P->hsys[s].lam = lam;
+  int prev = -1;
+  (void)hipGetDevice(&prev);
+  if (prev != P->device) (void)hipSetDevice(P->device);
+  const hipError_t e =
+      hipMemcpy(P->d_sys + s, &P->hsys[s], sizeof(PoolSys), hipMemcpyHostToDevice);
+  if (prev >= 0 && prev != P->device) (void)hipSetDevice(prev);
+  return e == hipSuccess ? NECG_OK : NECG_ERR_DEVICE;
+}
+
+int necg_pool_eig(necg_pool *P, int32_t ntr, const int32_t *tsys, const int32_t *tfun,
+                  const double *rowH, const double *rowS, int32_t rstride, const double *E,
+                  double *bg, double *Eeig, void *stream) {
+  if (!P || ntr < 0 || (ntr > 0 && (!tsys || !tfun || !rowH || !rowS || !E || !bg || !Eeig)) ||
+      rstride < P->kmax)
+    return NECG_ERR_ARG;
+  if (ntr == 0) return NECG_OK;
+  int prev = -1;
+  (void)hipGetDevice(&prev);
+  if (prev != P->device) (void)hipSetDevice(P->device);
+  hipStream_t st = (hipStream_t)stream;
+  const int nsys = (int)P->hsys.size();
+  double *bgs = bg + (size_t)ntr * kProjSlices * 2 * P->kmax;
+  hipLaunchKernelGGL(t_proj, dim3((P->kmax + 255) / 256, ntr, kProjSlices), dim3(256), 0, st,
+                     P->d_sys, tsys, tfun, nsys, rowH, rowS, rstride, bg, P->kmax);
+  hipLaunchKernelGGL(t_eig, dim3(ntr), dim3(256), 0, st, P->d_sys, tsys, tfun, nsys, bg, P->kmax,
+                     E, bgs, Eeig, P->flag);
   const int rc = hipGetLastError() == hipSuccess ? NECG_OK : NECG_ERR_DEVICE;
   if (prev >= 0 && prev != P->device) (void)hipSetDevice(prev);
   return rc;

@@ -17,6 +17,9 @@ struct PoolSys {
   SysConst sc;
   int n, nst, K;
   const double *bra, *H, *S, *HC, *SC, *scal, *sml, *sms, *scvv;
+  // resident factors of the pencil (H, S), or null: V (K x K row-major) with
+  // V^T S V = I and V^T H V = diag(lam), lam ascending
+  const double *V, *lam;
 };
 
 // A pool over resident records (device pointers that stay valid while the

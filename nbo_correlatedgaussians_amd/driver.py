@@ -36,9 +36,11 @@ def _torch():
     return torch
 
 
-def lowest_eigenpair(H, S):
-    """(lambda_min, v) of H v = lambda S v for symmetric H and positive
-    definite S (device tensors or arrays), v normalised so v^T S v = 1."""
+def generalized_eigh(H, S):
+    """All eigenpairs of H v = lambda S v (symmetric H, positive definite S;
+    device tensors or arrays): (lam ascending, V) with V^T S V = I and
+    V^T H V = diag(lam), by a Cholesky factorisation S = L L^T and the
+    symmetric eigensolver on L^-1 H L^-T, in FP64 on the device."""
     torch = _torch()
     dev = H.device if isinstance(H, torch.Tensor) else torch.device("cuda:0")
     H = torch.as_tensor(H, dtype=torch.float64, device=dev)
@@ -48,8 +50,15 @@ def lowest_eigenpair(H, S):
     A = torch.linalg.solve_triangular(L, Y.T.contiguous(), upper=False)  # L^-1 H L^-T
     A = 0.5 * (A + A.T)
     w, U = torch.linalg.eigh(A)
-    v = torch.linalg.solve_triangular(L.T.contiguous(), U[:, :1], upper=True)[:, 0]
-    return float(w[0].item()), v
+    V = torch.linalg.solve_triangular(L.T.contiguous(), U.contiguous(), upper=True)
+    return w.contiguous(), V.contiguous()
+
+
+def lowest_eigenpair(H, S):
+    """(lambda_min, v) of H v = lambda S v for symmetric H and positive
+    definite S (device tensors or arrays), v normalised so v^T S v = 1."""
+    w, V = generalized_eigh(H, S)
+    return float(w[0].item()), V[:, 0].contiguous()
 
 
 def energetics(engine: Engine, x):

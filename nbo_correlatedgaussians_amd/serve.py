@@ -5,8 +5,10 @@ its last evaluation (bra/ket tables, H, S, H C, S C, C^T H C, C^T S C).  Two
 request kinds arrive over time:
 
 * ``trial``: replace basis function k of system s by new (L_k, s_k); the new
-  row/column of H and S and the trial energy C^T H' C / C^T S' C (resident C)
-  are computed; the resident state is not modified.
+  row/column of H and S, the trial energy C^T H' C / C^T S' C (resident C) and,
+  from the resident factors of (H, S), the lowest eigenvalue of the trial
+  pencil (H', S') (C re-optimised) are computed; the resident state is not
+  modified.
 * ``full``: energy and analytic gradient of system s at its resident x
   (``outprod``, python/outprod.py:20).
 
@@ -53,6 +55,7 @@ class Request:
     newp: np.ndarray = None   # trial: new [vech L | s] block
     done: float = None
     E: float = None
+    Eeig: float = None        # trial: lowest eigenvalue of (H', S')
     g: np.ndarray = None
     pairs: int = 0            # unordered basis pairs of the work
 
@@ -76,13 +79,15 @@ class Server:
     """Resident systems on one GPU with continuous batching of requests.
 
     All systems live in one ``Batch`` (their parameters, bra/ket tables, H, S,
-    H C, S C and quotients resident in HBM); trials run through a pool over
-    that resident state, and the full evaluations admitted in one scheduling
-    pass run as one masked batch evaluation (only their systems active)."""
+    H C, S C and quotients resident in HBM) together with the factors of
+    their pencils (V with V^T S V = I, V^T H V = diag(lam)); trials run
+    through a pool over that resident state, and the full evaluations admitted
+    in one scheduling pass run as one masked batch evaluation (only their
+    systems active)."""
 
     def __init__(self, systems, xs, device=None, mem_budget: int = 4 << 30,
                  max_trials: int = 512, trial_streams: int = 2, eval_streams: int = 2,
-                 item_flops: float = 0.0, tune: bool = False):
+                 item_flops: float = 0.0, tune: bool = False, factors: bool = True):
         torch = _torch()
         from .batch import Batch, autotune
         self.device = torch.device(device if device is not None else "cuda:0")
@@ -110,6 +115,10 @@ class Server:
         self.lib.necg_pool_sizes(self._pool, ct.byref(kmax), ct.byref(wst))
         self.kmax, self.wstride = kmax.value, wst.value
         self.pmax = max(s.blk for s in systems)
+        self.factors = factors
+        self._fac = []                          # (V, lam) per system, kept alive here
+        if factors:
+            self._make_factors()
         self.max_trials = max_trials
         self.mem_budget = mem_budget
         nsys = len(self.systems)
@@ -124,8 +133,10 @@ class Server:
                 newp=torch.empty(T * self.pmax, **f64), ws=torch.empty(T * self.wstride, **f64),
                 E=torch.empty(T, **f64), rowH=torch.empty(T * self.kmax, **f64),
                 rowS=torch.empty(T * self.kmax, **f64),
+                bg=torch.empty(T * 18 * self.kmax, **f64), Eeig=torch.empty(T, **f64),
                 h_in=torch.empty(T * (2 + self.pmax), dtype=torch.float64).pin_memory(),
-                h_E=torch.empty(T, dtype=torch.float64).pin_memory()))
+                h_E=torch.empty(T, dtype=torch.float64).pin_memory(),
+                h_Eeig=torch.empty(T, dtype=torch.float64).pin_memory()))
         self._ebuf = []
         for _ in range(eval_streams):
             self._ebuf.append(dict(
@@ -137,8 +148,29 @@ class Server:
                 h_g=torch.empty(self.batch.nx, dtype=torch.float64).pin_memory()))
         self._engines = None
         # bytes a request holds while in flight (admission budget)
-        self.trial_bytes = 8 * (self.pmax + self.wstride + 1 + 2 * self.kmax)
+        self.trial_bytes = 8 * (self.pmax + self.wstride + 2 + 20 * self.kmax)
         self.full_bytes = [self._eval_bytes(s, K) for s, K in zip(self.systems, self.Ks)]
+
+    def _make_factors(self):
+        """V, lam of every system's pencil (H, S) from the resident matrices."""
+        torch = _torch()
+        from .driver import generalized_eigh
+        self.factor_bytes = 0
+        for s, K in enumerate(self.Ks):
+            H = torch.empty(K, K, dtype=torch.float64, device=self.device)
+            S = torch.empty(K, K, dtype=torch.float64, device=self.device)
+            st = ct.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+            _lib.raise_for(self.lib.necg_batch_matrices(self.batch._h, s, ct.c_void_p(H.data_ptr()),
+                                                        ct.c_void_p(S.data_ptr()), st),
+                           "necg_batch_matrices")
+            lam, V = generalized_eigh(H, S)
+            del H, S
+            self._fac.append((V, lam))
+            _lib.raise_for(self.lib.necg_pool_set_factors(self._pool, s, ct.c_void_p(V.data_ptr()),
+                                                          ct.c_void_p(lam.data_ptr())),
+                           "necg_pool_set_factors")
+            self.factor_bytes += 8 * (K * K + K)
+        torch.cuda.synchronize(self.device)
 
     @staticmethod
     def _eval_bytes(system, K):
@@ -184,6 +216,14 @@ class Server:
                 ct.c_void_p(b["E"].data_ptr()), ct.c_void_p(b["rowH"].data_ptr()),
                 ct.c_void_p(b["rowS"].data_ptr()), self.kmax, st)
             _lib.raise_for(rc, "necg_pool_trials")
+            if self.factors:
+                rc = self.lib.necg_pool_eig(
+                    self._pool, n, ct.c_void_p(b["tsys"].data_ptr()),
+                    ct.c_void_p(b["tfun"].data_ptr()), ct.c_void_p(b["rowH"].data_ptr()),
+                    ct.c_void_p(b["rowS"].data_ptr()), self.kmax, ct.c_void_p(b["E"].data_ptr()),
+                    ct.c_void_p(b["bg"].data_ptr()), ct.c_void_p(b["Eeig"].data_ptr()), st)
+                _lib.raise_for(rc, "necg_pool_eig")
+                b["h_Eeig"][:n].copy_(b["Eeig"][:n], non_blocking=True)
             b["h_E"][:n].copy_(b["E"][:n], non_blocking=True)
             ev = torch.cuda.Event()
             ev.record(b["stream"])
@@ -215,6 +255,7 @@ class Server:
         fills in done/E/g of each and returns them."""
         reqs = sorted(requests, key=lambda r: r.arrival)
         for r in reqs:
+            self._validate(r)
             K = self.Ks[r.sys]
             r.pairs = K if r.kind == "trial" else K * (K + 1) // 2
         total = len(reqs)
@@ -244,8 +285,10 @@ class Server:
                 if L.kind == "trial":
                     b = self._tbuf[L.slot]
                     E = b["h_E"].numpy()
+                    Ee = b["h_Eeig"].numpy()
                     for i, r in enumerate(L.reqs):
                         r.E = float(E[i])
+                        r.Eeig = float(Ee[i]) if self.factors else None
                         r.done = tdone
                         trial_busy[r.sys] -= 1
                     free_t.append(L.slot)
@@ -317,6 +360,18 @@ class Server:
             self._tbuf[0]["stream"].cuda_stream)), "serving trials")
         return reqs
 
+    def _validate(self, r: Request):
+        if r.kind not in ("trial", "full"):
+            raise ValueError(f"request {r.rid}: unknown kind {r.kind!r}")
+        if not 0 <= r.sys < len(self.systems):
+            raise ValueError(f"request {r.rid}: no system {r.sys}")
+        if r.kind == "trial":
+            if not 0 <= r.func < self.Ks[r.sys]:
+                raise ValueError(f"request {r.rid}: system {r.sys} has no function {r.func}")
+            blk = self.systems[r.sys].blk
+            if r.newp is None or np.asarray(r.newp).size != blk:
+                raise ValueError(f"request {r.rid}: new block must have {blk} numbers")
+
     def full_alone(self, s: int):
         """(E, g) of system s from a masked evaluation with only s active
         (reference for the in-mix result)."""
@@ -363,15 +418,18 @@ class Server:
         return dt, pairs
 
     def trial_alone(self, r: Request):
-        """One trial launched by itself (reference for the in-mix result)."""
+        """One trial launched by itself (reference for the in-mix result):
+        (E, (row of H', row of S'), lowest eigenvalue of (H', S') or None)."""
         q = Request(r.rid, "trial", r.sys, 0.0, r.func, r.newp)
+        self._validate(q)
         L = self._launch_trials(0, [q])
         L.event.synchronize()
         _lib.raise_for(self.lib.necg_pool_status(self._pool, ct.c_void_p(
             self._tbuf[0]["stream"].cuda_stream)), "trial")
-        return float(self._tbuf[0]["h_E"][0]), (
-            self._tbuf[0]["rowH"][:self.Ks[r.sys]].cpu().numpy(),
-            self._tbuf[0]["rowS"][:self.Ks[r.sys]].cpu().numpy())
+        b = self._tbuf[0]
+        return float(b["h_E"][0]), (b["rowH"][:self.Ks[r.sys]].cpu().numpy(),
+                                    b["rowS"][:self.Ks[r.sys]].cpu().numpy()), (
+            float(b["h_Eeig"][0]) if self.factors else None)
 
 
 def poisson_trace(systems, Ks, count: int, rate: float, trial_frac: float = 0.7,

@@ -22,6 +22,10 @@ def _systems():
 
 
 def test_trial_matches_full_recomputation():
+    """Row, Rayleigh quotient and lowest eigenvalue of (H', S') against a full
+    recomputation with the replaced function (scipy.linalg.eigh for the
+    eigenvalue: 1e-9 relative)."""
+    from scipy.linalg import eigh
     from nbo_correlatedgaussians_amd.engine import Engine
     from nbo_correlatedgaussians_amd.serve import Request, Server
     sx = _systems()
@@ -32,7 +36,7 @@ def test_trial_matches_full_recomputation():
         K = srv.Ks[si]
         for k in (0, K // 2, K - 1):
             iv1, _ = necg.generate_parameters(s, 1, int(rng.integers(1 << 30)))
-            E, (rh, rs) = srv.trial_alone(Request(0, "trial", si, 0.0, k, iv1))
+            E, (rh, rs), Ee = srv.trial_alone(Request(0, "trial", si, 0.0, k, iv1))
             x2 = x.copy()
             x2[k * s.blk:(k + 1) * s.blk] = iv1
             eng = Engine(s, "cuda:0")
@@ -40,6 +44,9 @@ def test_trial_matches_full_recomputation():
             assert abs(E - Ef) <= 1e-11 * max(1.0, abs(Ef)), (si, k, E, Ef)
             assert np.max(np.abs(rh - H[k])) <= 1e-12 * np.max(np.abs(H))
             assert np.max(np.abs(rs - S[k])) <= 1e-12 * np.max(np.abs(S))
+            w0 = eigh(H, S, eigvals_only=True, subset_by_index=[0, 0])[0]
+            assert abs(Ee - w0) <= 1e-9 * max(1.0, abs(w0)), (si, k, Ee, w0)
+            assert Ee <= E + 1e-12 * max(1.0, abs(E))
 
 
 def test_mix_equals_alone_and_budget():
@@ -61,8 +68,8 @@ def test_mix_equals_alone_and_budget():
     nfull = 0
     for r in out:
         if r.kind == "trial":
-            E, _ = srv.trial_alone(r)
-            assert r.E == E, r.rid
+            E, _, Ee = srv.trial_alone(r)
+            assert r.E == E and r.Eeig == Ee, r.rid
         else:
             nfull += 1
             Ef, gf = full_alone[r.sys]
@@ -78,3 +85,27 @@ def test_mix_equals_alone_and_budget():
     small = Server(systems, xs, "cuda:0", mem_budget=2 * srv.trial_bytes + max(srv.full_bytes))
     out2 = small.run(poisson_trace(systems, small.Ks, 60, rate=5000.0, seed=2))
     assert all(r.done is not None and np.isfinite(r.E) for r in out2)
+
+
+def test_trial_eigenvalue_edge_cases():
+    """A replacement by a copy of another function (S' singular) has no
+    eigenvalue; replacing a function by itself gives the resident lowest
+    eigenvalue; malformed requests are refused on the host."""
+    from scipy.linalg import eigh
+    from nbo_correlatedgaussians_amd.serve import Request, Server
+    sx = _systems()
+    srv = Server([s for s, _ in sx], [x for _, x in sx], "cuda:0", trial_streams=1,
+                 eval_streams=1)
+    for si, (s, x) in enumerate(sx):
+        K = srv.Ks[si]
+        blocks = x[:K * s.blk].reshape(K, s.blk)
+        E, _, Ee = srv.trial_alone(Request(0, "trial", si, 0.0, 3, blocks[7].copy()))
+        assert np.isnan(Ee) and np.isfinite(E)
+        H, S = srv.batch.matrices(si)
+        w0 = eigh(H, S, eigvals_only=True, subset_by_index=[0, 0])[0]
+        E, _, Ee = srv.trial_alone(Request(0, "trial", si, 0.0, 5, blocks[5].copy()))
+        assert abs(Ee - w0) <= 1e-9 * max(1.0, abs(w0))
+    with pytest.raises(ValueError):
+        srv.trial_alone(Request(0, "trial", 0, 0.0, srv.Ks[0], blocks[0].copy()))
+    with pytest.raises(ValueError):
+        srv.run([Request(0, "trial", 7, 0.0, 0, blocks[0].copy())])
