@@ -147,6 +147,47 @@ def outprod_leg(eng, xd, K, reps=5):
             "host_buffers_pairs_per_s": pairs / th, "energy": float(f.item())}
 
 
+def optimizer_leg(maxiter=20):
+    """LiH K = 12, seed 0: a 20-iteration truncated-Newton run (LMQN constants
+    through scipy TNC) driven by the GPU outprod.  The same run on the host is
+    part of the CPU baseline."""
+    import nbo_correlatedgaussians_amd as necg
+    from nbo_correlatedgaussians_amd.driver import optimize
+    from nbo_correlatedgaussians_amd.engine import Engine
+    s = necg.lih_system()
+    K = 12
+    iv, C = necg.generate_parameters(s, K, 0)
+    x0 = necg.pack_x(iv, C)
+    eng = Engine(s, "cuda:0")
+    E0 = eng.outprod(x0)[0]
+    t0 = time.perf_counter()
+    r = optimize(eng.outprod, x0, method="TN", maxiter=maxiter, siv=K * s.blk, M=K)
+    return {"workload": f"LiH K={K} seed 0: one outprod and a {maxiter}-iteration truncated "
+                        f"Newton run (scipy TNC with the LMQN constants)",
+            "maxiter": maxiter, "E0": E0, "seconds": time.perf_counter() - t0, "nit": r.nit,
+            "nfev": r.nfev, "E_final": r.fun, "energies": r.energies}
+
+
+def cpu_optimizer(opt):
+    """The optimizer leg's run with the oracle (one thread) as the objective."""
+    import nbo_correlatedgaussians_amd as necg
+    from nbo_correlatedgaussians_amd.driver import optimize
+    from oracle.oracle import Oracle
+    s = necg.lih_system()
+    K = 12
+    iv, C = necg.generate_parameters(s, K, 0)
+    x0 = necg.pack_x(iv, C)
+    o = Oracle(s)
+    t0 = time.perf_counter()
+    r = optimize(lambda x: o.outprod(x, 1), x0, method="TN", maxiter=opt["maxiter"],
+                 siv=K * s.blk, M=K)
+    out = {"seconds": time.perf_counter() - t0, "cores": 1, "nit": r.nit, "nfev": r.nfev,
+           "E_final": r.fun, "energies": r.energies}
+    if r.energies and opt["energies"]:
+        out["first_iteration_dE_vs_gpu"] = abs(r.energies[0] - opt["energies"][0])
+    return out
+
+
 def properties_leg(eng, xd, K, reps=5):
     """The driver's dipole pass (both sides symmetrised: nst^2 term pairs per
     basis pair) on the headline basis (LiH K = 2048, seed 1)."""
@@ -220,6 +261,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-props", action="store_true")
+    ap.add_argument("--no-opt", action="store_true")
     ap.add_argument("--path", choices=("autograd", "outprod"), default="autograd",
                     help="timed step: E = ecg_energy(x); E.backward(), or outprod(x)")
     ap.add_argument("--serve-requests", type=int, default=10000)
@@ -319,6 +361,9 @@ def main():
     if not args.no_props:
         out["properties"] = properties_leg(eng, xd, K)
         print(f"bench: properties leg done", file=sys.stderr, flush=True)
+    if not args.no_opt:
+        out["optimizer"] = optimizer_leg()
+        print(f"bench: optimizer leg done", file=sys.stderr, flush=True)
     if not args.no_batch:
         out["batched"] = batched_leg()
         print(f"bench: batched leg done", file=sys.stderr, flush=True)
@@ -328,6 +373,8 @@ def main():
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(sysd, K, x, args.cpu_rows, threads)
+        if "optimizer" in out:
+            out["cpu_baseline"]["optimizer"] = cpu_optimizer(out["optimizer"])
     print(json.dumps(out), flush=True)
 
 

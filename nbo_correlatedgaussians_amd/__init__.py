@@ -4,10 +4,10 @@ NECG)."""
 
 from .system import (System, read_input, write_input, format_input,
                      parse_tokens, lih_system, he_system, h2_system,
-                     hydrogen_atom, generate_parameters, pack_x, unpack_x,
+                     be_system, c_system, hydrogen_atom, generate_parameters, pack_x, unpack_x,
                      lih_oldquit_basis)
 
 __all__ = ["System", "read_input", "write_input", "format_input",
            "parse_tokens", "lih_system", "he_system", "h2_system",
-           "hydrogen_atom", "generate_parameters", "pack_x", "unpack_x",
+           "be_system", "c_system", "hydrogen_atom", "generate_parameters", "pack_x", "unpack_x",
            "lih_oldquit_basis"]

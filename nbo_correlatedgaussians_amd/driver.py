@@ -101,13 +101,30 @@ class OptResult:
     seconds: float = 0.0
 
 
+def write_punch(path: str, x) -> None:
+    """The restart punch file of the truncated-Newton driver: x, one value
+    per line, rewritten after every outer iteration (source/tn.f:338-347);
+    read_punch reads it back."""
+    with open(path + ".tmp", "w", encoding="utf-8") as fh:
+        for v in np.asarray(x, dtype=np.float64):
+            fh.write(f"{float(v)!r}\n")
+    import os
+    os.replace(path + ".tmp", path)
+
+
+def read_punch(path: str) -> np.ndarray:
+    with open(path, "r", encoding="utf-8") as fh:
+        return np.array([float(t.replace("D", "E").replace("d", "e")) for t in fh.read().split()])
+
+
 def optimize(fun, x0, method: str = "L-BFGS-B", maxiter: int = 20, siv: int = None,
-             M: int = None, options: dict = None):
+             M: int = None, options: dict = None, punch: str = None):
     """Minimise E(x) with the analytic gradient of ``fun(x) -> (E, g)``.
 
     method "L-BFGS-B": the reference's call (python/necg.py:159-171).
     method "TN": truncated Newton with the LMQN constants, through scipy's TNC,
     stopped after ``maxiter`` outer iterations.
+    punch: a path to write x to after every outer iteration (TNRESTRT.PT).
     """
     from scipy.optimize import minimize
     x0 = np.asarray(x0, dtype=np.float64)
@@ -125,8 +142,12 @@ def optimize(fun, x0, method: str = "L-BFGS-B", maxiter: int = 20, siv: int = No
     if method.upper() == "L-BFGS-B":
         opts = {"maxiter": maxiter, "maxfun": 150 * (siv + M), "ftol": EPS, "gtol": np.sqrt(EPS)}
         opts.update(options or {})
-        res = minimize(f, x0, method="L-BFGS-B", jac=True, options=opts,
-                       callback=lambda xk: energies.append(fun(xk)[0]))
+        def cb_l(xk):
+            energies.append(fun(xk)[0])
+            if punch:
+                write_punch(punch, xk)
+
+        res = minimize(f, x0, method="L-BFGS-B", jac=True, options=opts, callback=cb_l)
         return OptResult(res.x, float(res.fun), int(res.nit), int(res.nfev), str(res.message),
                          energies, time.perf_counter() - t0)
     if method.upper() in ("TN", "TNC"):
@@ -140,6 +161,8 @@ def optimize(fun, x0, method: str = "L-BFGS-B", maxiter: int = 20, siv: int = No
             E = fun(xk)[0]
             energies.append(E)
             last["x"], last["f"] = xk.copy(), E
+            if punch:
+                write_punch(punch, xk)
             if state["nit"] >= maxiter:
                 raise _Stop
 

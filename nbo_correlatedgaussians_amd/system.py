@@ -28,6 +28,8 @@ _DATA = os.path.join(os.path.dirname(__file__), "data")
 M_PROTON = 1836.15267
 M_LI7_NUCLEUS = 12786.39
 M_HE4_NUCLEUS = 7294.2995
+M_BE9_NUCLEUS = 16424.2037    # 9.0121831 u less 4 electrons
+M_C12_NUCLEUS = 21868.6617    # 12 u less 6 electrons
 
 
 @dataclass
@@ -275,7 +277,31 @@ def hydrogen_atom() -> System:
     return _from_perms("H", 1, 0.0, mass, [1.0, -1.0], [[0]], [1.0])
 
 
-SYSTEMS = {"LiH": lih_system, "He": he_system, "H2": h2_system}
+def be_system() -> System:
+    """Be-like atom, nucleus as reference: four electrons (n = 4), the group
+    {E, P12, P34, P12 P34} with weights [1, 1, 1, 1] and a Stark field.
+    A test system for n = 4 (not a spin-adapted projector)."""
+    mass = inverse_mass_tensor(M_BE9_NUCLEUS, [1.0] * 4)
+    return _from_perms("Be", 4, 0.001, mass, [4.0, -1.0, -1.0, -1.0, -1.0],
+                       [[0, 1, 2, 3], [1, 0, 2, 3], [0, 1, 3, 2], [1, 0, 3, 2]],
+                       [1.0, 1.0, 1.0, 1.0])
+
+
+def c_system() -> System:
+    """C-like atom, nucleus as reference: six electrons (n = 6), the group
+    {E, a, b, ab} with a = P12, b = P34 P56 and characters [1, -1, 1, -1], and
+    a Stark field.  A test system for n = 6 (not a spin-adapted projector)."""
+    mass = inverse_mass_tensor(M_C12_NUCLEUS, [1.0] * 6)
+    e = [0, 1, 2, 3, 4, 5]
+    a = [1, 0, 2, 3, 4, 5]
+    b = [0, 1, 3, 2, 5, 4]
+    ab = [1, 0, 3, 2, 5, 4]
+    return _from_perms("C", 6, -0.0005, mass, [6.0] + [-1.0] * 6, [e, a, b, ab],
+                       [1.0, -1.0, 1.0, -1.0])
+
+
+SYSTEMS = {"LiH": lih_system, "He": he_system, "H2": h2_system, "Be": be_system,
+           "C": c_system}
 
 
 # ---------------------------------------------------------------------------

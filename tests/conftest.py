@@ -26,7 +26,8 @@ def golden(name):
 
 def system_by_name(name):
     import nbo_correlatedgaussians_amd as necg
-    return {"LiH": necg.lih_system, "He": necg.he_system, "H2": necg.h2_system}[str(name)]()
+    return {"LiH": necg.lih_system, "He": necg.he_system, "H2": necg.h2_system,
+            "Be": necg.be_system, "C": necg.c_system}[str(name)]()
 
 
 FULL_CASES = ["lih_oldquit", "lih_k12_s0", "he_k8_s3", "h2_k8_s4", "lih_k32_s2"]

@@ -1,0 +1,35 @@
+"""The optimiser driver on the CPU, with the oracle as the objective: the
+truncated-Newton run with the LMQN constants and its restart punch file
+(source/tn.f:338-347), and L-BFGS-B with the reference's options."""
+
+import numpy as np
+import pytest
+
+import nbo_correlatedgaussians_amd as necg
+from nbo_correlatedgaussians_amd.driver import optimize, read_punch, write_punch
+
+
+def test_punch_roundtrip(tmp_path):
+    x = np.random.default_rng(0).normal(size=37)
+    p = str(tmp_path / "TNRESTRT.PT")
+    write_punch(p, x)
+    np.testing.assert_array_equal(read_punch(p), x)
+    with open(p, "w") as fh:                     # Fortran list-directed output
+        fh.write("  1.2345678901234567D+00\n -2.5D-03\n")
+    np.testing.assert_array_equal(read_punch(p), [1.2345678901234567, -2.5e-3])
+
+
+@pytest.mark.parametrize("method", ["TN", "L-BFGS-B"])
+def test_optimize_with_oracle_writes_punch(oracle_lib, tmp_path, method):
+    s = necg.he_system()
+    iv, C = necg.generate_parameters(s, 4, 3)
+    x0 = necg.pack_x(iv, C)
+    o = oracle_lib.Oracle(s)
+    p = str(tmp_path / "TNRESTRT.PT")
+    r = optimize(o.outprod, x0, method=method, maxiter=3, siv=4 * s.blk, M=4, punch=p)
+    assert 1 <= r.nit <= 3 and len(r.energies) == r.nit
+    assert r.energies[-1] < o.outprod(x0)[0]
+    assert all(b <= a + 1e-12 for a, b in zip(r.energies, r.energies[1:]))
+    xp = read_punch(p)
+    assert xp.size == x0.size
+    assert abs(o.outprod(xp)[0] - r.energies[-1]) <= 1e-12 * abs(r.energies[-1])

@@ -21,7 +21,8 @@ def _mixed_small():
     for name in ("lih_k12_s0", "he_k8_s3", "h2_k8_s4"):
         d = golden(name)
         cases.append((system_by_name(d["system"]), d["x"], d))
-    for name, K, seed in (("LiH", 70, 11), ("He", 130, 12), ("H2", 200, 13), ("LiH", 1, 14)):
+    for name, K, seed in (("LiH", 70, 11), ("He", 130, 12), ("H2", 200, 13), ("LiH", 1, 14),
+                          ("Be", 40, 15), ("C", 33, 16)):
         s = system_by_name(name)
         iv, C = necg.generate_parameters(s, K, seed)
         cases.append((s, necg.pack_x(iv, C), None))

@@ -109,7 +109,8 @@ def test_pair_sums_vs_reference_golden(case):
 
 
 @pytest.mark.parametrize("name,K,seed", [("LiH", 40, 3), ("He", 64, 4), ("H2", 64, 5),
-                                         ("LiH", 130, 6)])
+                                         ("LiH", 130, 6), ("Be", 48, 7), ("C", 40, 8),
+                                         ("C", 1, 9)])
 def test_outprod_vs_oracle(oracle_lib, name, K, seed):
     s = system_by_name(name)
     iv, C = necg.generate_parameters(s, K, seed)
@@ -205,6 +206,29 @@ def test_error_paths():
     # the latch is cleared: a good call afterwards succeeds
     E, g = eng.outprod(necg.pack_x(iv, C))
     assert math.isfinite(E)
+
+
+@pytest.mark.parametrize("name,K,seed", [("Be", 30, 21), ("C", 26, 22)])
+def test_n4_n6_energy_vjp_and_dhds(oracle_lib, name, K, seed):
+    """n = 4 and n = 6: the element-only, adjoint and materialised-derivative
+    paths against the oracle."""
+    s = system_by_name(name)
+    iv, C = necg.generate_parameters(s, K, seed)
+    x = necg.pack_x(iv, C)
+    eng = engine_for(name)
+    o = oracle_lib.Oracle(s)
+    Eo, go, Ho, So = o.outprod(x, return_hs=True)
+    E, H, S = eng.energy(x, with_hs=True)
+    assert close_E(E, Eo)
+    assert np.max(np.abs(H - Ho)) <= 1e-12 * np.max(np.abs(Ho))
+    gx = eng.vjp_t(x).cpu().numpy()
+    eng.check()
+    assert close_g(gx, go)
+    DH, DS = eng.dhds(iv, K)
+    _, _, dh, ds = o.pair_sums(iv, [(3, 5), (5, 3), (7, 7)])
+    for t, (l, k) in enumerate([(3, 5), (5, 3), (7, 7)]):
+        assert np.max(np.abs(DH[l * s.blk:(l + 1) * s.blk, k] - dh[t])) <= 1e-11 * np.max(np.abs(dh))
+        assert np.max(np.abs(DS[l * s.blk:(l + 1) * s.blk, k] - ds[t])) <= 1e-11 * np.max(np.abs(ds))
 
 
 @pytest.mark.parametrize("case", ["lih_oldquit", "lih_k32_s2"])

@@ -23,7 +23,8 @@ def _close(gpu, ref):
 
 
 @pytest.mark.parametrize("name,K,seed", [("He", 24, 51), ("H2", 30, 52), ("LiH", 40, 53),
-                                         ("LiH", 1, 54), ("He", 150, 55)])
+                                         ("LiH", 1, 54), ("He", 150, 55), ("Be", 20, 56),
+                                         ("C", 14, 57)])
 def test_dipole_vs_oracle(oracle_lib, name, K, seed):
     from nbo_correlatedgaussians_amd.engine import Engine
     s = system_by_name(name)

@@ -133,7 +133,8 @@ def test_gaussian_product_theorem(O):
 
 # ------------------------------------------------------------- derivatives
 
-@pytest.mark.parametrize("name,K,seed", [("LiH", 5, 0), ("He", 5, 1), ("H2", 5, 2)])
+@pytest.mark.parametrize("name,K,seed", [("LiH", 5, 0), ("He", 5, 1), ("H2", 5, 2),
+                                         ("Be", 5, 3), ("C", 4, 4)])
 def test_oracle_gradient_central_fd(O, name, K, seed):
     s = system_by_name(name)
     o = O(s)
