@@ -235,18 +235,27 @@ def traffic_probe(K, seed):
     print("traffic probe done", flush=True)
 
 
-def recorded_traffic():
-    """HBM bytes per pair-kernel launch measured by the counter passes
-    (profiles/pair_kernel_lih_k2048_traffic.md), or None."""
-    path = os.path.join(ROOT, "profiles", "pair_kernel_lih_k2048_traffic.md")
+def _recorded(fname, key):
+    """A figure recorded by a counter pass in profiles/<fname>, or None."""
+    path = os.path.join(ROOT, "profiles", fname)
     try:
         with open(path, encoding="utf-8") as fh:
             for line in fh:
-                if line.startswith("traffic_bytes_per_launch:"):
+                if line.startswith(key + ":"):
                     return float(line.split(":")[1])
     except OSError:
         pass
     return None
+
+
+def recorded_traffic():
+    """HBM bytes per pair-kernel launch (FETCH_SIZE/WRITE_SIZE passes)."""
+    return _recorded("pair_kernel_lih_k2048_traffic.md", "traffic_bytes_per_launch")
+
+
+def recorded_executed_flop():
+    """Executed FP64 FLOP per pair-kernel launch (SQ_INSTS_VALU_*_F64 passes)."""
+    return _recorded("pair_kernel_lih_k2048_fp64_counters.md", "executed_flop_per_launch")
 
 
 def main():
@@ -326,6 +335,7 @@ def main():
     pair_ms = ms_pair / max(launches, 1)
     achieved = fl / (pair_ms / 1000.0) / 1e12
     kt, ts, w = eng.tiling()
+    ex_fl = recorded_executed_flop() if (K == 2048 and args.seed == 1) else None
     out = {
         "metric": "ECG pair energy+grad evals/s (FP64, % of peak); served p50/p99 latency",
         "value": value,
@@ -352,6 +362,10 @@ def main():
                      "traffic": recorded_traffic() if (K == 2048 and args.seed == 1) else None,
                      "traffic_source": "profiles/pair_kernel_lih_k2048_traffic.md",
                      "flop_per_launch": fl, "pair_kernel_ms": pair_ms,
+                     "executed_flop_per_launch": ex_fl,
+                     "executed_frac": (ex_fl / (pair_ms / 1000.0) / 1e12 / PEAK_FP64_TFLOPS
+                                       if ex_fl else None),
+                     "executed_source": "profiles/pair_kernel_lih_k2048_fp64_counters.md",
                      "tiling": {"ket_tile": kt, "term_split": ts, "waves": w}},
         "energy": float(E.item()),
     }

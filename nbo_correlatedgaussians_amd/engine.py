@@ -287,3 +287,32 @@ class Engine:
         _lib.raise_for(rc, "necg_dhds")
         self.check("dhds")
         return DH.cpu().numpy(), DS.cpu().numpy()
+
+
+class EngineGraph:
+    """One system's evaluation (``outprod``: E and g) captured in a HIP graph:
+    write new parameters into ``x`` in place, call ``replay()``, read ``f``
+    and ``g``.  The capture follows one warm-up call on a side stream, which
+    sizes the engine's workspace, so the captured sequence holds kernel
+    launches only."""
+
+    def __init__(self, engine: Engine, x0):
+        torch = _torch()
+        self.engine = engine
+        self.x = engine._dev(x0).clone()
+        self.f = engine._empty(1)
+        self.g = engine._empty(self.x.numel())
+        dev = engine.device
+        s = torch.cuda.Stream(device=dev)
+        s.wait_stream(torch.cuda.current_stream(dev))
+        with torch.cuda.stream(s):
+            engine.outprod_t(self.x, self.f, self.g)
+        torch.cuda.current_stream(dev).wait_stream(s)
+        engine.check("graph warm-up")
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            engine.outprod_t(self.x, self.f, self.g)
+
+    def replay(self):
+        self.graph.replay()
+        return self.f, self.g

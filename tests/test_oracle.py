@@ -258,3 +258,44 @@ def test_dipole_translation_covariance(O, name, K):
     assert abs(css2 - css) <= 1e-12 * abs(css)
     np.testing.assert_allclose(rbar2 - rbar, np.tile(t, (s.n, 1)), rtol=0, atol=1e-12)
     np.testing.assert_allclose(dip2 - dip, t * s.ncv[1:].sum(), rtol=0, atol=1e-11)
+
+
+def _rotate_shifts(s, x, K, R):
+    """x with every particle's shift of every function rotated by R."""
+    y = x.copy()
+    blocks = y[:K * s.blk].reshape(K, s.blk)
+    sh = blocks[:, s.svh:].reshape(K, s.n, 3)
+    blocks[:, s.svh:] = (sh @ R.T).reshape(K, -1)
+    return y
+
+
+def _rot(axis, ang):
+    c, sn = math.cos(ang), math.sin(ang)
+    if axis == "z":
+        return np.array([[c, -sn, 0], [sn, c, 0], [0, 0, 1.0]])
+    return np.array([[c, 0, sn], [0, 1.0, 0], [-sn, 0, c]])
+
+
+@pytest.mark.parametrize("name,axis", [("LiH", "z"), ("He", "y"), ("H2", "y"), ("Be", "z")])
+def test_oracle_rotation_invariance(O, name, axis):
+    """A rotation of all shifts leaves E unchanged and rotates the shift
+    gradients (about z only when the Stark field is on: LiH, Be)."""
+    s = system_by_name(name)
+    o = O(s)
+    K = 6
+    iv, C = necg.generate_parameters(s, K, 17)
+    x = necg.pack_x(iv, C)
+    R = _rot(axis, 0.7)
+    E, g = o.outprod(x)
+    E2, g2 = o.outprod(_rotate_shifts(s, x, K, R))
+    assert abs(E2 - E) <= 1e-12 * max(1.0, abs(E))
+    gs = g[:K * s.blk].reshape(K, s.blk)[:, s.svh:].reshape(K, s.n, 3)
+    gs2 = g2[:K * s.blk].reshape(K, s.blk)[:, s.svh:].reshape(K, s.n, 3)
+    scale = np.max(np.abs(g))
+    assert np.max(np.abs(gs2 - gs @ R.T)) <= 1e-10 * scale
+    gl = g[:K * s.blk].reshape(K, s.blk)[:, :s.svh]
+    gl2 = g2[:K * s.blk].reshape(K, s.blk)[:, :s.svh]
+    assert np.max(np.abs(gl2 - gl)) <= 1e-10 * scale
+    if s.efs != 0.0:     # a rotation about another axis does change E
+        E3, _ = o.outprod(_rotate_shifts(s, x, K, _rot("y", 0.7)))
+        assert abs(E3 - E) > 1e-9 * abs(s.efs)
