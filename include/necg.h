@@ -110,6 +110,15 @@ int necg_matrices(necg_ctx *ctx, int32_t K, const double *iv, double *H, double 
  * Async. */
 int necg_dipole(necg_ctx *ctx, int32_t K, const double *x, double *out, void *stream);
 
+/* How necg_dipole sums the nst^2 term pairs: 0 (default) uses the reduced
+ * form when every term is a permutation with sms = kron(sml^T, I3) and every
+ * product sml[q] sml[p]^T is a term (nst term pairs per basis pair, with
+ * per-term weights summed over the (p, q) that map to it), else the full
+ * double sum; 1 forces the reduced form (NECG_ERR_UNSUPPORTED from
+ * necg_dipole if the terms do not allow it); 2 forces the full double sum.
+ * No counterpart in the reference, which always forms the double sum. */
+int necg_set_dipole_mode(necg_ctx *ctx, int32_t mode);
+
 int necg_kinetic_potential(necg_ctx *ctx, int32_t K, const double *x, double *T, double *V,
                            void *stream);
 

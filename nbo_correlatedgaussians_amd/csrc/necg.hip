@@ -606,7 +606,7 @@ int necg_destroy(necg_ctx *c) {
     DeviceGuard g(c->device);
     double *bufs[] = {c->d_scvv, c->d_sml, c->d_sms, c->bra, c->ket, c->H, c->S,
                       c->HC, c->SC, c->scal, c->part, c->hpart, c->spart, c->hcp, c->scp, c->red,
-                      c->prop_rec, c->prop_part};
+                      c->prop_rec, c->prop_part, c->prop_tab};
     for (double *p : bufs)
       if (p) (void)hipFree(p);
     if (c->d_flag) (void)hipFree(c->d_flag);

@@ -28,6 +28,11 @@ struct necg_ctx {
   // properties pass: (function, term) records and per-wave partial rows
   double *prop_rec = nullptr, *prop_part = nullptr;
   size_t prop_rec_cap = 0, prop_part_cap = 0;
+  // symmetry-reduced properties pass: requested mode (0 auto, 1 reduced,
+  // 2 all term pairs), whether the terms were analysed and close under
+  // Q P^T, the identity term, and the per-term weight table
+  int prop_mode = 0, prop_analysed = 0, prop_grp_ok = 0, prop_ident = -1;
+  double *prop_tab = nullptr;
   int req_kt = 0, req_ts = 0, req_w = 0;
   int last_kt = 0, last_ts = 0, last_w = 0;
   int last_K = 0;

@@ -15,6 +15,15 @@
 // and w ov st to r.  Pairs k > l count twice (the reference mirrors its lower
 // triangle before contracting with C).
 //
+// Symmetry reduction.  When every term is a permutation (sml[p] = P, sms[p] =
+// kron(P^T, I3)) and every product Q P^T is again a term, the transformed
+// pair (P k, Q l) is P applied to the pair (k, R l) with R = Q P^T: the same
+// ov and det, and st(P k, Q l) = P^T st(k, R l).  The nst^2 term pairs then
+// collapse to nst: for each term r, w_r = sum of scvv_p scvv_q over the (p, q)
+// with Q P^T = R_r, and W_r = the same sum of c_p c_q P^T, so that the pair
+// (k, l) adds w_r ov and ov W_r st(k, R_r l) (bra k untransformed).  Terms
+// that do not close keep the full double sum.
+//
 // Launch: one wave per (64 consecutive (k, p) records, ket tile of functions);
 // the bra record sits in registers, the ket records (l, q) are wave-uniform
 // loads; each wave folds its lanes into one partial row (fixed butterfly), and
@@ -122,15 +131,16 @@ __global__ void __launch_bounds__(256)
   for (int i = 0; i < N; ++i) det *= L[i][i];
   const double dk = fabs(det);
   const double Ck = x[(size_t)K * R::BLK + k];
-  o[R::F] = exp2(0.75 * N) * dk * sqrt(dk) * Ck * scvv[p];
+  o[R::F] = exp2(0.75 * N) * dk * sqrt(dk) * Ck * (scvv ? scvv[p] : 1.0);
   if (!isfinite(i1) || !isfinite(dk) || !isfinite(Ck)) atomicOr(flag, (int)F_BAD_INPUT);
   else if (!(dk > 0.0)) atomicOr(flag, (int)F_NOT_PD);  // A_k singular (a zero L diagonal)
 }
 
-// One term pair: bra record b (registers), ket record r (wave-uniform).
+// One term pair: bra record b (registers), ket record r (wave-uniform);
+// returns w ov and st = M^-1 e.
 template <int N>
-__device__ __forceinline__ void prop_term(const double *b, const double *__restrict__ r,
-                                          double mult, double *acc, bool &pd) {
+__device__ __forceinline__ double prop_core(const double *b, const double *__restrict__ r,
+                                            double mult, double (&st)[N][3], bool &pd) {
   using R = PR<N>;
   double M[R::NS];
 #pragma unroll
@@ -171,7 +181,6 @@ __device__ __forceinline__ void prop_term(const double *b, const double *__restr
       for (int m = 0; m < i; ++m) v -= Lm[i][m] * y[m][c];
       y[i][c] = v;
     }
-  double st[N][3];
 #pragma unroll
   for (int i = N - 1; i >= 0; --i)
 #pragma unroll
@@ -186,13 +195,38 @@ __device__ __forceinline__ void prop_term(const double *b, const double *__restr
   for (int i = 0; i < N; ++i)
 #pragma unroll
     for (int c = 0; c < 3; ++c) i3 += e[i][c] * st[i][c];
-  const double wov =
-      mult * b[R::F] * r[R::F] * (invdet * sqrt(invdet)) * exp(i3 - b[R::I1] - r[R::I1]);
+  return mult * b[R::F] * r[R::F] * (invdet * sqrt(invdet)) * exp(i3 - b[R::I1] - r[R::I1]);
+}
+
+template <int N>
+__device__ __forceinline__ void prop_term(const double *b, const double *__restrict__ r,
+                                          double mult, double *acc, bool &pd) {
+  double st[N][3];
+  const double wov = prop_core<N>(b, r, mult, st, pd);
   acc[0] += wov;
 #pragma unroll
   for (int i = 0; i < N; ++i)
 #pragma unroll
     for (int c = 0; c < 3; ++c) acc[1 + i * 3 + c] += wov * st[i][c];
+}
+
+// The reduced form: tb = [w_r, W_r (n x n row-major)] of the ket's term.
+template <int N>
+__device__ __forceinline__ void prop_term_grp(const double *b, const double *__restrict__ r,
+                                              const double *__restrict__ tb, double mult,
+                                              double *acc, bool &pd) {
+  double st[N][3];
+  const double ov = prop_core<N>(b, r, mult, st, pd);
+  acc[0] = __builtin_fma(ov, tb[0], acc[0]);
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      double v = 0.0;
+#pragma unroll
+      for (int j = 0; j < N; ++j) v = __builtin_fma(tb[1 + i * N + j], st[j][c], v);
+      acc[1 + i * 3 + c] = __builtin_fma(ov, v, acc[1 + i * 3 + c]);
+    }
 }
 
 template <int N>
@@ -225,6 +259,56 @@ __global__ void __launch_bounds__(64)
       const double *kr = rec + (size_t)l * nst * R::RS;
       for (int q = 0; q < nst; ++q, kr += R::RS) {
         if (on) prop_term<N>(b, kr, mult, acc, pd);
+      }
+    }
+  }
+#pragma unroll
+  for (int v = 0; v < R::NV; ++v) {
+    double a = acc[v];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
+    acc[v] = a;
+  }
+  if (lane == 0) {
+    double *out = part + ((size_t)t * gridDim.x + nb) * R::NV;
+#pragma unroll
+    for (int v = 0; v < R::NV; ++v) out[v] = acc[v];
+  }
+  if (!pd) atomicOr(flag, (int)F_NOT_PD);
+}
+
+// Reduced form: lanes are bra functions k (record of the identity term), the
+// ket records (l, r) and the table rows of r are wave-uniform.
+template <int N>
+__global__ void __launch_bounds__(64)
+    prop_pair_grp(const double *__restrict__ rec, const double *__restrict__ tab, int K, int nst,
+                  int ident, int ket_tile, double *__restrict__ part, int *__restrict__ flag) {
+  using R = PR<N>;
+  constexpr int TS = 1 + N * N;
+  const int lane = threadIdx.x;
+  const int nb = blockIdx.x, t = blockIdx.y;
+  const int k0 = nb * 64;
+  const int k = k0 + lane;
+  const bool valid = k < K;
+  const int kmax = min(k0 + 63, K - 1);
+  const int l0 = t * ket_tile;
+  const int l1 = min(min(K, l0 + ket_tile), kmax + 1);
+  double acc[R::NV];
+#pragma unroll
+  for (int v = 0; v < R::NV; ++v) acc[v] = 0.0;
+  bool pd = true;
+  if (l0 < l1) {
+    double b[R::RS];
+    const double *br = rec + ((size_t)(valid ? k : 0) * nst + ident) * R::RS;
+#pragma unroll
+    for (int v = 0; v < R::RS; ++v) b[v] = br[v];
+    for (int l = l0; l < l1; ++l) {
+      const double mult = l < k ? 2.0 : 1.0;
+      const bool on = valid && l <= k;
+      const double *kr = rec + (size_t)l * nst * R::RS;
+      const double *tb = tab;
+      for (int r = 0; r < nst; ++r, kr += R::RS, tb += TS) {
+        if (on) prop_term_grp<N>(b, kr, tb, mult, acc, pd);
       }
     }
   }
@@ -300,32 +384,128 @@ struct DevSwitch {
   }
 };
 
-// Ket tile (functions) so that about 8192 (bra block, ket tile) waves carry work.
+// Ket tile (functions) so that about 8192 (bra block, ket tile) waves carry
+// work (about half of the grid: only ket tiles at or below the block's bras).
 int prop_ket_tile(int K, int nblocks) {
   const long long t = ((long long)nblocks * K) / (2LL * 8192);
   return (int)std::max(1LL, std::min<long long>(t, K));
 }
 
+// Analyse the terms once per context: permutations with sms = kron(P^T, I3)
+// whose products Q P^T are all terms give the reduced form and its table
+// [w_r, W_r] per term r.
+int prop_analyse(necg_ctx *c) {
+  if (c->prop_analysed) return NECG_OK;
+  const int n = c->n, nst = c->nst, n3 = 3 * n;
+  std::vector<double> sml((size_t)nst * n * n), sms((size_t)nst * n3 * n3), w(nst);
+  int rc;
+  if ((rc = hip_rc(hipMemcpy(sml.data(), c->d_sml, sizeof(double) * sml.size(),
+                             hipMemcpyDeviceToHost))) ||
+      (rc = hip_rc(hipMemcpy(sms.data(), c->d_sms, sizeof(double) * sms.size(),
+                             hipMemcpyDeviceToHost))) ||
+      (rc = hip_rc(hipMemcpy(w.data(), c->d_scvv, sizeof(double) * nst, hipMemcpyDeviceToHost))))
+    return rc;
+  auto P = [&](int p, int i, int j) { return sml[((size_t)p * n + i) * n + j]; };
+  bool ok = true;
+  for (int p = 0; p < nst && ok; ++p) {
+    for (int i = 0; i < n && ok; ++i) {
+      int rows = 0, cols = 0;
+      for (int j = 0; j < n; ++j) {
+        const double a = P(p, i, j), b = P(p, j, i);
+        if ((a != 0.0 && a != 1.0) || (b != 0.0 && b != 1.0)) ok = false;
+        rows += a == 1.0;
+        cols += b == 1.0;
+      }
+      ok = ok && rows == 1 && cols == 1;
+    }
+    for (int r = 0; r < n3 && ok; ++r)
+      for (int q = 0; q < n3 && ok; ++q) {
+        const double want = (r % 3 == q % 3) ? P(p, q / 3, r / 3) : 0.0;
+        ok = sms[((size_t)p * n3 + r) * n3 + q] == want;
+      }
+  }
+  int ident = -1;
+  std::vector<double> tab;
+  if (ok) {
+    auto find = [&](const std::vector<double> &m) {
+      for (int r = 0; r < nst; ++r)
+        if (std::equal(m.begin(), m.end(), sml.begin() + (size_t)r * n * n)) return r;
+      return -1;
+    };
+    std::vector<double> eye((size_t)n * n, 0.0);
+    for (int i = 0; i < n; ++i) eye[(size_t)i * n + i] = 1.0;
+    ident = find(eye);
+    const int ts = 1 + n * n;
+    tab.assign((size_t)nst * ts, 0.0);
+    std::vector<double> m((size_t)n * n);
+    for (int p = 0; p < nst && ok; ++p)
+      for (int q = 0; q < nst && ok; ++q) {
+        for (int i = 0; i < n; ++i)  // Q P^T
+          for (int j = 0; j < n; ++j) {
+            double v = 0.0;
+            for (int t = 0; t < n; ++t) v += P(q, i, t) * P(p, j, t);
+            m[(size_t)i * n + j] = v;
+          }
+        const int r = find(m);
+        if (r < 0) {
+          ok = false;
+          break;
+        }
+        const double cw = w[p] * w[q];
+        tab[(size_t)r * ts] += cw;
+        for (int i = 0; i < n; ++i)
+          for (int j = 0; j < n; ++j) tab[(size_t)r * ts + 1 + i * n + j] += cw * P(p, j, i);
+      }
+    ok = ok && ident >= 0;
+  }
+  if (ok) {
+    size_t cap = 0;  // the table is made once per context
+    if ((rc = grow(&c->prop_tab, cap, tab.size()))) return rc;
+    if ((rc = hip_rc(hipMemcpy(c->prop_tab, tab.data(), sizeof(double) * tab.size(),
+                               hipMemcpyHostToDevice))))
+      return rc;
+  }
+  c->prop_grp_ok = ok ? 1 : 0;
+  c->prop_ident = ident;
+  c->prop_analysed = 1;
+  return NECG_OK;
+}
+
 template <int N>
 int do_dipole(necg_ctx *c, int K, const double *x, double *out, hipStream_t st) {
   using R = PR<N>;
+  int rc;
+  if ((rc = prop_analyse(c))) return rc;
+  const bool grp = c->prop_mode == 1 || (c->prop_mode == 0 && c->prop_grp_ok);
+  if (grp && !c->prop_grp_ok) return NECG_ERR_UNSUPPORTED;
   const long long G = (long long)K * c->nst;
-  const int nb = (int)((G + 63) / 64);
+  const int nb = grp ? (K + 63) / 64 : (int)((G + 63) / 64);
   const int tile = prop_ket_tile(K, nb);
   const int nt = (K + tile - 1) / tile;
-  int rc;
   if ((rc = grow(&c->prop_rec, c->prop_rec_cap, (size_t)G * R::RS))) return rc;
   if ((rc = grow(&c->prop_part, c->prop_part_cap, (size_t)nb * nt * R::NV))) return rc;
   hipLaunchKernelGGL((prop_prep<N>), dim3((unsigned)((G + 255) / 256)), dim3(256), 0, st, x, K,
-                     c->nst, c->d_scvv, c->d_sml, c->d_sms, c->prop_rec, c->d_flag);
-  hipLaunchKernelGGL((prop_pair<N>), dim3(nb, nt), dim3(64), 0, st, c->prop_rec, K, c->nst, tile,
-                     c->prop_part, c->d_flag);
+                     c->nst, grp ? nullptr : c->d_scvv, c->d_sml, c->d_sms, c->prop_rec,
+                     c->d_flag);
+  if (grp)
+    hipLaunchKernelGGL((prop_pair_grp<N>), dim3(nb, nt), dim3(64), 0, st, c->prop_rec,
+                       c->prop_tab, K, c->nst, c->prop_ident, tile, c->prop_part, c->d_flag);
+  else
+    hipLaunchKernelGGL((prop_pair<N>), dim3(nb, nt), dim3(64), 0, st, c->prop_rec, K, c->nst,
+                       tile, c->prop_part, c->d_flag);
   hipLaunchKernelGGL((prop_reduce<N>), dim3(1), dim3(256), 0, st, c->prop_part, nb * nt, c->sc,
                      out, c->d_flag);
   return hip_rc(hipGetLastError());
 }
 
 }  // namespace
+
+extern "C" int necg_set_dipole_mode(necg_ctx *c, int32_t mode) {
+  if (!c || mode < 0 || mode > 2) return NECG_ERR_ARG;
+  std::lock_guard<std::mutex> lk(c->mu);
+  c->prop_mode = mode;
+  return NECG_OK;
+}
 
 extern "C" int necg_dipole(necg_ctx *c, int32_t K, const double *x, double *out, void *stream) {
   if (!c || K < 1 || !x || !out) return NECG_ERR_ARG;

@@ -189,6 +189,11 @@ class Engine:
         _lib.raise_for(rc, "necg_kinetic_potential")
         return T, V
 
+    def set_dipole_mode(self, mode: int) -> None:
+        """0: symmetry-reduced dipole pass when the terms allow it (default);
+        1: reduced form required; 2: the full nst^2 double sum."""
+        _lib.raise_for(self.lib.necg_set_dipole_mode(self._ctx, int(mode)), "necg_set_dipole_mode")
+
     def dipole_t(self, x, out=None):
         """The driver's dipole pass (python/necg.py:200-232) for x: a device
         vector [C^T s C, <r_i,a> (3n, particle-major), dipole (3)]."""

@@ -98,3 +98,29 @@ def test_dipole_error_paths():
         eng.dipole(necg.pack_x(iv, np.zeros(4)))
     css, r, d = eng.dipole(necg.pack_x(iv, C))
     assert np.isfinite(css) and np.all(np.isfinite(r))
+
+
+def test_dipole_reduced_and_full_forms_agree(oracle_lib):
+    """The symmetry-reduced pass (nst term pairs per basis pair) and the full
+    nst^2 double sum agree; a term set that does not close under Q P^T (LiH
+    without one term) falls back to the double sum and still matches the
+    oracle, and refuses the reduced form when it is forced."""
+    from nbo_correlatedgaussians_amd import _lib
+    from nbo_correlatedgaussians_amd.engine import Engine
+    s = necg.lih_system()
+    iv, C = necg.generate_parameters(s, 40, 58)
+    x = necg.pack_x(iv, C)
+    eng = Engine(s, "cuda:0")
+    red = eng.dipole(x)
+    eng.set_dipole_mode(2)
+    full = eng.dipole(x)
+    _close(red, full)
+    _close(red, oracle_lib.Oracle(s).dipole(x, nthreads=16))
+    keep = [p for p in range(s.nst) if p != 7]
+    s23 = necg.System(n=s.n, nst=len(keep), efs=s.efs, mass=s.mass, ncv=s.ncv,
+                      scvv_raw=s.scvv_raw[keep], sml=s.sml[keep], sms=s.sms[keep], name="LiH-23")
+    e23 = Engine(s23, "cuda:0")
+    _close(e23.dipole(x), oracle_lib.Oracle(s23).dipole(x, nthreads=16))
+    e23.set_dipole_mode(1)
+    with pytest.raises(_lib.NecgError):
+        e23.dipole(x)
