@@ -138,6 +138,12 @@ int necg_gradel(necg_ctx *ctx, int32_t K, const double *iv, int32_t nq, const in
  * (python/make_grad.py:34-55).  Device, K*blk*K each.  Async. */
 int necg_dhds(necg_ctx *ctx, int32_t K, const double *iv, double *DH, double *DS, void *stream);
 
+/* 1 if the GPU can address p (device, managed or pinned host memory), else
+ * 0.  Every entry point applies this check to its pointer arguments and
+ * returns NECG_ERR_ARG for plain (pageable) host memory.  No counterpart in
+ * the reference, whose arrays are host NumPy/Fortran arrays. */
+int necg_device_visible(const void *p);
+
 /* Synchronise `stream` and return the first latched error (NECG_OK if none);
  * clears the latch.  Blocking. */
 int necg_status(necg_ctx *ctx, void *stream);

@@ -618,6 +618,7 @@ int necg_destroy(necg_ctx *c) {
 
 int necg_outprod(necg_ctx *c, int32_t K, const double *x, double *f, double *g, void *stream) {
   if (!c || K < 1 || !x || !g) return NECG_ERR_ARG;
+  NECG_REQUIRE_DEVICE((const void *)x, (const void *)f, (const void *)g);
   std::lock_guard<std::mutex> lk(c->mu);
   c->tape_K = 0;
   DeviceGuard dg(c->device);
@@ -630,6 +631,7 @@ int necg_outprod(necg_ctx *c, int32_t K, const double *x, double *f, double *g, 
 int necg_energy(necg_ctx *c, int32_t K, const double *x, double *f, double *H, double *S,
                 void *stream) {
   if (!c || K < 1 || !x) return NECG_ERR_ARG;
+  NECG_REQUIRE_DEVICE((const void *)x, (const void *)f, (const void *)H, (const void *)S);
   std::lock_guard<std::mutex> lk(c->mu);
   c->tape_K = 0;
   DeviceGuard dg(c->device);
@@ -642,6 +644,7 @@ int necg_energy(necg_ctx *c, int32_t K, const double *x, double *f, double *H, d
 int necg_energy_fwd(necg_ctx *c, int32_t K, const double *x, double *f, int64_t *tape,
                     void *stream) {
   if (!c || K < 1 || !x || !tape) return NECG_ERR_ARG;
+  NECG_REQUIRE_DEVICE((const void *)x, (const void *)f);
   std::lock_guard<std::mutex> lk(c->mu);
   c->tape_K = 0;
   *tape = 0;
@@ -657,6 +660,7 @@ int necg_energy_fwd(necg_ctx *c, int32_t K, const double *x, double *f, int64_t 
 int necg_energy_bwd(necg_ctx *c, int32_t K, int64_t tape, const double *gbar, double *gx,
                     void *stream) {
   if (!c || K < 1 || !gx) return NECG_ERR_ARG;
+  NECG_REQUIRE_DEVICE((const void *)gbar, (const void *)gx);
   std::lock_guard<std::mutex> lk(c->mu);
   if (c->tape_K != K || tape == 0 || tape != c->tape_id) return NECG_ERR_STALE;
   DeviceGuard dg(c->device);
@@ -667,6 +671,7 @@ int necg_energy_bwd(necg_ctx *c, int32_t K, int64_t tape, const double *gbar, do
 int necg_energy_vjp(necg_ctx *c, int32_t K, const double *x, const double *gbar, double *gx,
                     void *stream) {
   if (!c || K < 1 || !x || !gx) return NECG_ERR_ARG;
+  NECG_REQUIRE_DEVICE((const void *)x, (const void *)gbar, (const void *)gx);
   std::lock_guard<std::mutex> lk(c->mu);
   c->tape_K = 0;
   if (c->last_K != K || K > c->Kcap) return NECG_ERR_ARG;  // needs necg_energy first
@@ -676,6 +681,7 @@ int necg_energy_vjp(necg_ctx *c, int32_t K, const double *x, const double *gbar,
 
 int necg_matrices(necg_ctx *c, int32_t K, const double *iv, double *H, double *S, void *stream) {
   if (!c || K < 1 || !iv) return NECG_ERR_ARG;
+  NECG_REQUIRE_DEVICE((const void *)iv, (const void *)H, (const void *)S);
   std::lock_guard<std::mutex> lk(c->mu);
   c->tape_K = 0;
   DeviceGuard dg(c->device);
@@ -687,6 +693,7 @@ int necg_matrices(necg_ctx *c, int32_t K, const double *iv, double *H, double *S
 int necg_kinetic_potential(necg_ctx *c, int32_t K, const double *x, double *T, double *V,
                            void *stream) {
   if (!c || K < 1 || !x || !T || !V) return NECG_ERR_ARG;
+  NECG_REQUIRE_DEVICE((const void *)x, (const void *)T, (const void *)V);
   std::lock_guard<std::mutex> lk(c->mu);
   c->tape_K = 0;
   DeviceGuard dg(c->device);
@@ -698,6 +705,7 @@ int necg_kinetic_potential(necg_ctx *c, int32_t K, const double *x, double *T, d
 int necg_matel(necg_ctx *c, int32_t K, const double *iv, int32_t nq, const int32_t *q,
                double *out, void *stream) {
   if (!c || K < 1 || !iv || nq < 0 || (nq > 0 && (!q || !out))) return NECG_ERR_ARG;
+  NECG_REQUIRE_DEVICE((const void *)iv, (const void *)q, (const void *)out);
   std::lock_guard<std::mutex> lk(c->mu);
   c->tape_K = 0;
   DeviceGuard dg(c->device);
@@ -709,6 +717,8 @@ int necg_matel(necg_ctx *c, int32_t K, const double *iv, int32_t nq, const int32
 int necg_gradel(necg_ctx *c, int32_t K, const double *iv, int32_t nq, const int32_t *q,
                 double *tgvl, double *tgvls, void *stream) {
   if (!c || K < 1 || !iv || nq < 0 || (nq > 0 && (!q || !tgvl || !tgvls))) return NECG_ERR_ARG;
+  NECG_REQUIRE_DEVICE((const void *)iv, (const void *)q, (const void *)tgvl,
+                      (const void *)tgvls);
   std::lock_guard<std::mutex> lk(c->mu);
   c->tape_K = 0;
   DeviceGuard dg(c->device);
@@ -720,6 +730,7 @@ int necg_gradel(necg_ctx *c, int32_t K, const double *iv, int32_t nq, const int3
 
 int necg_dhds(necg_ctx *c, int32_t K, const double *iv, double *DH, double *DS, void *stream) {
   if (!c || K < 1 || !iv || !DH || !DS) return NECG_ERR_ARG;
+  NECG_REQUIRE_DEVICE((const void *)iv, (const void *)DH, (const void *)DS);
   std::lock_guard<std::mutex> lk(c->mu);
   c->tape_K = 0;
   DeviceGuard dg(c->device);
@@ -727,6 +738,8 @@ int necg_dhds(necg_ctx *c, int32_t K, const double *iv, double *DH, double *DS, 
   if (rc) return rc;
   DISPATCH_N(c->n, (do_dhds<NN>(c, K, iv, DH, DS, (hipStream_t)stream)));
 }
+
+int necg_device_visible(const void *p) { return p && necg::device_visible({p}) ? 1 : 0; }
 
 int necg_status(necg_ctx *c, void *stream) {
   if (!c) return NECG_ERR_ARG;

@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "../../include/necg.h"
+#include "necg_abi.hpp"
 #include "necg_finish.hpp"
 #include "necg_pool_impl.hpp"
 
@@ -525,6 +526,7 @@ int necg_batch_set_tiling(necg_batch *B, double item_flops, int32_t grid_per_cu)
 int necg_batch_outprod_masked(necg_batch *B, const int32_t *active, const double *x, double *f,
                               double *g, void *stream) {
   if (!B || !x || !g) return NECG_ERR_ARG;
+  NECG_REQUIRE_DEVICE((const void *)active, (const void *)x, (const void *)f, (const void *)g);
   std::lock_guard<std::mutex> lk(B->mu);
   DevGuard dg(B->device);
   hipStream_t st = (hipStream_t)stream;
@@ -579,6 +581,7 @@ int necg_batch_pool(necg_batch *B, necg_pool **out) {
 
 int necg_batch_matrices(necg_batch *B, int32_t s, double *H, double *S, void *stream) {
   if (!B || s < 0 || s >= B->nsys) return NECG_ERR_ARG;
+  NECG_REQUIRE_DEVICE((const void *)H, (const void *)S);
   std::lock_guard<std::mutex> lk(B->mu);
   DevGuard dg(B->device);
   const BatchSys &b = B->hsys[s];

@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "../../include/necg.h"
+#include "necg_abi.hpp"
 #include "necg_finish.hpp"
 
 struct necg_ctx {
@@ -46,3 +47,4 @@ struct necg_ctx {
   int nev = 0;
   std::mutex mu;
 };
+

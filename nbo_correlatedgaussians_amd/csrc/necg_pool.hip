@@ -487,6 +487,8 @@ int necg_pool_trials(necg_pool *P, int32_t ntr, const int32_t *tsys, const int32
       rstride < P->kmax || pstride < 1)
     return NECG_ERR_ARG;
   if (ntr == 0) return NECG_OK;
+  NECG_REQUIRE_DEVICE((const void *)tsys, (const void *)tfun, (const void *)newp,
+                      (const void *)ws, (const void *)E, (const void *)rowH, (const void *)rowS);
   // a context-backed pool: the contexts must still hold the state the pool
   // was made from
   if (!P->ctx.empty()) {
@@ -514,6 +516,7 @@ int necg_pool_trials(necg_pool *P, int32_t ntr, const int32_t *tsys, const int32
 
 int necg_pool_set_factors(necg_pool *P, int32_t s, const double *V, const double *lam) {
   if (!P || s < 0 || s >= (int)P->hsys.size() || (!V) != (!lam)) return NECG_ERR_ARG;
+  NECG_REQUIRE_DEVICE((const void *)V, (const void *)lam);
   P->hsys[s].V = V;
   P->hsys[s].lam = lam;
   int prev = -1;
@@ -532,6 +535,8 @@ int necg_pool_eig(necg_pool *P, int32_t ntr, const int32_t *tsys, const int32_t 
       rstride < P->kmax)
     return NECG_ERR_ARG;
   if (ntr == 0) return NECG_OK;
+  NECG_REQUIRE_DEVICE((const void *)tsys, (const void *)tfun, (const void *)rowH,
+                      (const void *)rowS, (const void *)E, (const void *)bg, (const void *)Eeig);
   int prev = -1;
   (void)hipGetDevice(&prev);
   if (prev != P->device) (void)hipSetDevice(P->device);

@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "../../include/necg.h"
+#include "necg_abi.hpp"
 #include "necg_finish.hpp"
 
 struct necg_pool;

@@ -510,6 +510,7 @@ extern "C" int necg_set_dipole_mode(necg_ctx *c, int32_t mode) {
 extern "C" int necg_dipole(necg_ctx *c, int32_t K, const double *x, double *out, void *stream) {
   if (!c || K < 1 || !x || !out) return NECG_ERR_ARG;
   if ((long long)K * c->nst > (1LL << 31) - 64) return NECG_ERR_ARG;
+  NECG_REQUIRE_DEVICE((const void *)x, (const void *)out);
   std::lock_guard<std::mutex> lk(c->mu);
   DevSwitch dg(c->device);
   const hipStream_t st = (hipStream_t)stream;

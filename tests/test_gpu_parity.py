@@ -296,3 +296,27 @@ def test_engine_graph_replays():
         Eg, gg = float(f.item()), g.cpu().numpy()
         Ee, ge = eng.outprod(x)          # eager work between replays
         assert Eg == Ee and np.array_equal(gg, ge), rep
+
+
+def test_host_pointers_are_refused():
+    """Pageable host memory is refused at the boundary (NECG_ERR_ARG), not
+    handed to a kernel; device tensors pass the same check."""
+    import ctypes as ct
+    import torch
+    from nbo_correlatedgaussians_amd import _lib
+    lib = _lib.load()
+    s = necg.he_system()
+    iv, C = necg.generate_parameters(s, 4, 0)
+    xh = necg.pack_x(iv, C)
+    gh = np.zeros_like(xh)
+    xd = torch.as_tensor(xh, device="cuda:0")
+    assert lib.necg_device_visible(ct.c_void_p(xd.data_ptr())) == 1
+    assert lib.necg_device_visible(ct.c_void_p(xh.ctypes.data)) == 0
+    assert lib.necg_device_visible(ct.c_void_p(gh.ctypes.data)) == 0
+    eng = engine_for("He")
+    f = torch.empty(1, dtype=torch.float64, device="cuda:0")
+    rc = lib.necg_outprod(eng._ctx, 4, ct.c_void_p(xh.ctypes.data), ct.c_void_p(f.data_ptr()),
+                          ct.c_void_p(gh.ctypes.data), ct.c_void_p(0))
+    assert rc == _lib.NECG_ERR_ARG
+    E, g = eng.outprod(xh)          # the engine path stages host arrays on the device
+    assert np.isfinite(E)
