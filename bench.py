@@ -147,36 +147,38 @@ def outprod_leg(eng, xd, K, reps=5):
             "host_buffers_pairs_per_s": pairs / th, "energy": float(f.item())}
 
 
-def optimizer_leg(maxiter=20):
-    """LiH K = 12, seed 0: a 20-iteration truncated-Newton run (LMQN constants
-    through scipy TNC) driven by the GPU outprod.  The same run on the host is
-    part of the CPU baseline."""
+def small_lih_input():
+    """LiH K = 12 as the reference's input generator makes it from seed 0
+    (input_file_generation/generate_lih_input.py; restated byte for byte by
+    generated_lih_input), with the file's C = 1/K used as is."""
     import nbo_correlatedgaussians_amd as necg
+    _, s, iv, C = necg.generated_lih_input(12, 0)
+    return s, 12, necg.pack_x(iv, C)
+
+
+def optimizer_leg(maxiter=20):
+    """LiH K = 12 from the generator (seed 0): one outprod and a 20-iteration
+    truncated-Newton run (LMQN constants through scipy TNC) driven by the GPU
+    outprod.  The same run on the host is part of the CPU baseline."""
     from nbo_correlatedgaussians_amd.driver import optimize
     from nbo_correlatedgaussians_amd.engine import Engine
-    s = necg.lih_system()
-    K = 12
-    iv, C = necg.generate_parameters(s, K, 0)
-    x0 = necg.pack_x(iv, C)
+    s, K, x0 = small_lih_input()
     eng = Engine(s, "cuda:0")
     E0 = eng.outprod(x0)[0]
     t0 = time.perf_counter()
     r = optimize(eng.outprod, x0, method="TN", maxiter=maxiter, siv=K * s.blk, M=K)
-    return {"workload": f"LiH K={K} seed 0: one outprod and a {maxiter}-iteration truncated "
-                        f"Newton run (scipy TNC with the LMQN constants)",
+    return {"workload": f"LiH K={K} made by the input generator from seed 0: one outprod and "
+                        f"a {maxiter}-iteration truncated Newton run (scipy TNC with the LMQN "
+                        f"constants)",
             "maxiter": maxiter, "E0": E0, "seconds": time.perf_counter() - t0, "nit": r.nit,
             "nfev": r.nfev, "E_final": r.fun, "energies": r.energies}
 
 
 def cpu_optimizer(opt):
     """The optimizer leg's run with the oracle (one thread) as the objective."""
-    import nbo_correlatedgaussians_amd as necg
     from nbo_correlatedgaussians_amd.driver import optimize
     from oracle.oracle import Oracle
-    s = necg.lih_system()
-    K = 12
-    iv, C = necg.generate_parameters(s, K, 0)
-    x0 = necg.pack_x(iv, C)
+    s, K, x0 = small_lih_input()
     o = Oracle(s)
     t0 = time.perf_counter()
     r = optimize(lambda x: o.outprod(x, 1), x0, method="TN", maxiter=opt["maxiter"],

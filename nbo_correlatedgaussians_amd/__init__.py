@@ -5,9 +5,11 @@ NECG)."""
 from .system import (System, read_input, write_input, format_input,
                      parse_tokens, lih_system, he_system, h2_system,
                      be_system, c_system, hydrogen_atom, generate_parameters, pack_x, unpack_x,
-                     lih_oldquit_basis)
+                     lih_oldquit_basis, generate_like_template, format_generated_input,
+                     generated_lih_input)
 
 __all__ = ["System", "read_input", "write_input", "format_input",
            "parse_tokens", "lih_system", "he_system", "h2_system",
            "be_system", "c_system", "hydrogen_atom", "generate_parameters", "pack_x", "unpack_x",
-           "lih_oldquit_basis"]
+           "lih_oldquit_basis", "generate_like_template", "format_generated_input",
+           "generated_lih_input"]

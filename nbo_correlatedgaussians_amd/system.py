@@ -339,6 +339,113 @@ def generate_parameters(sysd: System, K: int, seed: int,
     return iv.reshape(-1), C
 
 
+def _vech_diagonal_slots(n: int):
+    """Positions of L_00, L_11, ... in the column-major vech order."""
+    out, k = [], 0
+    for j in range(n):
+        out.append(k)
+        k += n - j
+    return out
+
+
+def template_statistics(sysd: System, iv_template: np.ndarray):
+    """Per-column mean and standard deviation (population, ddof 0) of a
+    basis's parameter blocks, with standard deviations below 1e-10 raised to
+    1e-6 (input_file_generation/generate_lih_input.py:230-234)."""
+    blocks = np.asarray(iv_template, dtype=np.float64).reshape(-1, sysd.blk)
+    mean = blocks.mean(axis=0)
+    sd = blocks.std(axis=0)
+    return mean, np.where(sd < 1e-10, 1e-6, sd)
+
+
+def _lognormal_fit(mean: float, sd: float):
+    """(mu, sigma) of the log-normal variate with the given mean and standard
+    deviation (input_file_generation/generate_lih_input.py:139-154): a
+    non-positive mean becomes |mean| + 1e-3, a standard deviation below 1e-10
+    becomes 5 % of the mean."""
+    mean, sd = np.float64(mean), np.float64(sd)
+    if mean <= 0:
+        mean = abs(mean) + 1e-3
+    if sd < 1e-10:
+        sd = mean * 0.05
+    s2 = np.log1p((sd / mean) ** 2)        # numpy's log/log1p/sqrt, so the
+    return np.log(mean) - 0.5 * s2, np.sqrt(s2)   # draws match bit for bit
+
+
+def generate_like_template(sysd: System, K: int, seed, iv_template: np.ndarray = None):
+    """K functions drawn the way the reference's input generator draws them
+    (input_file_generation/generate_lih_input.py:157-189, :228-245): every
+    block is one normal draw of all blk columns with the template's column
+    means and standard deviations, after which each diagonal entry of L is
+    replaced, in column order, by a log-normal draw fitted to that column's
+    mean and standard deviation; C_k = 1/K.  The stream is
+    ``numpy.random.default_rng(seed)``.  The template defaults to the
+    12-function basis of ``oldquit.dat`` (LiH).  Returns (iv, C) at full
+    precision; the generator's file rounds them to 16 significant digits
+    (``format_generated_input``)."""
+    if K < 1:
+        raise ValueError(f"K must be >= 1, got {K}")
+    if iv_template is None:
+        iv_template, _ = lih_oldquit_basis()
+    mean, sd = template_statistics(sysd, iv_template)
+    diag = _vech_diagonal_slots(sysd.n)
+    fits = [_lognormal_fit(mean[i], sd[i]) for i in diag]
+    rng = np.random.default_rng(seed)
+    iv = np.empty((K, sysd.blk))
+    for k in range(K):
+        row = rng.normal(mean, sd)
+        for i, (mu, sg) in zip(diag, fits):
+            row[i] = rng.lognormal(mu, sg)
+        iv[k] = row
+    return iv.reshape(-1), np.full(K, 1.0 / K)
+
+
+def _gen_float(v: float) -> str:
+    return "0.0" if v == 0.0 else f"{v:25.15E}"
+
+
+def format_generated_input(sysd: System, iv: np.ndarray, C: np.ndarray) -> str:
+    """The input file in the generator's text layout, character for character
+    (input_file_generation/generate_lih_input.py:212-307): header lines,
+    mass rows, charges with one decimal, one parameter per line, C, raw
+    weights (integers where whole), then sml and sms rows with one decimal.
+    The reader (``read_input``) accepts it like any other input file."""
+    K, n = len(C), sysd.n
+    if len(iv) != K * sysd.blk:
+        raise ValueError("iv length does not match K*blk")
+    out = [str(K), str(n), str(sysd.nst), f"{sysd.efs:.10E}".replace("E", "d"), ""]
+    out += ["  ".join(_gen_float(v) for v in row) for row in sysd.mass]
+    out.append("")
+    out += [f"{v:.1f}" for v in sysd.ncv]
+    out += ["", ""]
+    for blkv in np.asarray(iv, dtype=np.float64).reshape(K, sysd.blk):
+        out += [f"   {_gen_float(v)}" for v in blkv]
+        out.append("")
+    out.append("")
+    out += [f"  {_gen_float(v)}" for v in C]
+    out.append("")
+    out += [str(int(v)) if v == int(v) else f"{v:.6g}" for v in sysd.scvv_raw]
+    out.append("")
+    for mats in (sysd.sml, sysd.sms):
+        for m in mats:
+            out += ["  ".join(f"{v:.1f}" for v in row) for row in m]
+            out.append("")
+    return "\n".join(out) + "\n"
+
+
+def generated_lih_input(K: int, seed):
+    """Config-0-style LiH input: the oldquit.dat physics block with K
+    functions drawn by ``generate_like_template`` and written by
+    ``format_generated_input``.  Returns (text, System, iv, C) with iv and C
+    as read back from the text (what a run on that file sees)."""
+    sysd = lih_system()
+    iv, C = generate_like_template(sysd, K, seed)
+    text = format_generated_input(sysd, iv, C)
+    s2, K2, iv2, C2 = parse_tokens(text.split())
+    s2.name = "LiH"
+    return text, s2, iv2, C2
+
+
 def pack_x(iv: np.ndarray, C: np.ndarray) -> np.ndarray:
     return np.concatenate([np.asarray(iv, dtype=np.float64),
                            np.asarray(C, dtype=np.float64)])

@@ -320,3 +320,16 @@ def test_host_pointers_are_refused():
     assert rc == _lib.NECG_ERR_ARG
     E, g = eng.outprod(xh)          # the engine path stages host arrays on the device
     assert np.isfinite(E)
+
+
+def test_outprod_on_generator_input_vs_reference_golden():
+    """LiH K = 12 made by the reference's input generator from seed 0, read
+    back from the generator's file: outprod against the reference's outprod
+    on the same file."""
+    gd = golden("generator")
+    _, s, iv, C = necg.generated_lih_input(12, 0)
+    from nbo_correlatedgaussians_amd.engine import Engine
+    print("case generator m12_s0", flush=True)
+    E, g = Engine(s, "cuda:0").outprod(necg.pack_x(iv, C))
+    assert close_E(E, float(gd["m12_s0_E"])), (E, float(gd["m12_s0_E"]))
+    assert close_g(g, gd["m12_s0_g"])

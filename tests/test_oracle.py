@@ -299,3 +299,16 @@ def test_oracle_rotation_invariance(O, name, axis):
     if s.efs != 0.0:     # a rotation about another axis does change E
         E3, _ = o.outprod(_rotate_shifts(s, x, K, _rot("y", 0.7)))
         assert abs(E3 - E) > 1e-9 * abs(s.efs)
+
+
+def test_oracle_on_generator_input_vs_reference_golden(O):
+    """LiH K = 12 made by the reference's generator from seed 0 (the file of
+    input_file_generation/generate_lih_input.py): one outprod against the
+    reference's outprod on that file."""
+    gd = golden("generator")
+    _, s, iv, C = necg.generated_lih_input(12, 0)
+    x = necg.pack_x(iv, C)
+    E, g = O(s).outprod(x)
+    Eref = float(gd["m12_s0_E"])
+    assert abs(E - Eref) <= RTOL_E * max(abs(Eref), 1.0)
+    assert np.max(np.abs(g - gd["m12_s0_g"])) <= RTOL_G * max(np.max(np.abs(gd["m12_s0_g"])), 1.0)

@@ -82,3 +82,38 @@ def test_unpack_x():
     assert K == 5
     with pytest.raises(ValueError):
         necg.unpack_x(s, np.zeros(7))
+
+
+def test_generator_matches_reference_generator_bytes():
+    """The generator restatement writes the reference generator's file
+    character for character (SHA-256 of the text from
+    input_file_generation/generate_lih_input.py, tests/golden/
+    make_generator_golden.py), and reading it back gives the (iv, C) the
+    reference's read_input gave."""
+    import hashlib
+    gd = np.load(os.path.join(os.path.dirname(__file__), "golden", "golden_generator.npz"))
+    for M, seed in gd["cases"]:
+        tag = f"m{M}_s{seed}"
+        text, sysd, iv, C = necg.generated_lih_input(int(M), int(seed))
+        assert len(text) == int(gd[f"{tag}_chars"]), tag
+        assert hashlib.sha256(text.encode()).hexdigest() == str(gd[f"{tag}_sha256"]), tag
+        if f"{tag}_iv" in gd:
+            np.testing.assert_array_equal(iv, gd[f"{tag}_iv"])
+            np.testing.assert_array_equal(C, gd[f"{tag}_C"])
+        assert sysd.n == 5 and sysd.nst == 24
+        assert len(C) == M and np.all(C == 1.0 / M)     # C_k = 1/M
+
+
+def test_generator_template_statistics():
+    """Diagonals positive, other columns centred on the template's column
+    means (a plausible mix-up of columns or of mean and spread fails)."""
+    s = necg.lih_system()
+    tiv, _ = necg.lih_oldquit_basis()
+    mean = tiv.reshape(-1, s.blk).mean(axis=0)
+    sd = tiv.reshape(-1, s.blk).std(axis=0)
+    iv, C = necg.generate_like_template(s, 20000, 5)
+    b = iv.reshape(-1, s.blk)
+    diag_idx = [0, 5, 9, 12, 14]
+    assert np.all(b[:, diag_idx] > 0)
+    assert np.allclose(b.mean(axis=0), mean, atol=4 * sd.max() / np.sqrt(20000) + 1e-12)
+    assert np.allclose(b.std(axis=0), np.where(sd < 1e-10, 1e-6, sd), rtol=0.05)
