@@ -119,6 +119,21 @@ int necg_dipole(necg_ctx *ctx, int32_t K, const double *x, double *out, void *st
  * No counterpart in the reference, which always forms the double sum. */
 int necg_set_dipole_mode(necg_ctx *ctx, int32_t mode);
 
+/* Form of the gradient pair pass of necg_outprod / necg_energy_fwd.  The
+ * reference evaluates every ordered pair (l, k) and takes bra derivatives
+ * (python/make_grad.py:34-55); when every term is a particle permutation
+ * (sms = kron(P^T, I3)) that leaves masses and charges unchanged and P^T is a
+ * term of the same weight, the two-sided form evaluates each unordered pair
+ * once per term and takes both the bra and the ket derivative (n <= 5).
+ * mode 0 (default) uses it whenever it applies; 1 forces it
+ * (NECG_ERR_UNSUPPORTED here if it does not apply); 2 forces the per-ordering
+ * form.  No counterpart in the reference. */
+int necg_set_pair_mode(necg_ctx *ctx, int32_t mode);
+
+/* *available = 1 if the two-sided form applies to the context's system;
+ * *last = 1 if the last gradient pair pass used it.  Host pointers. */
+int necg_pair_form(necg_ctx *ctx, int32_t *available, int32_t *last);
+
 int necg_kinetic_potential(necg_ctx *ctx, int32_t K, const double *x, double *T, double *V,
                            void *stream);
 

@@ -25,6 +25,7 @@
 
 #include "../../include/necg.h"
 #include "necg_ctx.hpp"
+#include "necg_pair2.hpp"
 
 using namespace necg;
 
@@ -35,13 +36,13 @@ namespace {
 
 // One lane per basis function k: the bra record.
 template <int N>
-__global__ void prep_bra(const double *__restrict__ x, int K, int with_c,
+__global__ void prep_bra(const double *__restrict__ x, int K, int with_c, SysConst sc,
                          double *__restrict__ bra, int *__restrict__ flag) {
   using D = Dim<N>;
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= K) return;
   make_bra<N>(x + (size_t)k * D::BLK, with_c ? x[(size_t)K * D::BLK + k] : 0.0, with_c != 0,
-              bra + (size_t)k * D::BS, flag);
+              sc, bra + (size_t)k * D::BS, flag);
 }
 
 // One lane per (ket l, term p): the transformed ket record.
@@ -86,6 +87,71 @@ __global__ void __launch_bounds__(64 * pair_waves(N), NECG_MIN_WAVES) pair_kerne
                      min(nst, p0 + tps), sc, (MODE == MODE_COMBINED) ? scal[0] : 0.0,
                      part + tile * D::NACC * K, hpart + (size_t)blockIdx.z * K * K,
                      spart + (size_t)blockIdx.z * K * K, flag, acc_s);
+}
+
+// Two-sided pair pass over the lower triangle.  blockIdx.x indexes the tile
+// list (tiles[2i], tiles[2i+1]) = (bra tile, ket tile) with ket tile start
+// <= bra tile end; blockIdx.y = term group z of gridDim.y.
+// bpart layout: [(ket tile * ts + z) * NACC + v] * K + k;
+// kpart layout: [((bra tile * ts + z) * K + l) * NACC + v].
+template <int N>
+__global__ void __launch_bounds__(64 * pair_waves(N), NECG_MIN_WAVES) pair2_kernel(
+    const double *__restrict__ bra, const double *__restrict__ ket, int K, int nst, int kt,
+    int tps, SysConst sc, const int *__restrict__ tiles, const unsigned *__restrict__ tperm,
+    double *__restrict__ bpart, double *__restrict__ kpart, double *__restrict__ hpart,
+    double *__restrict__ spart, int *__restrict__ flag) {
+  using D = Dim<N>;
+  static_assert(Two<N>::NCH <= 4, "two-sided form: at most 64 accumulator slots");
+  __shared__ double acc_s[PairLds<N, MODE_SPLIT>::ACC];
+  __shared__ double kacc_s[pair_waves(N) * 64];
+  __shared__ double bra_s[BraLds<N>::NF * 64];
+  const int bx = tiles[2 * blockIdx.x], by = tiles[2 * blockIdx.x + 1];
+  const int k0 = bx * 64, l0 = by * kt;
+  const int l1 = min(min(K, l0 + kt), k0 + 64);
+  const int z = blockIdx.y, ts = gridDim.y;
+  const int p0 = z * tps;
+  pair2_tile<N>(bra, ket, K, nst, k0, l0, l1, p0, min(nst, p0 + tps), sc, tperm,
+                bpart + (size_t)(by * ts + z) * D::NACC * K,
+                kpart + (size_t)(bx * ts + z) * K * D::NACC, hpart + (size_t)z * K * K,
+                spart + (size_t)z * K * K, flag, acc_s, kacc_s, bra_s);
+}
+
+// Element-only pass (MODE_ELEMENTS / MODE_TV) over the same lower-triangle
+// tile list: pairs with bra k >= ket l only.
+template <int N, int MODE>
+__global__ void __launch_bounds__(64 * pair_waves(N), NECG_MIN_WAVES) pair_tri_kernel(
+    const double *__restrict__ bra, const double *__restrict__ ket, int K, int nst, int kt,
+    int tps, SysConst sc, const int *__restrict__ tiles, double *__restrict__ hpart,
+    double *__restrict__ spart, int *__restrict__ flag) {
+  static_assert(!mode_has_grad(MODE), "element modes only");
+  __shared__ double acc_s[PairLds<N, MODE>::ACC];
+  const int bx = tiles[2 * blockIdx.x], by = tiles[2 * blockIdx.x + 1];
+  const int k0 = bx * 64, l0 = by * kt;
+  const int l1 = min(min(K, l0 + kt), k0 + 64);
+  const int z = blockIdx.y;
+  const int p0 = z * tps;
+  pair_tile<N, MODE>(bra, ket, K, nst, k0, l0, l1, p0, min(nst, p0 + tps), sc, 0.0, nullptr,
+                     hpart + (size_t)z * K * K, spart + (size_t)z * K * K, flag, acc_s);
+}
+
+// Sum of the two-sided partials of function k, slot v: its bra partials over
+// the ket tiles of its bra tile's row (ket tile start <= bra tile end) and
+// its ket partials over the bra tiles from its own onwards.  Fixed order.
+// Grid (ceil(K/256), nv), 256 lanes.
+__global__ void __launch_bounds__(256) tile_reduce2(const double *__restrict__ bpart,
+                                                    const double *__restrict__ kpart, int K,
+                                                    int kt, int ts, int nacc,
+                                                    double *__restrict__ red) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int v = blockIdx.y;
+  if (k >= K) return;
+  const int bx = k >> 6;
+  const int nkt = (K + kt - 1) / kt, nbt = (K + 63) / 64;
+  const int ny = min(nkt, (64 * bx + 63) / kt + 1);
+  double s = 0.0;
+  for (int t = 0; t < ny * ts; ++t) s += bpart[((size_t)t * nacc + v) * K + k];
+  for (int t = bx * ts; t < nbt * ts; ++t) s += kpart[((size_t)t * K + k) * nacc + v];
+  red[(size_t)v * K + k] = s;
 }
 
 // Full symmetric H and S and the per-chunk partials of H C and S C.
@@ -342,7 +408,7 @@ template <int N>
 int run_prep(necg_ctx *c, int K, const double *x, int with_c, hipStream_t st) {
   const int tb = 256;
   hipLaunchKernelGGL((prep_bra<N>), dim3((K + tb - 1) / tb), dim3(tb), 0, st, x, K, with_c,
-                     c->bra, c->d_flag);
+                     c->sc, c->bra, c->d_flag);
   const int nk = K * c->nst;
   hipLaunchKernelGGL((prep_ket<N>), dim3((nk + tb - 1) / tb), dim3(tb), 0, st, c->bra, K,
                      c->nst, c->d_sml, c->d_sms, c->d_scvv, c->sc, c->ket);
@@ -420,36 +486,227 @@ int finish_grad(necg_ctx *c, int K, int ntiles, const double *gbar, double *g, h
   return hip_status(hipGetLastError());
 }
 
+// ----------------------------------------------------- lower triangle ----
+
+// Lower-triangle tiles (bra tile bx of 64, ket tile by of kt) with
+// by * kt <= 64 bx + 63: per bra tile, the ket tiles up to its last bra.
+long long tri_count(int K, int kt) {
+  const int nbt = (K + 63) / 64, nkt = (K + kt - 1) / kt;
+  long long n = 0;
+  for (int bx = 0; bx < nbt; ++bx) n += std::min(nkt, (64 * bx + 63) / kt + 1);
+  return n;
+}
+
+// Ket tile and term groups of the lower-triangle passes: the ket tile that
+// fills whole rounds of one workgroup per CU best (time ~ rounds x kt), then
+// term groups while there are fewer than two workgroups per CU.
+void choose_tiling2(necg_ctx *c, int K, int &kt, int &ts) {
+  const int ncu = std::max(1, c->ncu);
+  if (c->req_kt > 0) {
+    kt = c->req_kt;
+  } else {
+    static const int cand[] = {4, 6, 8, 10, 12, 14, 16, 20, 24, 32, 48, 64};
+    double best = 0.0;
+    kt = 16;
+    for (int q : cand) {
+      const long long n = tri_count(K, q);
+      const double cost = (double)((n + ncu - 1) / ncu) * (q + 1.0);
+      if (best == 0.0 || cost < best) {
+        best = cost;
+        kt = q;
+      }
+    }
+  }
+  const long long n = tri_count(K, kt);
+  if (c->req_ts > 0) {
+    ts = std::min(c->req_ts, c->nst);
+  } else {
+    ts = 1;
+    while (n * ts < 2LL * ncu && ts < c->nst) ts *= 2;
+    ts = std::min(ts, c->nst);
+  }
+}
+
+// The tile list for (K, kt) on the device: full tiles first, then those cut
+// short by the triangle or by K.
+int ensure_tiles(necg_ctx *c, int K, int kt) {
+  if (c->tiles_K == K && c->tiles_kt == kt) return NECG_OK;
+  const int nbt = (K + 63) / 64, nkt = (K + kt - 1) / kt;
+  std::vector<int> full, part;
+  for (int bx = 0; bx < nbt; ++bx) {
+    const int ny = std::min(nkt, (64 * bx + 63) / kt + 1);
+    for (int by = 0; by < ny; ++by) {
+      const int l0 = by * kt, l1 = std::min(std::min(K, l0 + kt), 64 * bx + 64);
+      auto &dst = (l1 - l0 == kt && 64 * bx + 64 <= K) ? full : part;
+      dst.push_back(bx);
+      dst.push_back(by);
+    }
+  }
+  full.insert(full.end(), part.begin(), part.end());
+  if (full.size() > c->tiles_cap) {
+    if (c->d_tiles) (void)hipFree(c->d_tiles);
+    c->d_tiles = nullptr;
+    c->tiles_cap = 0;
+    HIP_TRY(hipMalloc((void **)&c->d_tiles, full.size() * sizeof(int)));
+    c->tiles_cap = full.size();
+  }
+  HIP_TRY(hipMemcpy(c->d_tiles, full.data(), full.size() * sizeof(int), hipMemcpyHostToDevice));
+  c->tiles_K = K;
+  c->tiles_kt = kt;
+  c->ntri = (int)(full.size() / 2);
+  return NECG_OK;
+}
+
+int ensure_kpart(necg_ctx *c, size_t count) {
+  if (count <= c->kpart_cap) return NECG_OK;
+  int rc = dev_alloc(&c->kpart, count);
+  if (rc) return rc;
+  c->kpart_cap = count;
+  return NECG_OK;
+}
+
+// Timing events around one pair launch (necg_timing).
+int timing_begin(necg_ctx *c, hipStream_t st, hipEvent_t &e1) {
+  e1 = nullptr;
+  if (!c->timing) return NECG_OK;
+  if ((size_t)(2 * c->nev + 2) > c->ev.size()) {
+    hipEvent_t a, b;
+    HIP_TRY(hipEventCreate(&a));
+    HIP_TRY(hipEventCreate(&b));
+    c->ev.push_back(a);
+    c->ev.push_back(b);
+  }
+  hipEvent_t e0 = c->ev[2 * c->nev];
+  e1 = c->ev[2 * c->nev + 1];
+  c->nev++;
+  HIP_TRY(hipEventRecord(e0, st));
+  return NECG_OK;
+}
+
+// Two-sided gradient pair pass (MODE_SPLIT partials) over the lower triangle.
+template <int N>
+int run_pairs2(necg_ctx *c, int K, hipStream_t st, int &kt, int &ts) {
+  using D = Dim<N>;
+  choose_tiling2(c, K, kt, ts);
+  const int tps = (c->nst + ts - 1) / ts;
+  ts = (c->nst + tps - 1) / tps;
+  const int nkt = (K + kt - 1) / kt, nbt = (K + 63) / 64;
+  int rc;
+  if ((rc = ensure_tiles(c, K, kt))) return rc;
+  if ((rc = ensure_part(c, (size_t)nkt * ts * D::NACC * K))) return rc;
+  if ((rc = ensure_kpart(c, (size_t)nbt * ts * K * D::NACC))) return rc;
+  if ((rc = ensure_hpart(c, (size_t)ts * K * K))) return rc;
+  hipEvent_t e1;
+  if ((rc = timing_begin(c, st, e1))) return rc;
+  hipLaunchKernelGGL((pair2_kernel<N>), dim3(c->ntri, ts), dim3(64 * pair_waves(N)), 0, st,
+                     c->bra, c->ket, K, c->nst, kt, tps, c->sc, c->d_tiles, c->d_tperm, c->part,
+                     c->kpart, c->hpart, c->spart, c->d_flag);
+  if (c->timing) HIP_TRY(hipEventRecord(e1, st));
+  c->last_kt = kt;
+  c->last_ts = ts;
+  c->last_w = pair_waves(N);
+  return hip_status(hipGetLastError());
+}
+
+// Element-only pass (MODE_ELEMENTS or MODE_TV) over the lower triangle.
+template <int N, int MODE>
+int run_tri(necg_ctx *c, int K, hipStream_t st, int &ts) {
+  int kt;
+  choose_tiling2(c, K, kt, ts);
+  const int tps = (c->nst + ts - 1) / ts;
+  ts = (c->nst + tps - 1) / tps;
+  int rc;
+  if ((rc = ensure_tiles(c, K, kt))) return rc;
+  if ((rc = ensure_hpart(c, (size_t)ts * K * K))) return rc;
+  hipEvent_t e1;
+  if ((rc = timing_begin(c, st, e1))) return rc;
+  hipLaunchKernelGGL((pair_tri_kernel<N, MODE>), dim3(c->ntri, ts), dim3(64 * pair_waves(N)), 0,
+                     st, c->bra, c->ket, K, c->nst, kt, tps, c->sc, c->d_tiles, c->hpart,
+                     c->spart, c->d_flag);
+  if (c->timing) HIP_TRY(hipEventRecord(e1, st));
+  c->last_kt = kt;
+  c->last_ts = ts;
+  c->last_w = pair_waves(N);
+  return hip_status(hipGetLastError());
+}
+
+template <int N>
+int finish_grad2(necg_ctx *c, int K, int kt, int ts, const double *gbar, double *g,
+                 hipStream_t st) {
+  using D = Dim<N>;
+  hipLaunchKernelGGL(tile_reduce2, dim3((K + 255) / 256, D::NACC), dim3(256), 0, st, c->part,
+                     c->kpart, K, kt, ts, D::NACC, c->red);
+  hipLaunchKernelGGL((grad_final<N, MODE_SPLIT>), dim3((K + 255) / 256), dim3(256), 0, st, c->red,
+                     K, c->bra, c->scal, c->HC, c->SC, gbar, g);
+  return hip_status(hipGetLastError());
+}
+
+// Whether the gradient pair pass takes the two-sided form.
+bool use_two(const necg_ctx *c) {
+  return c->two_ok && c->pair_mode != 2 && c->n <= 5;
+}
+
+// Gradient pair pass of x (bra/ket tables ready): H/S partials and the
+// gradient partials in either form; records the form for the finish.
+template <int N>
+int grad_pass(necg_ctx *c, int K, hipStream_t st, int &ntiles, int &ts, int &kt) {
+  int rc;
+  if constexpr (N <= 5) {
+    if (use_two(c)) {
+      if ((rc = run_pairs2<N>(c, K, st, kt, ts))) return rc;
+      ntiles = 0;
+      c->last_two = 1;
+      return NECG_OK;
+    }
+  }
+  if ((rc = run_pairs<N, MODE_SPLIT>(c, K, st, ntiles, ts))) return rc;
+  kt = c->last_kt;
+  c->last_two = 0;
+  return NECG_OK;
+}
+
+template <int N>
+int grad_finish(necg_ctx *c, int K, bool two, int ntiles, int kt, int ts, const double *gbar,
+                double *g, hipStream_t st) {
+  if constexpr (N <= 5) {
+    if (two) return finish_grad2<N>(c, K, kt, ts, gbar, g, st);
+  }
+  return finish_grad<N, MODE_SPLIT>(c, K, ntiles, gbar, g, st);
+}
+
 template <int N>
 int do_outprod(necg_ctx *c, int K, const double *x, double *f, double *g, hipStream_t st) {
-  int rc, ntiles, ts;
+  int rc, ntiles, ts, kt;
   if ((rc = run_prep<N>(c, K, x, 1, st))) return rc;
-  if ((rc = run_pairs<N, MODE_SPLIT>(c, K, st, ntiles, ts))) return rc;
+  if ((rc = grad_pass<N>(c, K, st, ntiles, ts, kt))) return rc;
   if ((rc = finish_hs(c, K, ts, st))) return rc;
   if ((rc = finish_quotient(c, K, f, st))) return rc;
-  return finish_grad<N, MODE_SPLIT>(c, K, ntiles, nullptr, g, st);
+  return grad_finish<N>(c, K, c->last_two != 0, ntiles, kt, ts, nullptr, g, st);
 }
 
 // Energy with the gradient's pair partials kept for a later backward
-// (necg_energy_bwd): the split pair kernel, H/S and the quotient.
+// (necg_energy_bwd): the gradient pair pass, H/S and the quotient.
 template <int N>
 int do_energy_fwd(necg_ctx *c, int K, const double *x, double *f, hipStream_t st) {
-  int rc, ntiles, ts;
+  int rc, ntiles, ts, kt;
   if ((rc = run_prep<N>(c, K, x, 1, st))) return rc;
-  if ((rc = run_pairs<N, MODE_SPLIT>(c, K, st, ntiles, ts))) return rc;
+  if ((rc = grad_pass<N>(c, K, st, ntiles, ts, kt))) return rc;
   if ((rc = finish_hs(c, K, ts, st))) return rc;
   if ((rc = finish_quotient(c, K, f, st))) return rc;
   c->tape_K = K;
   c->tape_ntiles = ntiles;
+  c->tape_two = c->last_two;
+  c->tape_kt = kt;
+  c->tape_ts = ts;
   return NECG_OK;
 }
 
 template <int N>
 int do_energy(necg_ctx *c, int K, const double *x, double *f, double *H, double *S,
               hipStream_t st) {
-  int rc, ntiles, ts;
+  int rc, ts;
   if ((rc = run_prep<N>(c, K, x, 1, st))) return rc;
-  if ((rc = run_pairs<N, MODE_ELEMENTS>(c, K, st, ntiles, ts))) return rc;
+  if ((rc = run_tri<N, MODE_ELEMENTS>(c, K, st, ts))) return rc;
   if ((rc = finish_hs(c, K, ts, st))) return rc;
   if ((rc = finish_quotient(c, K, f, st))) return rc;
   if (H) HIP_TRY(hipMemcpyAsync(H, c->H, sizeof(double) * K * K, hipMemcpyDeviceToDevice, st));
@@ -469,9 +726,9 @@ int do_vjp(necg_ctx *c, int K, const double *x, const double *gbar, double *gx,
 
 template <int N>
 int do_matrices(necg_ctx *c, int K, const double *iv, double *H, double *S, hipStream_t st) {
-  int rc, ntiles, ts;
+  int rc, ts;
   if ((rc = run_prep<N>(c, K, iv, 0, st))) return rc;
-  if ((rc = run_pairs<N, MODE_ELEMENTS>(c, K, st, ntiles, ts))) return rc;
+  if ((rc = run_tri<N, MODE_ELEMENTS>(c, K, st, ts))) return rc;
   if ((rc = finish_hs(c, K, ts, st))) return rc;
   if (H) HIP_TRY(hipMemcpyAsync(H, c->H, sizeof(double) * K * K, hipMemcpyDeviceToDevice, st));
   if (S) HIP_TRY(hipMemcpyAsync(S, c->S, sizeof(double) * K * K, hipMemcpyDeviceToDevice, st));
@@ -480,9 +737,9 @@ int do_matrices(necg_ctx *c, int K, const double *iv, double *H, double *S, hipS
 
 template <int N>
 int do_tv(necg_ctx *c, int K, const double *x, double *T, double *V, hipStream_t st) {
-  int rc, ntiles, ts;
+  int rc, ts;
   if ((rc = run_prep<N>(c, K, x, 1, st))) return rc;
-  if ((rc = run_pairs<N, MODE_TV>(c, K, st, ntiles, ts))) return rc;
+  if ((rc = run_tri<N, MODE_TV>(c, K, st, ts))) return rc;
   return finish_hs(c, K, ts, st, T, V);
 }
 
@@ -516,6 +773,98 @@ int do_dhds(necg_ctx *c, int K, const double *iv, double *DH, double *DS, hipStr
     case 6: { constexpr int NN = 6; return call; } \
     default: return NECG_ERR_UNSUPPORTED;   \
   }
+
+// The two-sided pair form applies when every term is a particle
+// permutation P (sml[p] a permutation matrix, sms[p] = kron(P^T, I3)) that
+// leaves masses and charges unchanged, and P^T is a term of the same weight.
+// Fills the per-(term, lane) words of target slots: for chunk c, byte c of
+// lane j's word is the slot of the ket's own accumulator record that receives
+// value 16 c + (j >> 2) of the two-sided term's ket-side values (X_H, X_S
+// interleaved per packed (a, b), Y_H, Y_S per (a, x), sigma_H, sigma_S), with
+// X' -> P X' P^T and Y' -> P Y' applied; 255 where there is no value.
+bool two_sided_table(const necg_system *sys, std::vector<unsigned> &tab) {
+  const int n = sys->n, nst = sys->nst, n3 = 3 * n;
+  if (n > 5) return false;
+  auto P = [&](int p, int i, int j) { return sys->sml[((size_t)p * n + i) * n + j]; };
+  std::vector<int> pinv((size_t)nst * n);
+  for (int p = 0; p < nst; ++p) {
+    for (int i = 0; i < n; ++i) {
+      int hits = 0, col = -1;
+      for (int j = 0; j < n; ++j) {
+        const double v = P(p, i, j);
+        if (v != 0.0 && v != 1.0) return false;
+        if (v == 1.0) {
+          ++hits;
+          col = j;
+        }
+      }
+      if (hits != 1) return false;
+      pinv[(size_t)p * n + col] = i;  // P[i][pi(i)] = 1, pinv = pi^-1
+    }
+    for (int j = 0; j < n; ++j) {
+      int hits = 0;
+      for (int i = 0; i < n; ++i) hits += P(p, i, j) == 1.0;
+      if (hits != 1) return false;
+    }
+    for (int r = 0; r < n3; ++r)
+      for (int q = 0; q < n3; ++q) {
+        const double want = (r % 3 == q % 3) ? P(p, q / 3, r / 3) : 0.0;
+        if (sys->sms[((size_t)p * n3 + r) * n3 + q] != want) return false;
+      }
+    // masses and charges unchanged: M[pi(i)][pi(j)] = M[i][j], q[pi(i)] = q[i]
+    for (int i = 0; i < n; ++i) {
+      int pii = -1;
+      for (int j = 0; j < n; ++j)
+        if (P(p, i, j) == 1.0) pii = j;
+      if (sys->ncv[pii + 1] != sys->ncv[i + 1]) return false;
+      for (int j = 0; j < n; ++j) {
+        int pij = -1;
+        for (int m = 0; m < n; ++m)
+          if (P(p, j, m) == 1.0) pij = m;
+        if (sys->mass[pii * n + pij] != sys->mass[i * n + j]) return false;
+      }
+    }
+    // P^T is a term with the same weight
+    bool found = false;
+    for (int q = 0; q < nst && !found; ++q) {
+      bool same = sys->scvv[q] == sys->scvv[p];
+      for (int i = 0; i < n && same; ++i)
+        for (int j = 0; j < n && same; ++j) same = P(q, i, j) == P(p, j, i);
+      found = same;
+    }
+    if (!found) return false;
+  }
+  const int ns = n * (n + 1) / 2;
+  const int a_xh = 0, a_yh = ns, a_xs = ns + n3, a_ys = 2 * ns + n3, a_sh = 2 * ns + 2 * n3;
+  const int nv = a_sh + 2;
+  if (nv > 64) return false;
+  tab.assign((size_t)nst * 64, 0u);
+  for (int p = 0; p < nst; ++p) {
+    const int *iv = &pinv[(size_t)p * n];
+    for (int lane = 0; lane < 64; ++lane) {
+      unsigned word = 0;
+      for (int c = 0; c < 4; ++c) {
+        const int s = 16 * c + (lane >> 2);
+        int t = 255;
+        if (s < 2 * ns) {
+          const int pk = s / 2;
+          int a = 0;
+          while ((a + 1) * (a + 2) / 2 <= pk) ++a;
+          const int b = pk - a * (a + 1) / 2;
+          t = ((s & 1) ? a_xs : a_xh) + sx(iv[a], iv[b]);
+        } else if (s < 2 * ns + 2 * n3) {
+          const int u = (s - 2 * ns) / 2;
+          t = (((s - 2 * ns) & 1) ? a_ys : a_yh) + 3 * iv[u / 3] + u % 3;
+        } else if (s < nv) {
+          t = s - 2 * ns - 2 * n3 + a_sh;
+        }
+        word |= (unsigned)t << (8 * c);
+      }
+      tab[(size_t)p * 64 + lane] = word;
+    }
+  }
+  return true;
+}
 
 struct DeviceGuard {
   int prev = -1;
@@ -596,6 +945,20 @@ int necg_create(const necg_system *sys, int device, necg_ctx **out) {
     necg_destroy(c);
     return rc;
   }
+  int ncu = 0;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess &&
+      ncu > 0)
+    c->ncu = ncu;
+  std::vector<unsigned> tab;
+  if (two_sided_table(sys, tab)) {
+    if ((rc = hip_status(hipMalloc((void **)&c->d_tperm, tab.size() * sizeof(unsigned)))) ||
+        (rc = hip_status(hipMemcpy(c->d_tperm, tab.data(), tab.size() * sizeof(unsigned),
+                                   hipMemcpyHostToDevice)))) {
+      necg_destroy(c);
+      return rc;
+    }
+    c->two_ok = 1;
+  }
   *out = c;
   return NECG_OK;
 }
@@ -609,6 +972,9 @@ int necg_destroy(necg_ctx *c) {
                       c->prop_rec, c->prop_part, c->prop_tab};
     for (double *p : bufs)
       if (p) (void)hipFree(p);
+    if (c->kpart) (void)hipFree(c->kpart);
+    if (c->d_tiles) (void)hipFree(c->d_tiles);
+    if (c->d_tperm) (void)hipFree(c->d_tperm);
     if (c->d_flag) (void)hipFree(c->d_flag);
     for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
   }
@@ -664,8 +1030,8 @@ int necg_energy_bwd(necg_ctx *c, int32_t K, int64_t tape, const double *gbar, do
   std::lock_guard<std::mutex> lk(c->mu);
   if (c->tape_K != K || tape == 0 || tape != c->tape_id) return NECG_ERR_STALE;
   DeviceGuard dg(c->device);
-  DISPATCH_N(c->n, (finish_grad<NN, MODE_SPLIT>(c, K, c->tape_ntiles, gbar, gx,
-                                                (hipStream_t)stream)));
+  DISPATCH_N(c->n, (grad_finish<NN>(c, K, c->tape_two != 0, c->tape_ntiles, c->tape_kt,
+                                    c->tape_ts, gbar, gx, (hipStream_t)stream)));
 }
 
 int necg_energy_vjp(necg_ctx *c, int32_t K, const double *x, const double *gbar, double *gx,
@@ -788,6 +1154,23 @@ int necg_timing(necg_ctx *c, int32_t mode, double *ms, int64_t *launches) {
   }
   if (ms) *ms = tot;
   if (launches) *launches = c->nev;
+  return NECG_OK;
+}
+
+int necg_set_pair_mode(necg_ctx *c, int32_t mode) {
+  if (!c || mode < 0 || mode > 2) return NECG_ERR_ARG;
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (mode == 1 && !c->two_ok) return NECG_ERR_UNSUPPORTED;
+  c->pair_mode = mode;
+  c->tape_K = 0;
+  return NECG_OK;
+}
+
+int necg_pair_form(necg_ctx *c, int32_t *available, int32_t *last) {
+  if (!c || !available || !last) return NECG_ERR_ARG;
+  std::lock_guard<std::mutex> lk(c->mu);
+  *available = c->two_ok;
+  *last = c->last_two;
   return NECG_OK;
 }
 

@@ -79,7 +79,7 @@ __global__ void b_prep_bra(const BatchSys *__restrict__ sys, int nsys,
   const int k = (int)(idx - fpre[s]);
   const double *xs = x + b.x_off;
   SWITCH_N(b.n, (make_bra<NN>(xs + (size_t)k * Dim<NN>::BLK, xs[(size_t)b.K * Dim<NN>::BLK + k],
-                              true, bra + b.bra_off + (size_t)k * Dim<NN>::BS, flag)));
+                              true, b.sc, bra + b.bra_off + (size_t)k * Dim<NN>::BS, flag)));
 }
 
 // one lane per (system, ket, term)

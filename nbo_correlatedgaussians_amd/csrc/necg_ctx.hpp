@@ -37,9 +37,24 @@ struct necg_ctx {
   int req_kt = 0, req_ts = 0, req_w = 0;
   int last_kt = 0, last_ts = 0, last_w = 0;
   int last_K = 0;
+  int ncu = 256;  // compute units of the device (launch sizing)
+  // two-sided pair form: requested mode (0 auto, 1 two-sided, 2 one-sided),
+  // whether the terms allow it, the per-(term, lane) target-slot words, and
+  // whether the last gradient pair pass used it
+  int pair_mode = 0, two_ok = 0, last_two = 0;
+  unsigned *d_tperm = nullptr;
+  // lower-triangle tile list (bra tile, ket tile) for K = tiles_K, kt = tiles_kt
+  int *d_tiles = nullptr;
+  size_t tiles_cap = 0;
+  int tiles_K = 0, tiles_kt = 0, ntri = 0;
+  // per-(bra tile, term group) ket-side accumulator records, [t][l][slot]
+  double *kpart = nullptr;
+  size_t kpart_cap = 0;
   // forward tape of necg_energy_fwd: the pair partials of x at K = tape_K
-  // (0: none; any other evaluation on the context clears it), named tape_id
+  // (0: none; any other evaluation on the context clears it), named tape_id;
+  // tape_two: made by the two-sided pass with ket tile tape_kt, tape_ts groups
   int tape_K = 0, tape_ntiles = 0;
+  int tape_two = 0, tape_kt = 0, tape_ts = 0;
   long long tape_id = 0, tape_seq = 0;
   // pair-kernel timing (necg_timing)
   bool timing = false;

@@ -9,10 +9,12 @@
 namespace necg {
 
 // Bra record of one basis function from its parameter block p (vech L | s)
-// and coefficient c (python/matel.py:56-70, :125-136).
+// and coefficient c (python/matel.py:56-70, :125-136), with A M A and
+// tr(M A) for the two-sided pair term.
 template <int N>
 __device__ __forceinline__ void make_bra(const double *__restrict__ p, double c, bool with_c,
-                                         double *__restrict__ r, int *__restrict__ flag) {
+                                         const SysConst &sc, double *__restrict__ r,
+                                         int *__restrict__ flag) {
   using D = Dim<N>;
   double L[N][N];
 #pragma unroll
@@ -67,6 +69,29 @@ __device__ __forceinline__ void make_bra(const double *__restrict__ p, double c,
       i1 = __builtin_fma(S[3 * i + x], s, i1);
     }
   r[D::B_I1] = i1;
+  // A M A (symmetric, packed) and tr(M A)
+  double AM[N][N];
+  double trma = 0.0;
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      double s = 0.0;
+#pragma unroll
+      for (int m = 0; m < N; ++m) s = __builtin_fma(A[sx(i, m)], sc.mass[m * N + j], s);
+      AM[i][j] = s;
+      if (i == j) trma += s;
+    }
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = 0; j <= i; ++j) {
+      double s = 0.0;
+#pragma unroll
+      for (int m = 0; m < N; ++m) s = __builtin_fma(AM[i][m], A[sx(m, j)], s);
+      r[D::B_AMA + sx(i, j)] = s;
+    }
+  r[D::B_TRMA] = trma;
   // pk = 8^(n/4) |det L|^(3/2): the bra half of
   // sqrt(8^n) sqrt((|det L_k| |det L_l| / det B)^3) (python/matel.py:135)
   const double ad = fabs(detL);

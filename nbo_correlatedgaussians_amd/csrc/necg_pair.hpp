@@ -71,7 +71,7 @@ struct Dim {
   static constexpr int K_WC = K_PL + 1;  // scvv_P * C_l
   static constexpr int K_SC = K_WC + 1;  // scvv_P
   static constexpr int KS = (K_SC + 1 + 1) & ~1;
-  // bra-table record: A | S | AS | i1 pk C | L (vech) | 1/L_ii
+  // bra-table record: A | S | AS | i1 pk C | L (vech) | 1/L_ii | A M A | tr(M A)
   static constexpr int B_A = 0;
   static constexpr int B_S = B_A + NS;
   static constexpr int B_AS = B_S + N3;
@@ -80,7 +80,9 @@ struct Dim {
   static constexpr int B_C = B_PK + 1;
   static constexpr int B_L = B_C + 1;
   static constexpr int B_LD = B_L + NS;
-  static constexpr int BS = (B_LD + N + 1) & ~1;
+  static constexpr int B_AMA = B_LD + N;
+  static constexpr int B_TRMA = B_AMA + NS;
+  static constexpr int BS = (B_TRMA + 1 + 1) & ~1;
   // accumulator record of one bra: X_H Y_H X_S Y_S sigma_H sigma_S
   static constexpr int A_XH = 0;
   static constexpr int A_YH = A_XH + NS;

@@ -1,0 +1,595 @@
+// necg_pair2.hpp -- the two-sided pair term: one evaluation of the (bra k,
+// ket l, term P) element gives the bra-side gradient (w.r.t. L_k, s_k) and
+// the ket-side gradient (w.r.t. L_l, s_l) together, so every unordered pair
+// {k, l} is evaluated once per term instead of once per ordering.
+//
+// Why this is the reference's gradient.  make_grad takes, for every ordered
+// pair (l, k), the bra derivative of sum_P c_P <l|H|P k> (python/make_grad.py:
+// 34-55, python/gradel.py:27).  When every term P is a particle permutation
+// (sms = kron(P^T, I3)) under which masses and charges are invariant, and P^-1
+// is a term with c_{P^-1} = c_P, then <l|H|P k> = <k|H|P^-1 l>, so
+// sum_P c_P d_bra <l|H|P k> = sum_P c_P d_ket <k|H|P l>: the ket derivative of
+// the (k, l) evaluation replaces the separate (l, k) evaluation.
+//
+// The element is symmetric under (A, S) <-> (A', S'), so the ket-side
+// derivative w.r.t. (A', S') has the bra-side form with A <-> A',
+// dif1 <-> dif2 and 12 A'MA' <-> 12 A M A in Gam.  With a1 = A dif1
+// (A' dif2 = -a1) and Fc = Lb st - efs q e_z^T, B^-1 A' = I - B^-1 A gives
+//   Phi_bra = B^-1 (8 A'M a1 + Fc) = B^-1 Fc + 8 (M a1 - w),
+//   Phi_ket = B^-1 (-8 A M a1 + Fc) = B^-1 Fc - 8 w,       w = B^-1 A M a1,
+// and the kinetic trace 3 tr(M A B^-1 A') = 3 (tr(M A) - <A M A, B^-1>).
+// The ket-side (X', Y') are in the frame of A' = P^T A_l P, S' = P^T S_l; in
+// the ket's own frame they are P X' P^T and P Y', a relabelling of slots that
+// the per-term target table applies when the ket-side sums are added up.
+//
+// Cross-lane sums.  Lanes are 64 bras of one ket, so the ket side is summed
+// over the wave: 16 values at a time by a halving exchange (permlane32 swap,
+// permlane16 swap, row mirror, half-row mirror, then two quad exchanges),
+// after which lane j holds the full sum of value j >> 2 of the chunk.  Every
+// sum is formed in a fixed order, so results are bitwise reproducible.
+#pragma once
+
+#include "necg_pair.hpp"
+
+namespace necg {
+
+// Bra-side data of the two-sided term: per field f, the tile's 64 bras in
+// LDS ([f][lane], stride 64 doubles), so that no part of the record has to
+// stay in registers across the term.  Fields: A | S | AS | A M A | i1 pk C tr(MA).
+template <int N>
+struct BraLds {
+  static constexpr int F_A = 0;
+  static constexpr int F_S = F_A + Dim<N>::NS;
+  static constexpr int F_AS = F_S + Dim<N>::N3;
+  static constexpr int F_AMA = F_AS + Dim<N>::N3;
+  static constexpr int F_I1 = F_AMA + Dim<N>::NS;
+  static constexpr int F_PK = F_I1 + 1;
+  static constexpr int F_C = F_PK + 1;
+  static constexpr int F_TRMA = F_C + 1;
+  static constexpr int NF = F_TRMA + 1;
+  const double *p;  // this lane's field 0
+  __device__ __forceinline__ double f(int t) const { return p[64 * t]; }
+  __device__ __forceinline__ double A(int t) const { return f(F_A + t); }
+  __device__ __forceinline__ double S(int t) const { return f(F_S + t); }
+  __device__ __forceinline__ double AS(int t) const { return f(F_AS + t); }
+  __device__ __forceinline__ double AMA(int t) const { return f(F_AMA + t); }
+};
+
+// Fill the tile's bra table: field f of bra k0 + lane from the bra record.
+template <int N>
+__device__ __forceinline__ void fill_bra_lds(double *dst, const double *__restrict__ rec, int f,
+                                             int lane) {
+  using D = Dim<N>;
+  using L = BraLds<N>;
+  int src;
+  if (f < L::F_S) src = D::B_A + f;
+  else if (f < L::F_AS) src = D::B_S + (f - L::F_S);
+  else if (f < L::F_AMA) src = D::B_AS + (f - L::F_AS);
+  else if (f < L::F_I1) src = D::B_AMA + (f - L::F_AMA);
+  else if (f == L::F_I1) src = D::B_I1;
+  else if (f == L::F_PK) src = D::B_PK;
+  else if (f == L::F_C) src = D::B_C;
+  else src = D::B_TRMA;
+  dst[64 * f + lane] = rec[src];
+}
+
+// Compiler barrier between the phases of a term: memory operations are not
+// moved across it, so each phase reads the bra fields it needs right where
+// it needs them (no instruction is emitted).
+__device__ __forceinline__ void phase_barrier() { asm volatile("" ::: "memory"); }
+
+// F0 and dF0/dz as boys_f0 (python/utils.py:78-106), written so that the
+// 15 evaluations of a term form one straight-line block: the Pade part is
+// evaluated at min(z, 16.3578) (finite, and not used above the cut), each
+// Horner pass yields the polynomial and its derivative together (only the
+// reference's 11 constants appear, as addends), and the branch is a select
+// of finished values.  v = (P Q^3)^-1/2 gives F0 = P Q v and
+// dF0 = (Q P' - P Q') v / 2; above the cut v = z^-1/2, F0 = sqrt(pi)/2 v,
+// dF0 = -F0 v^2 / 2.
+__device__ __forceinline__ void boys_f0_line(double z, double &f, double &fp) {
+  const bool big = z > kZcut;
+  const double zc = big ? kZcut : z;
+  double P = kC1, dP = kC1;
+  P = __builtin_fma(P, zc, kC2);
+  dP = __builtin_fma(dP, zc, P);
+  P = __builtin_fma(P, zc, kC3);
+  dP = __builtin_fma(dP, zc, P);
+  P = __builtin_fma(P, zc, kC4);
+  dP = __builtin_fma(dP, zc, P);
+  P = __builtin_fma(P, zc, kC5);
+  dP = __builtin_fma(dP, zc, P);
+  P = __builtin_fma(P, zc, 1.0);
+  double Q = kC6, dQ = kC6;
+  Q = __builtin_fma(Q, zc, kC7);
+  dQ = __builtin_fma(dQ, zc, Q);
+  Q = __builtin_fma(Q, zc, kC8);
+  dQ = __builtin_fma(dQ, zc, Q);
+  Q = __builtin_fma(Q, zc, kC9);
+  dQ = __builtin_fma(dQ, zc, Q);
+  Q = __builtin_fma(Q, zc, kC10);
+  dQ = __builtin_fma(dQ, zc, Q);
+  Q = __builtin_fma(Q, zc, kC11);
+  dQ = __builtin_fma(dQ, zc, Q);
+  Q = __builtin_fma(Q, zc, 1.0);
+  const double PQ = P * Q;
+  const double v = rsq(big ? z : PQ * (Q * Q));
+  const double fa = kSqrtPiHalf * v;
+  double fb = 0.5 * __builtin_fma(Q, dP, -P * dQ) * v;
+  asm volatile("" : "+v"(fb));   // formed on every lane: no branch around it
+  f = big ? fa : PQ * v;
+  fp = big ? -0.5 * fa * (v * v) : fb;
+}
+
+// Ket-side values per evaluation: X_H, X_S interleaved per packed (i, j),
+// then Y_H, Y_S interleaved per (i, x), then sigma_H, sigma_S; padded to
+// whole chunks of 16.
+template <int N>
+struct Two {
+  static constexpr int NV = Dim<N>::NACC;
+  static constexpr int NCH = (NV + 15) / 16;
+  static constexpr int KV = NCH * 16;
+};
+
+__device__ __forceinline__ double lane_swap32_add(double x, double y) {
+  // v_permlane32_swap: the upper 32 lanes of x trade places with the lower
+  // 32 lanes of y.  Lanes 0-31 then hold (x, partner's x), lanes 32-63
+  // (partner's y, y): the sum is x summed over lanes i, i^32 in the lower
+  // half and y summed likewise in the upper half.
+  const unsigned long long ux = __builtin_bit_cast(unsigned long long, x);
+  const unsigned long long uy = __builtin_bit_cast(unsigned long long, y);
+  const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)ux, (unsigned)uy, false, false);
+  const auto hi = __builtin_amdgcn_permlane32_swap((unsigned)(ux >> 32), (unsigned)(uy >> 32),
+                                                   false, false);
+  const double nx = __builtin_bit_cast(double, ((unsigned long long)hi[0] << 32) | lo[0]);
+  const double ny = __builtin_bit_cast(double, ((unsigned long long)hi[1] << 32) | lo[1]);
+  return nx + ny;
+}
+
+__device__ __forceinline__ double lane_swap16_add(double x, double y) {
+  // v_permlane16_swap: odd 16-lane rows of x trade places with even rows of
+  // y; even rows then sum x over rows (2r, 2r+1), odd rows sum y.
+  const unsigned long long ux = __builtin_bit_cast(unsigned long long, x);
+  const unsigned long long uy = __builtin_bit_cast(unsigned long long, y);
+  const auto lo = __builtin_amdgcn_permlane16_swap((unsigned)ux, (unsigned)uy, false, false);
+  const auto hi = __builtin_amdgcn_permlane16_swap((unsigned)(ux >> 32), (unsigned)(uy >> 32),
+                                                   false, false);
+  const double nx = __builtin_bit_cast(double, ((unsigned long long)hi[0] << 32) | lo[0]);
+  const double ny = __builtin_bit_cast(double, ((unsigned long long)hi[1] << 32) | lo[1]);
+  return nx + ny;
+}
+
+// v from the DPP partner lane (CTRL: 0x140 row mirror, 0x141 half-row
+// mirror, quad permutations below 0x100); every lane has a partner.
+template <int CTRL>
+__device__ __forceinline__ double lane_dpp(double v) {
+  const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = __builtin_amdgcn_update_dpp(0u, (unsigned)u, CTRL, 0xf, 0xf, false);
+  const unsigned hi = __builtin_amdgcn_update_dpp(0u, (unsigned)(u >> 32), CTRL, 0xf, 0xf, false);
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+// One halving step with a DPP involution that pairs lanes with bit b clear
+// and set: lanes with the bit clear keep x summed over the pair, lanes with
+// it set keep y.
+template <int CTRL>
+__device__ __forceinline__ double lane_xchg_add(double x, double y, bool upper) {
+  const double keep = upper ? y : x;
+  const double send = upper ? x : y;
+  return keep + lane_dpp<CTRL>(send);
+}
+
+// Sum over the 64 lanes of v[0..15]; lane j returns the sum of v[j >> 2].
+__device__ __forceinline__ double wave_sum16(const double *v, int lane) {
+  double w[8], u[4], t[2];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) w[j] = lane_swap32_add(v[j], v[j + 8]);   // lane bit 5
+#pragma unroll
+  for (int j = 0; j < 4; ++j) u[j] = lane_swap16_add(w[j], w[j + 4]);   // lane bit 4
+  const bool b3 = (lane & 8) != 0, b2 = (lane & 4) != 0;
+#pragma unroll
+  for (int j = 0; j < 2; ++j) t[j] = lane_xchg_add<0x140>(u[j], u[j + 2], b3);  // row mirror
+  double r = lane_xchg_add<0x141>(t[0], t[1], b2);                     // half-row mirror
+  r += lane_dpp<0x4E>(r);   // quad_perm [2,3,0,1]: lane bit 1
+  r += lane_dpp<0xB1>(r);   // quad_perm [1,0,3,2]: lane bit 0
+  return r;
+}
+
+// One (bra, ket-under-term) evaluation, both sides.  kd: the ket record;
+// mb, mk: 1 where this lane's bra-side / ket-side contribution counts, else
+// 0; pw: this lane's four target slots (one byte per chunk, 255 = none) in
+// the ket's accumulator record for the values it holds after each chunk sum.
+// Adds sc*ov*alpha to hel and sc*ov to sel, the bra-side gradient (weight
+// scvv_P C_l) to the lane's LDS slots and the ket-side gradient (weight
+// scvv_P C_k, summed over the wave) to the ket accumulator kacc.
+template <int N>
+__device__ __forceinline__ bool pair_term2(const BraLds<N> &b, const double *__restrict__ kd,
+                                           const SysConst &sc_, double mb, double mk,
+                                           unsigned pw, double &hel, double &sel, LdsSink &sink,
+                                           double *kacc, int lane) {
+  using D = Dim<N>;
+  constexpr int NS = D::NS, N3 = D::N3;
+  const double *__restrict__ Ap = kd + D::K_AP;
+  const double *__restrict__ Sp = kd + D::K_SP;
+  const double *__restrict__ ApSp = kd + D::K_APSP;
+  const double *__restrict__ Np = kd + D::K_NP;
+  const double i2 = kd[D::K_I2], pl = kd[D::K_PL], scv = kd[D::K_SC];
+  const double wb = mb * kd[D::K_WC];      // scvv_P C_l
+
+  // Cholesky of B = A + A' (lower, packed), rd[j] = 1 / R_jj.
+  double R[NS], rd[N];
+  bool ok = true;
+#pragma unroll
+  for (int j = 0; j < N; ++j) {
+    double s = b.A(sx(j, j)) + Ap[sx(j, j)];
+#pragma unroll
+    for (int m = 0; m < j; ++m) s = __builtin_fma(-R[sx(j, m)], R[sx(j, m)], s);
+    ok = ok && (s > 0.0);
+    const double r = rsq(s);
+    rd[j] = r;
+    R[sx(j, j)] = s * r;
+#pragma unroll
+    for (int i = j + 1; i < N; ++i) {
+      double t = b.A(sx(i, j)) + Ap[sx(i, j)];
+#pragma unroll
+      for (int m = 0; m < j; ++m) t = __builtin_fma(-R[sx(i, m)], R[sx(j, m)], t);
+      R[sx(i, j)] = t * r;
+    }
+  }
+  double Ri[NS];
+#pragma unroll
+  for (int j = 0; j < N; ++j) {
+    Ri[sx(j, j)] = rd[j];
+#pragma unroll
+    for (int i = j + 1; i < N; ++i) {
+      double t = 0.0;
+#pragma unroll
+      for (int m = j; m < i; ++m) t = __builtin_fma(R[sx(i, m)], Ri[sx(m, j)], t);
+      Ri[sx(i, j)] = -t * rd[i];
+    }
+  }
+  double Bi[NS];
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = 0; j <= i; ++j) {
+      double t = 0.0;
+#pragma unroll
+      for (int m = i; m < N; ++m) t = __builtin_fma(Ri[sx(m, i)], Ri[sx(m, j)], t);
+      Bi[sx(i, j)] = t;
+    }
+  double det32 = 1.0;
+#pragma unroll
+  for (int j = 0; j < N; ++j) det32 *= rd[j] * rd[j] * rd[j];
+  phase_barrier();
+
+  // E, st = B^-1 E, i3 (python/matel.py:125-133)
+  double st[N3];
+  double i3 = 0.0;
+  {
+    double Em[N3];
+#pragma unroll
+    for (int t = 0; t < N3; ++t) Em[t] = b.AS(t) + ApSp[t];
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+      for (int x = 0; x < 3; ++x) {
+        double t = 0.0;
+#pragma unroll
+        for (int j = 0; j < N; ++j) t = __builtin_fma(Bi[sx(i, j)], Em[3 * j + x], t);
+        st[3 * i + x] = t;
+        i3 = __builtin_fma(Em[3 * i + x], t, i3);
+      }
+  }
+  const double ov = b.f(BraLds<N>::F_PK) * pl * det32 * exp(i3 - b.f(BraLds<N>::F_I1) - i2);
+  phase_barrier();
+
+  // Kinetic (python/matel.py:144-155): a1 = A dif1, Ma1 = M a1,
+  // <A dif1, M A' dif2> = -<a1, M a1>; 3 tr(M A B^-1 A') = 3 (tr(MA) - <AMA, B^-1>).
+  double Ma1[N3];
+  double i3k = 0.0;
+  {
+    double a1[N3];
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+      for (int x = 0; x < 3; ++x) {
+        double v = 0.0;
+#pragma unroll
+        for (int j = 0; j < N; ++j)
+          v = __builtin_fma(b.A(sx(i, j)), st[3 * j + x] - b.S(3 * j + x), v);
+        a1[3 * i + x] = v;
+      }
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+      for (int x = 0; x < 3; ++x) {
+        double t = 0.0;
+#pragma unroll
+        for (int j = 0; j < N; ++j) t = __builtin_fma(sc_.msym[sx(i, j)], a1[3 * j + x], t);
+        Ma1[3 * i + x] = t;
+        i3k = __builtin_fma(-a1[3 * i + x], t, i3k);
+      }
+  }
+  double ab = 0.0;  // <AMA, B^-1> over the full symmetric matrices
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = 0; j <= i; ++j)
+      ab = __builtin_fma((i == j ? 1.0 : 2.0) * b.AMA(sx(i, j)), Bi[sx(i, j)], ab);
+  const double tfac = 2.0 * i3k + 3.0 * (b.f(BraLds<N>::F_TRMA) - ab);
+  phase_barrier();
+
+  // Coulomb (python/matel.py:162-197; gradients python/gradel.py:266-337).
+  double csum = 0.0;
+  double La[NS], Lb[NS];
+#pragma unroll
+  for (int t = 0; t < NS; ++t) {
+    La[t] = 0.0;
+    Lb[t] = 0.0;
+  }
+  {
+    int pi = 0;
+#pragma unroll
+    for (int p = 0; p <= N; ++p)
+#pragma unroll
+      for (int q = p + 1; q <= N; ++q, ++pi) {
+        const int a = p - 1, bq = q - 1;
+        double i2p, dx, dy, dz;
+        if (p == 0) {
+          i2p = Bi[sx(bq, bq)];
+          dx = st[3 * bq];
+          dy = st[3 * bq + 1];
+          dz = st[3 * bq + 2];
+        } else {
+          i2p = Bi[sx(a, a)] + Bi[sx(bq, bq)] - 2.0 * Bi[sx(a, bq)];
+          dx = st[3 * a] - st[3 * bq];
+          dy = st[3 * a + 1] - st[3 * bq + 1];
+          dz = st[3 * a + 2] - st[3 * bq + 2];
+        }
+        const double ri = rsq(i2p);
+        const double inv2 = ri * ri;
+        const double z = (dx * dx + dy * dy + dz * dz) * inv2;
+        double f, fp;
+        boys_f0_line(z, f, fp);
+        const double c = sc_.cpair[pi];
+        const double cr0 = c * ri * kInvSqrtPi;
+        csum = __builtin_fma(cr0, f, csum);
+        const double cb = 4.0 * fp * cr0 * inv2;
+        const double ca = __builtin_fma(cb, z, 2.0 * f * cr0 * inv2);
+        if (p == 0) {
+          La[sx(bq, bq)] += ca;
+          Lb[sx(bq, bq)] += cb;
+        } else {
+          La[sx(a, a)] += ca;
+          La[sx(bq, bq)] += ca;
+          La[sx(a, bq)] -= ca;
+          Lb[sx(a, a)] += cb;
+          Lb[sx(bq, bq)] += cb;
+          Lb[sx(a, bq)] -= cb;
+        }
+      }
+  }
+  // Stark (python/matel.py:205-208)
+  double zsum = 0.0;
+#pragma unroll
+  for (int i = 0; i < N; ++i) zsum = __builtin_fma(sc_.q[i + 1], st[3 * i + 2], zsum);
+  const double alpha = 2.0 * tfac + 2.0 * csum - sc_.efs * zsum;
+  hel = __builtin_fma(scv * ov, alpha, hel);
+  sel = __builtin_fma(scv, ov, sel);
+
+  phase_barrier();
+  // Shared gradient pieces: Phic = B^-1 (Lb st - efs q e_z^T), w = B^-1 A M a1.
+  double Phic[N3], w[N3];
+  {
+    double Fc[N3], v[N3];
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+      for (int x = 0; x < 3; ++x) {
+        double t = 0.0, u = 0.0;
+#pragma unroll
+        for (int m = 0; m < N; ++m) {
+          t = __builtin_fma(Lb[sx(i, m)], st[3 * m + x], t);
+          u = __builtin_fma(b.A(sx(i, m)), Ma1[3 * m + x], u);
+        }
+        if (x == 2) t = __builtin_fma(-sc_.efs, sc_.q[i + 1], t);
+        Fc[3 * i + x] = t;
+        v[3 * i + x] = u;
+      }
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+      for (int x = 0; x < 3; ++x) {
+        double t = 0.0, u = 0.0;
+#pragma unroll
+        for (int m = 0; m < N; ++m) {
+          t = __builtin_fma(Bi[sx(i, m)], Fc[3 * m + x], t);
+          u = __builtin_fma(Bi[sx(i, m)], v[3 * m + x], u);
+        }
+        Phic[3 * i + x] = t;
+        w[3 * i + x] = u;
+      }
+  }
+
+  phase_barrier();
+  // Bra side: Phi = Phic + 8 (M a1 - w), G = B^-1 (12 A'MA' + La) B^-1.
+  {
+    double Phi[N3], d1[N3];
+#pragma unroll
+    for (int t = 0; t < N3; ++t) {
+      Phi[t] = __builtin_fma(8.0, Ma1[t] - w[t], Phic[t]);
+      d1[t] = st[t] - b.S(t);
+    }
+    double Tm[N * N];
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+      for (int j = 0; j < N; ++j) {
+        double t = 0.0;
+#pragma unroll
+        for (int m = 0; m < N; ++m)
+          t = __builtin_fma(__builtin_fma(12.0, Np[sx(i, m)], La[sx(i, m)]), Bi[sx(m, j)], t);
+        Tm[i * N + j] = t;
+      }
+    const double wo = wb * ov;
+    const double woa = wo * alpha;
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+      for (int j = 0; j <= i; ++j) {
+        double g = 0.0;
+#pragma unroll
+        for (int m = 0; m < N; ++m) g = __builtin_fma(Bi[sx(i, m)], Tm[m * N + j], g);
+        double dd = 0.0, dp = 0.0;
+#pragma unroll
+        for (int x = 0; x < 3; ++x) {
+          dd = __builtin_fma(d1[3 * i + x], d1[3 * j + x], dd);
+          dp = __builtin_fma(d1[3 * i + x], Phi[3 * j + x], dp);
+          dp = __builtin_fma(Phi[3 * i + x], d1[3 * j + x], dp);
+        }
+        const double x0 = __builtin_fma(-3.0, Bi[sx(i, j)], -2.0 * dd);
+        const int t = sx(i, j);
+        sink.add(D::A_XH + t, __builtin_fma(woa, x0, wo * (g - dp)));
+        sink.add(D::A_XS + t, wo * x0);
+      }
+#pragma unroll
+    for (int t = 0; t < N3; ++t) {
+      sink.add(D::A_YH + t, __builtin_fma(2.0 * woa, d1[t], wo * Phi[t]));
+      sink.add(D::A_YS + t, 2.0 * wo * d1[t]);
+    }
+    sink.add(D::A_SH, woa);
+    sink.add(D::A_SS, wo);
+  }
+
+  phase_barrier();
+  // Ket side: Phi' = Phic - 8 w, G' = B^-1 (12 A M A + La) B^-1, dif2 = st - S'.
+  {
+    const double wk = mk * scv * b.f(BraLds<N>::F_C);   // scvv_P C_k
+    double kv[Two<N>::KV];
+    double Phk[N3], d2[N3];
+#pragma unroll
+    for (int t = 0; t < N3; ++t) {
+      Phk[t] = __builtin_fma(-8.0, w[t], Phic[t]);
+      d2[t] = st[t] - Sp[t];
+    }
+    double Tm[N * N];
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+      for (int j = 0; j < N; ++j) {
+        double t = 0.0;
+#pragma unroll
+        for (int m = 0; m < N; ++m)
+          t = __builtin_fma(__builtin_fma(12.0, b.AMA(sx(i, m)), La[sx(i, m)]), Bi[sx(m, j)], t);
+        Tm[i * N + j] = t;
+      }
+    const double wo = wk * ov;
+    const double woa = wo * alpha;
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+      for (int j = 0; j <= i; ++j) {
+        double g = 0.0;
+#pragma unroll
+        for (int m = 0; m < N; ++m) g = __builtin_fma(Bi[sx(i, m)], Tm[m * N + j], g);
+        double dd = 0.0, dp = 0.0;
+#pragma unroll
+        for (int x = 0; x < 3; ++x) {
+          dd = __builtin_fma(d2[3 * i + x], d2[3 * j + x], dd);
+          dp = __builtin_fma(d2[3 * i + x], Phk[3 * j + x], dp);
+          dp = __builtin_fma(Phk[3 * i + x], d2[3 * j + x], dp);
+        }
+        const double x0 = __builtin_fma(-3.0, Bi[sx(i, j)], -2.0 * dd);
+        const int t = sx(i, j);
+        kv[2 * t] = __builtin_fma(woa, x0, wo * (g - dp));
+        kv[2 * t + 1] = wo * x0;
+      }
+#pragma unroll
+    for (int t = 0; t < N3; ++t) {
+      kv[2 * NS + 2 * t] = __builtin_fma(2.0 * woa, d2[t], wo * Phk[t]);
+      kv[2 * NS + 2 * t + 1] = 2.0 * wo * d2[t];
+    }
+    kv[2 * NS + 2 * N3] = woa;
+    kv[2 * NS + 2 * N3 + 1] = wo;
+#pragma unroll
+    for (int t = Two<N>::NV; t < Two<N>::KV; ++t) kv[t] = 0.0;
+#pragma unroll
+    for (int ch = 0; ch < Two<N>::NCH; ++ch) {
+      const double r = wave_sum16(kv + 16 * ch, lane);
+      const unsigned tgt = (pw >> (8 * ch)) & 255u;
+      if ((lane & 3) == 0 && tgt != 255u)
+        __hip_atomic_fetch_add(kacc + tgt, r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+    }
+  }
+  return ok;
+}
+
+// One two-sided pair tile: bras [k0, k0+64) x kets [l0, l1) x terms [p0, p1)
+// with l1 <= k0 + 64, run by the pair_waves(N) waves of a workgroup (each
+// takes alternate kets).  Only pairs with bra k >= ket l contribute: the
+// bra side of every such pair and the ket side of those with k > l.
+// Writes the element partials of k >= l to hp/sp ([l][k]), the tile's bra
+// accumulators to bpt ([slot][k], NACC x K) and, per ket, its accumulator
+// record summed over the tile's bras to kpt ([l][slot], K x NACC).
+template <int N>
+__device__ __forceinline__ void pair2_tile(const double *__restrict__ bra,
+                                           const double *__restrict__ ket, int K, int nst, int k0,
+                                           int l0, int l1, int p0, int p1, const SysConst &sc,
+                                           const unsigned *__restrict__ tperm,
+                                           double *__restrict__ bpt, double *__restrict__ kpt,
+                                           double *__restrict__ hp, double *__restrict__ sp,
+                                           int *__restrict__ flag, double *acc_s,
+                                           double *kacc_s, double *bra_s) {
+  using D = Dim<N>;
+  constexpr int W = pair_waves(N);
+  constexpr int NA = D::NACC * 64;
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int k = k0 + lane;
+  const bool active = k < K;
+  const int kk = active ? k : K - 1;
+  for (int v = 0; v < D::NACC; ++v) acc_s[wv * NA + v * 64 + lane] = 0.0;
+  double *kacc = kacc_s + wv * 64;
+  kacc[lane] = 0.0;
+  for (int f = wv; f < BraLds<N>::NF; f += W) fill_bra_lds<N>(bra_s, bra + (size_t)kk * D::BS, f, lane);
+  __syncthreads();
+  const BraLds<N> b{bra_s + lane};
+  LdsSink sink{acc_s + wv * NA + lane};
+  bool ok = true;
+  for (int l = l0 + wv; l < l1; l += W) {
+    const bool valid = active && k >= l;
+    const double mb = valid ? 1.0 : 0.0;
+    const double mk = (valid && k != l) ? 1.0 : 0.0;
+    double hel = 0.0, sel = 0.0;
+    const double *kl = ket + (size_t)l * nst * D::KS;
+    for (int p = p0; p < p1; ++p)
+      ok &= pair_term2<N>(b, kl + (size_t)p * D::KS, sc, mb, mk, tperm[p * 64 + lane], hel, sel,
+                          sink, kacc, lane);
+    if (valid) {
+      const size_t o = (size_t)l * K + k;
+      hp[o] = hel;
+      sp[o] = sel;
+    }
+    // this ket's record, summed over the tile's bras and terms: read and
+    // clear (the wave's own LDS operations complete in order)
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    if (lane < D::NACC) {
+      const double v = __hip_atomic_exchange(kacc + lane, 0.0, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_WAVEFRONT);
+      kpt[(size_t)l * D::NACC + lane] = v;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  }
+  if (active && !ok) atomicOr(flag, (int)F_NOT_PD);
+  __syncthreads();
+  if (!active) return;
+  double *o = bpt + k;
+  for (int v = wv; v < D::NACC; v += W) {
+    double t = 0.0;
+#pragma unroll
+    for (int u = 0; u < W; ++u) t += acc_s[u * NA + v * 64 + lane];
+    o[(size_t)v * K] = t;
+  }
+}
+
+}  // namespace necg

@@ -65,8 +65,8 @@ __global__ void __launch_bounds__(256) t_prep(const PoolSys *__restrict__ sys,
   double *w = ws + (size_t)t * wstride;
   if (threadIdx.x == 0) {
     POOL_SWITCH_N(b.n, (make_bra<NN>(newp + (size_t)t * pstride,
-                                     b.bra[(size_t)k * Dim<NN>::BS + Dim<NN>::B_C], true, w,
-                                     flag)));
+                                     b.bra[(size_t)k * Dim<NN>::BS + Dim<NN>::B_C], true, b.sc,
+                                     w, flag)));
   }
   __syncthreads();
   for (int p = threadIdx.x; p < b.nst; p += blockDim.x) {

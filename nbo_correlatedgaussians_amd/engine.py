@@ -189,6 +189,19 @@ class Engine:
         _lib.raise_for(rc, "necg_kinetic_potential")
         return T, V
 
+    def set_pair_mode(self, mode: int) -> None:
+        """Form of the gradient pair pass: 0 two-sided where it applies
+        (default), 1 two-sided (raises if the terms do not allow it), 2 one
+        evaluation per ordered pair."""
+        _lib.raise_for(self.lib.necg_set_pair_mode(self._ctx, int(mode)), "necg_set_pair_mode")
+
+    def pair_form(self):
+        """(two-sided form available for this system, last gradient pass used it)."""
+        a, b = ct.c_int32(), ct.c_int32()
+        _lib.raise_for(self.lib.necg_pair_form(self._ctx, ct.byref(a), ct.byref(b)),
+                       "necg_pair_form")
+        return bool(a.value), bool(b.value)
+
     def set_dipole_mode(self, mode: int) -> None:
         """0: symmetry-reduced dipole pass when the terms allow it (default);
         1: reduced form required; 2: the full nst^2 double sum."""

@@ -1,0 +1,126 @@
+"""The two-sided gradient pair pass (each unordered pair once per term, bra
+and ket derivatives together) against the per-ordering pass and the oracle.
+
+Tolerances: the two forms sum the same terms in different orders, so they
+agree to |dE| <= 1e-13 max(|E|, 1) and max|dg| <= 1e-11 max(max|g|, 1); each
+matches the oracle to the parity bar (1e-12 energy, 1e-9 gradient)."""
+
+import numpy as np
+import pytest
+
+import nbo_correlatedgaussians_amd as necg
+from nbo_correlatedgaussians_amd import _lib
+from conftest import system_by_name
+
+pytestmark = pytest.mark.gpu
+
+
+def _engine(s):
+    from nbo_correlatedgaussians_amd.engine import Engine
+    return Engine(s, "cuda:0")
+
+
+def _close(E, g, Eref, gref, te, tg):
+    return (abs(E - Eref) <= te * max(abs(Eref), 1.0)
+            and np.max(np.abs(g - gref)) <= tg * max(np.max(np.abs(gref)), 1.0))
+
+
+@pytest.mark.parametrize("name,K,seed", [("LiH", 37, 5), ("He", 70, 6), ("H2", 130, 7),
+                                         ("Be", 65, 8), ("LiH", 1, 9)])
+def test_two_sided_matches_per_ordering_and_oracle(oracle_lib, name, K, seed):
+    s = system_by_name(name)
+    iv, C = necg.generate_parameters(s, K, seed)
+    x = necg.pack_x(iv, C)
+    print(f"two-sided {name} K={K} seed={seed}", flush=True)
+    eng = _engine(s)
+    assert eng.pair_form()[0]
+    eng.set_pair_mode(1)
+    E2, g2 = eng.outprod(x)
+    assert eng.pair_form() == (True, True)
+    eng.set_pair_mode(2)
+    E1, g1 = eng.outprod(x)
+    assert eng.pair_form() == (True, False)
+    assert _close(E2, g2, E1, g1, 1e-13, 1e-11)
+    Eo, go = oracle_lib.Oracle(s).outprod(x)
+    assert _close(E2, g2, Eo, go, 1e-12, 1e-9)
+    # the no-gradient forward runs the lower triangle only; same energy
+    assert abs(eng.energy(x) - E2) <= 1e-13 * max(abs(E2), 1.0)
+
+
+def test_two_sided_tilings_agree_and_repeat(oracle_lib):
+    """Every ket tile / term-group shape of the two-sided pass gives the
+    oracle's result, and repeated calls with one shape are bitwise equal."""
+    s = necg.lih_system()
+    K = 150
+    iv, C = necg.generate_parameters(s, K, 3)
+    x = necg.pack_x(iv, C)
+    Eo, go = oracle_lib.Oracle(s).outprod(x)
+    eng = _engine(s)
+    eng.set_pair_mode(1)
+    for kt, ts in ((0, 0), (1, 1), (5, 3), (16, 24), (64, 2), (150, 1), (7, 4)):
+        print(f"two-sided tiling kt={kt} ts={ts}", flush=True)
+        eng.set_tiling(kt, ts, 0)
+        runs = [eng.outprod(x) for _ in range(3)]
+        for E, g in runs:
+            assert _close(E, g, Eo, go, 1e-12, 1e-9), (kt, ts)
+        assert all(E == runs[0][0] and np.array_equal(g, runs[0][1]) for E, g in runs), (kt, ts)
+        assert eng.pair_form()[1]
+
+
+def test_two_sided_tape_and_vjp():
+    """energy_fwd takes the two-sided pass and its tape gives the outprod
+    gradient; a change of form drops the tape (the backward recomputes)."""
+    import torch
+    s = necg.lih_system()
+    iv, C = necg.generate_parameters(s, 90, 11)
+    x = necg.pack_x(iv, C)
+    eng = _engine(s)
+    E, g = eng.outprod(x)
+    xd = torch.as_tensor(x, device="cuda:0")
+    Ef, tape = eng.energy_fwd_t(xd)
+    assert eng.pair_form()[1]
+    gt = eng.energy_bwd_t(xd, tape).cpu().numpy()
+    assert float(Ef) == E
+    assert np.array_equal(gt, g)
+    Ef, tape = eng.energy_fwd_t(xd)
+    eng.set_pair_mode(2)
+    assert eng.energy_bwd_t(xd, tape) is None       # stale: the form changed
+    gv = eng.vjp_t(xd).cpu().numpy()
+    eng.check()
+    assert _close(E, gv, E, g, 0.0, 1e-11)
+
+
+def _drop_term(s, p):
+    keep = [q for q in range(s.nst) if q != p]
+    t = necg.System(n=s.n, nst=len(keep), efs=s.efs, mass=s.mass.copy(), ncv=s.ncv.copy(),
+                    scvv_raw=s.scvv_raw[keep].copy(), sml=s.sml[keep].copy(),
+                    sms=s.sms[keep].copy(), name=s.name + "-drop")
+    t.validate()
+    return t
+
+
+def test_two_sided_refused_where_it_does_not_apply(oracle_lib):
+    """Without the inverse of a term (LiH less one 3-cycle), or with masses
+    that a term does not preserve, the two-sided form is unavailable: forcing
+    it is refused and the default runs the per-ordering pass (oracle parity)."""
+    s = necg.lih_system()
+    cyc = next(p for p in range(s.nst) if not np.array_equal(s.sml[p], s.sml[p].T))
+    cases = [_drop_term(s, cyc)]
+    m = s.mass.copy()
+    m[1, 1] *= 1.5                      # electron 1 heavier: P(1 2) no longer a symmetry
+    cases.append(necg.System(n=s.n, nst=s.nst, efs=s.efs, mass=m, ncv=s.ncv.copy(),
+                             scvv_raw=s.scvv_raw.copy(), sml=s.sml.copy(), sms=s.sms.copy(),
+                             name="LiH-mass"))
+    for t in cases:
+        print(f"fallback {t.name}", flush=True)
+        iv, C = necg.generate_parameters(t, 20, 4)
+        x = necg.pack_x(iv, C)
+        eng = _engine(t)
+        assert eng.pair_form()[0] is False
+        with pytest.raises(_lib.NecgError) as ei:
+            eng.set_pair_mode(1)
+        assert ei.value.code == _lib.NECG_ERR_UNSUPPORTED
+        E, g = eng.outprod(x)
+        assert eng.pair_form() == (False, False)
+        Eo, go = oracle_lib.Oracle(t).outprod(x)
+        assert _close(E, g, Eo, go, 1e-12, 1e-9)
