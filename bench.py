@@ -190,6 +190,34 @@ def cpu_optimizer(opt):
     return out
 
 
+def per_ordering_leg(eng, xd, K, reps=3):
+    """The same outprod with the per-ordering pair pass (one evaluation per
+    ordered pair, bra derivatives only, as the reference's make_grad loop):
+    the gain of the two-sided form on this workload."""
+    import torch
+    f = torch.empty(1, dtype=torch.float64, device=xd.device)
+    g = torch.empty_like(xd)
+    eng.set_pair_mode(2)
+    try:
+        eng.outprod_t(xd, f, g)
+        eng.timing(1)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            eng.outprod_t(xd, f, g)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1000.0 / reps
+        pair_ms, launches = eng.timing(2)
+        eng.timing(0)
+        eng.check("per-ordering leg")
+    finally:
+        eng.set_pair_mode(0)
+    return {"workload": f"LiH K={K}: outprod(x) with one pair evaluation per ordered pair",
+            "kernel": "pair_kernel<5,MODE_SPLIT>", "ms": ms,
+            "pair_kernel_ms": pair_ms / max(launches, 1),
+            "pairs_per_s": K * (K + 1) / 2 / (ms / 1000.0), "energy": float(f.item())}
+
+
 def properties_leg(eng, xd, K, reps=5):
     """The driver's dipole pass (both sides symmetrised: nst^2 term pairs per
     basis pair) on the headline basis (LiH K = 2048, seed 1)."""
@@ -266,12 +294,12 @@ def _recorded(fname, key):
 
 def recorded_traffic():
     """HBM bytes per pair-kernel launch (FETCH_SIZE/WRITE_SIZE passes)."""
-    return _recorded("pair_kernel_lih_k2048_traffic.md", "traffic_bytes_per_launch")
+    return _recorded("pair2_kernel_lih_k2048_traffic.md", "traffic_bytes_per_launch")
 
 
 def recorded_executed_flop():
     """Executed FP64 FLOP per pair-kernel launch (SQ_INSTS_VALU_*_F64 passes)."""
-    return _recorded("pair_kernel_lih_k2048_fp64_counters.md", "executed_flop_per_launch")
+    return _recorded("pair2_kernel_lih_k2048_fp64_counters.md", "executed_flop_per_launch")
 
 
 def main():
@@ -307,7 +335,8 @@ def main():
     import torch
     import nbo_correlatedgaussians_amd as necg
     from nbo_correlatedgaussians_amd.engine import Engine
-    from nbo_correlatedgaussians_amd.flops import flops_per_eval, pairs_per_eval
+    from nbo_correlatedgaussians_amd.flops import (flops_per_eval, flops_per_eval_v0,
+                                                   pairs_per_eval, two_sided_term_flops)
 
     sysd = necg.lih_system()
     K = args.K
@@ -347,11 +376,13 @@ def main():
     ms_step = (t1 - t0) * 1000.0 / args.steps
     pairs = pairs_per_eval(K)
     value = pairs / (ms_step / 1000.0)
+    two_sided = eng.pair_form()[1]
     fl = flops_per_eval(sysd.n, sysd.nst, K)
+    fl_v0 = flops_per_eval_v0(sysd.n, sysd.nst, K)
     pair_ms = ms_pair / max(launches, 1)
     achieved = fl / (pair_ms / 1000.0) / 1e12
     kt, ts, w = eng.tiling()
-    ex_fl = recorded_executed_flop() if (K == 2048 and args.seed == 1) else None
+    ex_fl = recorded_executed_flop() if (K == 2048 and args.seed == 1 and two_sided) else None
     out = {
         "metric": "ECG pair energy+grad evals/s (FP64, % of peak); served p50/p99 latency",
         "value": value,
@@ -372,22 +403,30 @@ def main():
                    "K": K, "pairs": pairs, "parallelism": "single GPU"},
         "roofline": {"bound": "mfma", "engine": "FP64 VALU (MI355X FP64 vector and matrix "
                                                  "peaks are both 78.6 TFLOP/s)",
-                     "kernel": "pair_kernel<5,MODE_SPLIT>",
+                     "kernel": "pair2_kernel<5>" if two_sided else "pair_kernel<5,MODE_SPLIT>",
                      "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64_TFLOPS,
-                     "traffic": recorded_traffic() if (K == 2048 and args.seed == 1) else None,
-                     "traffic_source": "profiles/pair_kernel_lih_k2048_traffic.md",
+                     "traffic": (recorded_traffic()
+                                 if (K == 2048 and args.seed == 1 and two_sided) else None),
+                     "traffic_source": "profiles/pair2_kernel_lih_k2048_traffic.md",
+                     "flop_model": (f"two-sided n-space op count, {two_sided_term_flops(sysd.n)} "
+                                    f"FLOP per (pair, term) (flops.two_sided_term)"),
                      "flop_per_launch": fl, "pair_kernel_ms": pair_ms,
+                     "v0_model_flop_per_launch": fl_v0,
+                     "v0_model_tflops": fl_v0 / (pair_ms / 1000.0) / 1e12,
                      "executed_flop_per_launch": ex_fl,
                      "executed_frac": (ex_fl / (pair_ms / 1000.0) / 1e12 / PEAK_FP64_TFLOPS
                                        if ex_fl else None),
-                     "executed_source": "profiles/pair_kernel_lih_k2048_fp64_counters.md",
+                     "executed_source": "profiles/pair2_kernel_lih_k2048_fp64_counters.md",
                      "tiling": {"ket_tile": kt, "term_split": ts, "waves": w}},
         "energy": float(E.item()),
     }
     if args.path == "autograd":
         out["outprod"] = outprod_leg(eng, xd, K)
         print(f"bench: outprod leg done", file=sys.stderr, flush=True)
+    if two_sided:
+        out["per_ordering"] = per_ordering_leg(eng, xd, K)
+        print(f"bench: per-ordering leg done", file=sys.stderr, flush=True)
     if not args.no_props:
         out["properties"] = properties_leg(eng, xd, K)
         print(f"bench: properties leg done", file=sys.stderr, flush=True)

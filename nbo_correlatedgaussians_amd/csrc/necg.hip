@@ -26,6 +26,7 @@
 #include "../../include/necg.h"
 #include "necg_ctx.hpp"
 #include "necg_pair2.hpp"
+#include "necg_two.hpp"
 
 using namespace necg;
 
@@ -107,10 +108,11 @@ __global__ void __launch_bounds__(64 * pair_waves(N), NECG_MIN_WAVES) pair2_kern
   __shared__ double bra_s[BraLds<N>::NF * 64];
   const int bx = tiles[2 * blockIdx.x], by = tiles[2 * blockIdx.x + 1];
   const int k0 = bx * 64, l0 = by * kt;
-  const int l1 = min(min(K, l0 + kt), k0 + 64);
+  const int up = tri_up(K, kt), lup = k0 < up ? up : K;
+  const int l1 = l0 < lup ? min(min(K, l0 + kt), k0 + 64) : min(K, l0 + kt);
   const int z = blockIdx.y, ts = gridDim.y;
   const int p0 = z * tps;
-  pair2_tile<N>(bra, ket, K, nst, k0, l0, l1, p0, min(nst, p0 + tps), sc, tperm,
+  pair2_tile<N>(bra, ket, K, nst, k0, l0, l1, lup, p0, min(nst, p0 + tps), sc, tperm,
                 bpart + (size_t)(by * ts + z) * D::NACC * K,
                 kpart + (size_t)(bx * ts + z) * K * D::NACC, hpart + (size_t)z * K * K,
                 spart + (size_t)z * K * K, flag, acc_s, kacc_s, bra_s);
@@ -127,17 +129,15 @@ __global__ void __launch_bounds__(64 * pair_waves(N), NECG_MIN_WAVES) pair_tri_k
   __shared__ double acc_s[PairLds<N, MODE>::ACC];
   const int bx = tiles[2 * blockIdx.x], by = tiles[2 * blockIdx.x + 1];
   const int k0 = bx * 64, l0 = by * kt;
-  const int l1 = min(min(K, l0 + kt), k0 + 64);
+  const int up = tri_up(K, kt), lup = k0 < up ? up : K;
+  const int l1 = l0 < lup ? min(min(K, l0 + kt), k0 + 64) : min(K, l0 + kt);
   const int z = blockIdx.y;
   const int p0 = z * tps;
   pair_tile<N, MODE>(bra, ket, K, nst, k0, l0, l1, p0, min(nst, p0 + tps), sc, 0.0, nullptr,
-                     hpart + (size_t)z * K * K, spart + (size_t)z * K * K, flag, acc_s);
+                     hpart + (size_t)z * K * K, spart + (size_t)z * K * K, flag, acc_s, lup);
 }
 
-// Sum of the two-sided partials of function k, slot v: its bra partials over
-// the ket tiles of its bra tile's row (ket tile start <= bra tile end) and
-// its ket partials over the bra tiles from its own onwards.  Fixed order.
-// Grid (ceil(K/256), nv), 256 lanes.
+// Two-sided partial sums per (function, slot).  Grid (ceil(K/256), nv), 256 lanes.
 __global__ void __launch_bounds__(256) tile_reduce2(const double *__restrict__ bpart,
                                                     const double *__restrict__ kpart, int K,
                                                     int kt, int ts, int nacc,
@@ -145,13 +145,7 @@ __global__ void __launch_bounds__(256) tile_reduce2(const double *__restrict__ b
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   const int v = blockIdx.y;
   if (k >= K) return;
-  const int bx = k >> 6;
-  const int nkt = (K + kt - 1) / kt, nbt = (K + 63) / 64;
-  const int ny = min(nkt, (64 * bx + 63) / kt + 1);
-  double s = 0.0;
-  for (int t = 0; t < ny * ts; ++t) s += bpart[((size_t)t * nacc + v) * K + k];
-  for (int t = bx * ts; t < nbt * ts; ++t) s += kpart[((size_t)t * K + k) * nacc + v];
-  red[(size_t)v * K + k] = s;
+  red[(size_t)v * K + k] = tile_reduce2_one(bpart, kpart, K, kt, ts, nacc, v, k);
 }
 
 // Full symmetric H and S and the per-chunk partials of H C and S C.
@@ -488,30 +482,25 @@ int finish_grad(necg_ctx *c, int K, int ntiles, const double *gbar, double *g, h
 
 // ----------------------------------------------------- lower triangle ----
 
-// Lower-triangle tiles (bra tile bx of 64, ket tile by of kt) with
-// by * kt <= 64 bx + 63: per bra tile, the ket tiles up to its last bra.
-long long tri_count(int K, int kt) {
-  const int nbt = (K + 63) / 64, nkt = (K + kt - 1) / kt;
-  long long n = 0;
-  for (int bx = 0; bx < nbt; ++bx) n += std::min(nkt, (64 * bx + 63) / kt + 1);
-  return n;
-}
-
-// Ket tile and term groups of the lower-triangle passes: the ket tile that
-// fills whole rounds of one workgroup per CU best (time ~ rounds x kt), then
-// term groups while there are fewer than two workgroups per CU.
+// Ket tile and term groups of the lower-triangle passes.  One workgroup per
+// CU at a time, its waves taking alternate kets, so a launch takes about
+// (rounds of ncu workgroups) x (kets per wave) ket-term sweeps; the ket tile
+// minimising that is chosen (measured on LiH K = 2048: the model orders nine
+// tile sizes within 2 %), then term groups while there are fewer than two
+// workgroups per CU.
 void choose_tiling2(necg_ctx *c, int K, int &kt, int &ts) {
   const int ncu = std::max(1, c->ncu);
+  const int w = pair_waves(c->n);
   if (c->req_kt > 0) {
     kt = c->req_kt;
   } else {
-    static const int cand[] = {4, 6, 8, 10, 12, 14, 16, 20, 24, 32, 48, 64};
-    double best = 0.0;
-    kt = 16;
+    static const int cand[] = {2, 4, 8, 16, 32, 64};   // divisors of 64 (upper pairs)
+    long long best = -1;
+    kt = 8;
     for (int q : cand) {
       const long long n = tri_count(K, q);
-      const double cost = (double)((n + ncu - 1) / ncu) * (q + 1.0);
-      if (best == 0.0 || cost < best) {
+      const long long cost = ((n + ncu - 1) / ncu) * ((q + w - 1) / w);
+      if (best < 0 || cost < best) {
         best = cost;
         kt = q;
       }
@@ -527,22 +516,14 @@ void choose_tiling2(necg_ctx *c, int K, int &kt, int &ts) {
   }
 }
 
-// The tile list for (K, kt) on the device: full tiles first, then those cut
-// short by the triangle or by K.
+// The tile list for (K, kt) on the device (tri_tiles order).
 int ensure_tiles(necg_ctx *c, int K, int kt) {
   if (c->tiles_K == K && c->tiles_kt == kt) return NECG_OK;
-  const int nbt = (K + 63) / 64, nkt = (K + kt - 1) / kt;
-  std::vector<int> full, part;
-  for (int bx = 0; bx < nbt; ++bx) {
-    const int ny = std::min(nkt, (64 * bx + 63) / kt + 1);
-    for (int by = 0; by < ny; ++by) {
-      const int l0 = by * kt, l1 = std::min(std::min(K, l0 + kt), 64 * bx + 64);
-      auto &dst = (l1 - l0 == kt && 64 * bx + 64 <= K) ? full : part;
-      dst.push_back(bx);
-      dst.push_back(by);
-    }
-  }
-  full.insert(full.end(), part.begin(), part.end());
+  std::vector<int> full;
+  tri_tiles(K, kt, [&](int bx, int by) {
+    full.push_back(bx);
+    full.push_back(by);
+  });
   if (full.size() > c->tiles_cap) {
     if (c->d_tiles) (void)hipFree(c->d_tiles);
     c->d_tiles = nullptr;
@@ -773,98 +754,6 @@ int do_dhds(necg_ctx *c, int K, const double *iv, double *DH, double *DS, hipStr
     case 6: { constexpr int NN = 6; return call; } \
     default: return NECG_ERR_UNSUPPORTED;   \
   }
-
-// The two-sided pair form applies when every term is a particle
-// permutation P (sml[p] a permutation matrix, sms[p] = kron(P^T, I3)) that
-// leaves masses and charges unchanged, and P^T is a term of the same weight.
-// Fills the per-(term, lane) words of target slots: for chunk c, byte c of
-// lane j's word is the slot of the ket's own accumulator record that receives
-// value 16 c + (j >> 2) of the two-sided term's ket-side values (X_H, X_S
-// interleaved per packed (a, b), Y_H, Y_S per (a, x), sigma_H, sigma_S), with
-// X' -> P X' P^T and Y' -> P Y' applied; 255 where there is no value.
-bool two_sided_table(const necg_system *sys, std::vector<unsigned> &tab) {
-  const int n = sys->n, nst = sys->nst, n3 = 3 * n;
-  if (n > 5) return false;
-  auto P = [&](int p, int i, int j) { return sys->sml[((size_t)p * n + i) * n + j]; };
-  std::vector<int> pinv((size_t)nst * n);
-  for (int p = 0; p < nst; ++p) {
-    for (int i = 0; i < n; ++i) {
-      int hits = 0, col = -1;
-      for (int j = 0; j < n; ++j) {
-        const double v = P(p, i, j);
-        if (v != 0.0 && v != 1.0) return false;
-        if (v == 1.0) {
-          ++hits;
-          col = j;
-        }
-      }
-      if (hits != 1) return false;
-      pinv[(size_t)p * n + col] = i;  // P[i][pi(i)] = 1, pinv = pi^-1
-    }
-    for (int j = 0; j < n; ++j) {
-      int hits = 0;
-      for (int i = 0; i < n; ++i) hits += P(p, i, j) == 1.0;
-      if (hits != 1) return false;
-    }
-    for (int r = 0; r < n3; ++r)
-      for (int q = 0; q < n3; ++q) {
-        const double want = (r % 3 == q % 3) ? P(p, q / 3, r / 3) : 0.0;
-        if (sys->sms[((size_t)p * n3 + r) * n3 + q] != want) return false;
-      }
-    // masses and charges unchanged: M[pi(i)][pi(j)] = M[i][j], q[pi(i)] = q[i]
-    for (int i = 0; i < n; ++i) {
-      int pii = -1;
-      for (int j = 0; j < n; ++j)
-        if (P(p, i, j) == 1.0) pii = j;
-      if (sys->ncv[pii + 1] != sys->ncv[i + 1]) return false;
-      for (int j = 0; j < n; ++j) {
-        int pij = -1;
-        for (int m = 0; m < n; ++m)
-          if (P(p, j, m) == 1.0) pij = m;
-        if (sys->mass[pii * n + pij] != sys->mass[i * n + j]) return false;
-      }
-    }
-    // P^T is a term with the same weight
-    bool found = false;
-    for (int q = 0; q < nst && !found; ++q) {
-      bool same = sys->scvv[q] == sys->scvv[p];
-      for (int i = 0; i < n && same; ++i)
-        for (int j = 0; j < n && same; ++j) same = P(q, i, j) == P(p, j, i);
-      found = same;
-    }
-    if (!found) return false;
-  }
-  const int ns = n * (n + 1) / 2;
-  const int a_xh = 0, a_yh = ns, a_xs = ns + n3, a_ys = 2 * ns + n3, a_sh = 2 * ns + 2 * n3;
-  const int nv = a_sh + 2;
-  if (nv > 64) return false;
-  tab.assign((size_t)nst * 64, 0u);
-  for (int p = 0; p < nst; ++p) {
-    const int *iv = &pinv[(size_t)p * n];
-    for (int lane = 0; lane < 64; ++lane) {
-      unsigned word = 0;
-      for (int c = 0; c < 4; ++c) {
-        const int s = 16 * c + (lane >> 2);
-        int t = 255;
-        if (s < 2 * ns) {
-          const int pk = s / 2;
-          int a = 0;
-          while ((a + 1) * (a + 2) / 2 <= pk) ++a;
-          const int b = pk - a * (a + 1) / 2;
-          t = ((s & 1) ? a_xs : a_xh) + sx(iv[a], iv[b]);
-        } else if (s < 2 * ns + 2 * n3) {
-          const int u = (s - 2 * ns) / 2;
-          t = (((s - 2 * ns) & 1) ? a_ys : a_yh) + 3 * iv[u / 3] + u % 3;
-        } else if (s < nv) {
-          t = s - 2 * ns - 2 * n3 + a_sh;
-        }
-        word |= (unsigned)t << (8 * c);
-      }
-      tab[(size_t)p * 64 + lane] = word;
-    }
-  }
-  return true;
-}
 
 struct DeviceGuard {
   int prev = -1;

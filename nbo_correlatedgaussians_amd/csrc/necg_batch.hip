@@ -18,7 +18,9 @@
 #include "../../include/necg.h"
 #include "necg_abi.hpp"
 #include "necg_finish.hpp"
+#include "necg_pair2.hpp"
 #include "necg_pool_impl.hpp"
+#include "necg_two.hpp"
 
 using namespace necg;
 
@@ -37,6 +39,10 @@ struct BatchSys {
   long long vec_off;                 // HC, SC: K
   long long hcp_off;                 // chunk partials: nchunk*K
   long long sml_off, sms_off, scvv_off;
+  // two-sided pair form (lower-triangle items): enabled, ket tile, the
+  // system's target-slot table and its ket partials ((K+63)/64 * ts * K * nacc)
+  int two, kt;
+  long long tperm_off, kpart_off;
 };
 
 struct Item {  // one pair tile
@@ -125,6 +131,50 @@ __global__ void __launch_bounds__(64 * pair_waves(N), 1) b_pair(
   }
 }
 
+// Two-sided persistent pair kernel: the lower-triangle items (tri_tiles) of
+// every system whose terms allow the two-sided form, all particle counts
+// n <= 5 in one launch (items sorted by decreasing cost, so the small
+// systems' items fill the tail).  LDS is sized for n = 5.
+__global__ void __launch_bounds__(256, 1) b_pair2_all(
+    const BatchSys *__restrict__ sys, const Item *__restrict__ items, int nitems,
+    const int32_t *__restrict__ active, const double *__restrict__ bra,
+    const double *__restrict__ ket, const unsigned *__restrict__ tperm,
+    double *__restrict__ part, double *__restrict__ kpart, double *__restrict__ hpart,
+    double *__restrict__ spart, int *__restrict__ flag) {
+  static_assert(pair_waves(5) == 4 && pair_waves(1) == 4, "one workgroup shape for n <= 5");
+  __shared__ double acc_s[PairLds<5, MODE_SPLIT>::ACC];
+  __shared__ double kacc_s[4 * 64];
+  __shared__ double bra_s[BraLds<5>::NF * 64];
+  for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+    const Item w = items[it];
+    if (active && !active[w.sys]) continue;  // uniform over the workgroup
+    const BatchSys &b = sys[w.sys];
+    const int K = b.K;
+    const int z = w.tile % b.ts, bx = w.k0 >> 6;
+    const int up = tri_up(K, b.kt), lup = w.k0 < up ? up : K;
+    const double *br = bra + b.bra_off, *kt = ket + b.ket_off;
+    const unsigned *tp = tperm + b.tperm_off;
+    double *hp = hpart + b.hp_off + (size_t)z * K * K, *sp = spart + b.hp_off + (size_t)z * K * K;
+    double *bp = part + b.part_off + (size_t)w.tile * b.nacc * K;
+    double *kp = kpart + b.kpart_off + (size_t)(bx * b.ts + z) * K * b.nacc;
+    switch (b.n) {
+#define NECG_B2(NN)                                                                          \
+  case NN:                                                                                   \
+    pair2_tile<NN>(br, kt, K, b.nst, w.k0, w.l0, w.l1, lup, w.p0, w.p1, b.sc, tp, bp, kp, hp, \
+                   sp, flag, acc_s, kacc_s, bra_s);                                          \
+    break;
+      NECG_B2(1)
+      NECG_B2(2)
+      NECG_B2(3)
+      NECG_B2(4)
+      NECG_B2(5)
+#undef NECG_B2
+      default: break;
+    }
+    __syncthreads();  // the next item overwrites the LDS tables
+  }
+}
+
 // one workgroup of 256 lanes per (system, row i, 256-column chunk)
 __global__ void __launch_bounds__(256) b_hs_finish(const BatchSys *__restrict__ sys, int nsys,
                                                    const long long *__restrict__ hpre,
@@ -172,7 +222,7 @@ __global__ void __launch_bounds__(1024) b_quotient(const BatchSys *__restrict__ 
 __global__ void b_tile_reduce(const BatchSys *__restrict__ sys, int nsys,
                               const long long *__restrict__ rpre,
                               const int32_t *__restrict__ active, const double *__restrict__ part,
-                              double *__restrict__ red) {
+                              const double *__restrict__ kpart, double *__restrict__ red) {
   const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= rpre[nsys]) return;
   const int s = seg_find(rpre, nsys, idx);
@@ -180,7 +230,11 @@ __global__ void b_tile_reduce(const BatchSys *__restrict__ sys, int nsys,
   const BatchSys &b = sys[s];
   const int r = (int)(idx - rpre[s]);
   const int v = r / b.K, k = r - v * b.K;
-  tile_reduce_one(part + b.part_off, b.ntiles, b.K, b.nacc, v, k, red + b.red_off);
+  if (b.two)
+    red[b.red_off + (size_t)v * b.K + k] = tile_reduce2_one(
+        part + b.part_off, kpart + b.kpart_off, b.K, b.kt, b.ts, b.nacc, v, k);
+  else
+    tile_reduce_one(part + b.part_off, b.ntiles, b.K, b.nacc, v, k, red + b.red_off);
 }
 
 // one lane per (system, function)
@@ -209,6 +263,14 @@ void launch_pair(int grid, hipStream_t st, const BatchSys *sys, const Item *item
                      nitems, active, bra, ket, part, hpart, spart, flag);
 }
 
+void launch_pair2(int grid, hipStream_t st, const BatchSys *sys, const Item *items, int nitems,
+                  const int32_t *active, const double *bra, const double *ket,
+                  const unsigned *tperm, double *part, double *kpart, double *hpart,
+                  double *spart, int *flag) {
+  hipLaunchKernelGGL(b_pair2_all, dim3(grid), dim3(256), 0, st, sys, items, nitems, active, bra,
+                     ket, tperm, part, kpart, hpart, spart, flag);
+}
+
 int hip_rc(hipError_t e) {
   if (e == hipSuccess) return NECG_OK;
   if (e == hipErrorOutOfMemory) return NECG_ERR_OOM;
@@ -225,6 +287,11 @@ int dalloc(T **p, size_t count) {
   return hip_rc(hipMalloc((void **)p, count * sizeof(T)));
 }
 
+// relative cost of one unordered (pair, term) of the two-sided form: it takes
+// about 1.15 times one ordered (pair, term) of the per-ordering form
+// (LiH K = 2048: 7.5 ms for K^2/2 against 13.0 ms for K^2)
+double term_cost2(int n);
+
 // relative cost of one ordered (pair, term): F_E + F_G of the FLOP model
 double term_cost(int n) {
   switch (n) {
@@ -236,6 +303,8 @@ double term_cost(int n) {
     default: return 15000.0;
   }
 }
+
+double term_cost2(int n) { return 1.15 * term_cost(n); }
 
 int dim_bs(int n) { int r = 0; SWITCH_N(n, r = Dim<NN>::BS); return r; }
 int dim_ks(int n) { int r = 0; SWITCH_N(n, r = Dim<NN>::KS); return r; }
@@ -253,11 +322,13 @@ struct necg_batch {
   double *sml = nullptr, *sms = nullptr, *scvv = nullptr;
   double *bra = nullptr, *ket = nullptr, *H = nullptr, *S = nullptr, *hpart = nullptr,
          *spart = nullptr, *part = nullptr, *red = nullptr, *HC = nullptr, *SC = nullptr,
-         *hcp = nullptr, *scp = nullptr, *scal = nullptr;
+         *hcp = nullptr, *scp = nullptr, *scal = nullptr, *kpart = nullptr;
+  unsigned *tperm = nullptr;  // target-slot tables of the two-sided systems
   int *flag = nullptr;
-  // pair work lists per n
+  // pair work lists per (n, form)
   struct Cls {
     int n = 0;
+    bool two = false;
     Item *d_items = nullptr;
     int nitems = 0;
     int grid = 0;
@@ -278,23 +349,33 @@ namespace {
 int plan(necg_batch *B) {
   int rc;
   const double target = B->item_flops > 0 ? B->item_flops : 6.0e7;
+  // lists[n]: the per-ordering items of the systems with n particles;
+  // lists[0]: the two-sided items of all systems that take that form
   std::vector<std::vector<Item>> lists(kMaxN + 1);
   std::vector<std::vector<double>> costs(kMaxN + 1);
-  long long part_tot = 0, hp_tot = 0;
+  long long part_tot = 0, hp_tot = 0, kp_tot = 0;
+  const int w = 4;  // waves per pair workgroup for n <= 5 (pair_waves)
   for (int s = 0; s < B->nsys; ++s) {
     BatchSys &b = B->hsys[s];
     const int K = b.K, nst = b.nst;
     const int nb = std::min(K, 64);
-    const double per_ket = nb * nst * term_cost(b.n);
+    const double tc = b.two ? term_cost2(b.n) : term_cost(b.n);
+    const double per_ket = nb * nst * tc;
     int kt = (int)std::max(1.0, std::floor(target / per_ket));
+    if (b.two) {  // a divisor of 64 (upper pairs), at least one ket per wave
+      int q = w;
+      while (q * 2 <= std::min(kt, 64)) q *= 2;
+      kt = q;
+    }
     kt = std::min(kt, K);
     int tps = nst;
     if (kt == 1) {  // a single ket is already above target: split the terms
-      tps = (int)std::max(1.0, std::ceil(target / (nb * term_cost(b.n))));
+      tps = (int)std::max(1.0, std::ceil(target / (nb * tc)));
       tps = std::min(tps, nst);
     }
     const int ts = (nst + tps - 1) / tps;
     const int kts = (K + kt - 1) / kt;
+    b.kt = kt;
     b.ts = ts;
     b.tps = tps;
     b.ntiles = kts * ts;
@@ -302,39 +383,56 @@ int plan(necg_batch *B) {
     part_tot += (long long)b.ntiles * b.nacc * K;
     b.hp_off = hp_tot;
     hp_tot += (long long)ts * K * K;
-    for (int bt = 0; bt < (K + 63) / 64; ++bt)
-      for (int y = 0; y < kts; ++y)
-        for (int z = 0; z < ts; ++z) {
-          Item it;
-          it.sys = s;
-          it.k0 = bt * 64;
-          it.l0 = y * kt;
-          it.l1 = std::min(K, it.l0 + kt);
-          it.p0 = z * tps;
-          it.p1 = std::min(nst, it.p0 + tps);
-          it.tile = y * ts + z;
-          const double c = (double)std::min(64, K - it.k0) * (it.l1 - it.l0) * (it.p1 - it.p0) *
-                           term_cost(b.n);
-          lists[b.n].push_back(it);
-          costs[b.n].push_back(c);
+    b.kpart_off = kp_tot;
+    if (b.two) kp_tot += (long long)((K + 63) / 64) * ts * K * b.nacc;
+    const int li = b.two ? 0 : b.n;
+    auto add = [&](int bt, int y) {
+      for (int z = 0; z < ts; ++z) {
+        Item it;
+        it.sys = s;
+        it.k0 = bt * 64;
+        it.l0 = y * kt;
+        it.l1 = std::min(K, it.l0 + kt);
+        if (b.two) {
+          const int up = tri_up(K, kt);
+          if (it.l0 < (it.k0 < up ? up : K)) it.l1 = std::min(it.l1, it.k0 + 64);
         }
+        it.p0 = z * tps;
+        it.p1 = std::min(nst, it.p0 + tps);
+        it.tile = y * ts + z;
+        const double c = (double)std::min(64, K - it.k0) * (it.l1 - it.l0) * (it.p1 - it.p0) *
+                         tc;
+        lists[li].push_back(it);
+        costs[li].push_back(c);
+      }
+    };
+    if (b.two) {
+      tri_tiles(K, kt, add);
+    } else {
+      for (int bt = 0; bt < (K + 63) / 64; ++bt)
+        for (int y = 0; y < kts; ++y) add(bt, y);
+    }
   }
   if ((rc = dalloc(&B->part, (size_t)part_tot))) return rc;
   if ((rc = dalloc(&B->hpart, (size_t)hp_tot))) return rc;
   if ((rc = dalloc(&B->spart, (size_t)hp_tot))) return rc;
+  if ((rc = dalloc(&B->kpart, (size_t)kp_tot))) return rc;
   for (auto &c : B->cls)
     if (c.d_items) (void)hipFree(c.d_items);
   B->cls.clear();
-  for (int n = 1; n <= kMaxN; ++n) {
-    if (lists[n].empty()) continue;
-    std::vector<size_t> ord(lists[n].size());
+  for (int li = 0; li <= kMaxN; ++li) {
+    if (lists[li].empty()) continue;
+    const int n = li;
+    const bool two = li == 0;
+    std::vector<size_t> ord(lists[li].size());
     std::iota(ord.begin(), ord.end(), 0);
     std::stable_sort(ord.begin(), ord.end(),
-                     [&](size_t a, size_t b) { return costs[n][a] > costs[n][b]; });
+                     [&](size_t a, size_t b) { return costs[li][a] > costs[li][b]; });
     std::vector<Item> sorted;
-    for (size_t i : ord) sorted.push_back(lists[n][i]);
+    for (size_t i : ord) sorted.push_back(lists[li][i]);
     necg_batch::Cls c;
     c.n = n;
+    c.two = two;
     c.nitems = (int)sorted.size();
     if ((rc = dalloc(&c.d_items, sorted.size()))) return rc;
     if ((rc = hip_rc(hipMemcpy(c.d_items, sorted.data(), sorted.size() * sizeof(Item),
@@ -344,8 +442,13 @@ int plan(necg_batch *B) {
     if (per_cu <= 0) {
       int occ = 1;
       const void *fn = nullptr;
-      SWITCH_N(n, fn = (const void *)&b_pair<NN>);
-      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, 64 * pair_waves(n), 0) !=
+      if (two) {
+        fn = (const void *)&b_pair2_all;
+      } else {
+        SWITCH_N(n, fn = (const void *)&b_pair<NN>);
+      }
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, two ? 256 : 64 * pair_waves(n),
+                                                       0) !=
               hipSuccess ||
           occ < 1)
         occ = 1;
@@ -386,7 +489,7 @@ int necg_batch_destroy(necg_batch *B) {
   {
     DevGuard g(B->device);
     double *bufs[] = {B->sml, B->sms, B->scvv, B->bra, B->ket, B->H, B->S, B->hpart, B->spart,
-                      B->part, B->red, B->HC, B->SC, B->hcp, B->scp, B->scal};
+                      B->part, B->red, B->HC, B->SC, B->hcp, B->scp, B->scal, B->kpart};
     for (double *p : bufs)
       if (p) (void)hipFree(p);
     long long *pre[] = {B->d_fpre, B->d_kpre, B->d_hpre, B->d_rpre};
@@ -394,6 +497,7 @@ int necg_batch_destroy(necg_batch *B) {
       if (p) (void)hipFree(p);
     if (B->d_sys) (void)hipFree(B->d_sys);
     if (B->flag) (void)hipFree(B->flag);
+    if (B->tperm) (void)hipFree(B->tperm);
     for (auto &c : B->cls)
       if (c.d_items) (void)hipFree(c.d_items);
   }
@@ -430,6 +534,7 @@ int necg_batch_create(const necg_system *systems, int32_t nsys, const int32_t *K
   B->ncu = prop.multiProcessorCount;
   // host-side system records and offsets
   std::vector<double> h_sml, h_sms, h_scvv;
+  std::vector<unsigned> h_tperm;
   long long xo = 0, bo = 0, ko = 0, hso = 0, ro = 0, vo = 0, hco = 0;
   B->fpre.assign(1, 0);
   B->kpre.assign(1, 0);
@@ -474,6 +579,10 @@ int necg_batch_create(const necg_system *systems, int32_t nsys, const int32_t *K
     h_sms.insert(h_sms.end(), y.sms, y.sms + (size_t)y.nst * 9 * n * n);
     b.scvv_off = (long long)h_scvv.size();
     h_scvv.insert(h_scvv.end(), y.scvv, y.scvv + y.nst);
+    std::vector<unsigned> tab;
+    b.two = two_sided_table(&y, tab) ? 1 : 0;
+    b.tperm_off = (long long)h_tperm.size();
+    h_tperm.insert(h_tperm.end(), tab.begin(), tab.end());
     B->hsys.push_back(b);
     B->fpre.push_back(B->fpre.back() + K);
     B->kpre.push_back(B->kpre.back() + (long long)K * y.nst);
@@ -499,7 +608,18 @@ int necg_batch_create(const necg_system *systems, int32_t nsys, const int32_t *K
       (rc = hip_rc(hipMemcpy(B->sms, h_sms.data(), h_sms.size() * 8, hipMemcpyHostToDevice))) ||
       (rc = hip_rc(hipMemcpy(B->scvv, h_scvv.data(), h_scvv.size() * 8,
                              hipMemcpyHostToDevice))) ||
-      (rc = hip_rc(hipMemset(B->flag, 0, sizeof(int)))) || (rc = plan(B))) {
+      (rc = hip_rc(hipMemset(B->flag, 0, sizeof(int))))) {
+    necg_batch_destroy(B);
+    return rc;
+  }
+  if (!h_tperm.empty() &&
+      ((rc = hip_rc(hipMalloc((void **)&B->tperm, h_tperm.size() * sizeof(unsigned)))) ||
+       (rc = hip_rc(hipMemcpy(B->tperm, h_tperm.data(), h_tperm.size() * sizeof(unsigned),
+                              hipMemcpyHostToDevice))))) {
+    necg_batch_destroy(B);
+    return rc;
+  }
+  if ((rc = plan(B))) {
     necg_batch_destroy(B);
     return rc;
   }
@@ -538,15 +658,20 @@ int necg_batch_outprod_masked(necg_batch *B, const int32_t *active, const double
   hipLaunchKernelGGL(b_prep_ket, dim3(blocks(B->kpre[n], tb)), dim3(tb), 0, st, B->d_sys, n,
                      B->d_kpre, active, B->sml, B->sms, B->scvv, B->bra, B->ket);
   for (const auto &c : B->cls) {
-    SWITCH_N(c.n, launch_pair<NN>(c.grid, st, B->d_sys, c.d_items, c.nitems, active, B->bra,
-                                  B->ket, B->part, B->hpart, B->spart, B->flag));
+    if (c.two) {
+      launch_pair2(c.grid, st, B->d_sys, c.d_items, c.nitems, active, B->bra, B->ket, B->tperm,
+                   B->part, B->kpart, B->hpart, B->spart, B->flag);
+    } else {
+      SWITCH_N(c.n, launch_pair<NN>(c.grid, st, B->d_sys, c.d_items, c.nitems, active, B->bra,
+                                    B->ket, B->part, B->hpart, B->spart, B->flag));
+    }
   }
   hipLaunchKernelGGL(b_hs_finish, dim3((unsigned)B->hpre[n]), dim3(256), 0, st, B->d_sys, n,
                      B->d_hpre, active, B->hpart, B->spart, B->bra, B->H, B->S, B->hcp, B->scp);
   hipLaunchKernelGGL(b_quotient, dim3(n), dim3(1024), 0, st, B->d_sys, active, B->hcp, B->scp,
                      B->bra, B->HC, B->SC, B->scal, f, B->flag);
   hipLaunchKernelGGL(b_tile_reduce, dim3(blocks(B->rpre[n], tb)), dim3(tb), 0, st, B->d_sys, n,
-                     B->d_rpre, active, B->part, B->red);
+                     B->d_rpre, active, B->part, B->kpart, B->red);
   hipLaunchKernelGGL(b_grad_final, dim3(blocks(B->fpre[n], tb)), dim3(tb), 0, st, B->d_sys, n,
                      B->d_fpre, active, B->red, B->bra, B->scal, B->HC, B->SC, g);
   return hip_rc(hipGetLastError());

@@ -498,14 +498,17 @@ struct PairLds {
 // ([slot][lane]) and takes alternate kets; ket records are wave-uniform
 // (scalar loads).
 // Writes the element partials of bra >= ket to hp/sp ([l][k], K x K) and the
-// tile's summed accumulators to pt ([slot][k], NACC x K).
+// tile's summed accumulators to pt ([slot][k], NACC x K).  Kets l >= lup
+// (element modes of the lower-triangle pass: the last, partial bra tile's
+// functions) pair with every bra of the tile, written at [k][l].
 template <int N, int MODE>
 __device__ __forceinline__ void pair_tile(const double *__restrict__ bra,
                                           const double *__restrict__ ket, int K, int nst, int k0,
                                           int l0, int l1, int p0, int p1, const SysConst &sc,
                                           double eshift, double *__restrict__ pt,
                                           double *__restrict__ hp, double *__restrict__ sp,
-                                          int *__restrict__ flag, double *acc_s) {
+                                          int *__restrict__ flag, double *acc_s,
+                                          int lup = 0x7fffffff) {
   using D = Dim<N>;
   constexpr int W = pair_waves(N);
   constexpr int NA = PairLds<N, MODE>::NA;
@@ -532,8 +535,8 @@ __device__ __forceinline__ void pair_tile(const double *__restrict__ bra,
     const double *kl = ket + (size_t)l * nst * D::KS;
     for (int p = p0; p < p1; ++p)
       ok &= pair_term<N, MODE>(b, kl + (size_t)p * D::KS, sc, eshift, hel, sel, sink, nullptr);
-    if (MODE != MODE_COMBINED && active && k >= l) {
-      const size_t o = (size_t)l * K + k;
+    if (MODE != MODE_COMBINED && active && (k >= l || l >= lup)) {
+      const size_t o = (l >= lup && k < l) ? (size_t)k * K + l : (size_t)l * K + k;
       hp[o] = hel;
       sp[o] = sel;
     }

@@ -74,8 +74,10 @@ __device__ __forceinline__ void fill_bra_lds(double *dst, const double *__restri
 }
 
 // Compiler barrier between the phases of a term: memory operations are not
-// moved across it, so each phase reads the bra fields it needs right where
-// it needs them (no instruction is emitted).
+// moved across it, so the gradient halves read the bra fields they need right
+// where they need them (no instruction is emitted).  Two barriers, before the
+// shared gradient pieces and before the ket side, measured fastest among the
+// placements tried (LiH K = 2048: 7.52 ms against 7.74 ms with one per phase).
 __device__ __forceinline__ void phase_barrier() { asm volatile("" ::: "memory"); }
 
 // F0 and dF0/dz as boys_f0 (python/utils.py:78-106), written so that the
@@ -118,6 +120,14 @@ __device__ __forceinline__ void boys_f0_line(double z, double &f, double &fp) {
   asm volatile("" : "+v"(fb));   // formed on every lane: no branch around it
   f = big ? fa : PQ * v;
   fp = big ? -0.5 * fa * (v * v) : fb;
+}
+
+// First ket of the upper pairs of a lower-triangle pass: when K is not a
+// multiple of 64 and kt divides 64, the functions of the last bra tile
+// P = (K-1)/64 are taken as kets of the full bra tiles (pairs k < l); else K
+// (no upper pairs).
+__host__ __device__ inline int tri_up(int K, int kt) {
+  return ((K & 63) != 0 && kt > 0 && (64 % kt) == 0) ? 64 * ((K - 1) / 64) : K;
 }
 
 // Ket-side values per evaluation: X_H, X_S interleaved per packed (i, j),
@@ -260,7 +270,6 @@ __device__ __forceinline__ bool pair_term2(const BraLds<N> &b, const double *__r
   double det32 = 1.0;
 #pragma unroll
   for (int j = 0; j < N; ++j) det32 *= rd[j] * rd[j] * rd[j];
-  phase_barrier();
 
   // E, st = B^-1 E, i3 (python/matel.py:125-133)
   double st[N3];
@@ -281,7 +290,6 @@ __device__ __forceinline__ bool pair_term2(const BraLds<N> &b, const double *__r
       }
   }
   const double ov = b.f(BraLds<N>::F_PK) * pl * det32 * exp(i3 - b.f(BraLds<N>::F_I1) - i2);
-  phase_barrier();
 
   // Kinetic (python/matel.py:144-155): a1 = A dif1, Ma1 = M a1,
   // <A dif1, M A' dif2> = -<a1, M a1>; 3 tr(M A B^-1 A') = 3 (tr(MA) - <AMA, B^-1>).
@@ -317,7 +325,6 @@ __device__ __forceinline__ bool pair_term2(const BraLds<N> &b, const double *__r
     for (int j = 0; j <= i; ++j)
       ab = __builtin_fma((i == j ? 1.0 : 2.0) * b.AMA(sx(i, j)), Bi[sx(i, j)], ab);
   const double tfac = 2.0 * i3k + 3.0 * (b.f(BraLds<N>::F_TRMA) - ab);
-  phase_barrier();
 
   // Coulomb (python/matel.py:162-197; gradients python/gradel.py:266-337).
   double csum = 0.0;
@@ -411,7 +418,6 @@ __device__ __forceinline__ bool pair_term2(const BraLds<N> &b, const double *__r
       }
   }
 
-  phase_barrier();
   // Bra side: Phi = Phic + 8 (M a1 - w), G = B^-1 (12 A'MA' + La) B^-1.
   {
     double Phi[N3], d1[N3];
@@ -524,18 +530,19 @@ __device__ __forceinline__ bool pair_term2(const BraLds<N> &b, const double *__r
   return ok;
 }
 
-// One two-sided pair tile: bras [k0, k0+64) x kets [l0, l1) x terms [p0, p1)
-// with l1 <= k0 + 64, run by the pair_waves(N) waves of a workgroup (each
-// takes alternate kets).  Only pairs with bra k >= ket l contribute: the
-// bra side of every such pair and the ket side of those with k > l.
-// Writes the element partials of k >= l to hp/sp ([l][k]), the tile's bra
-// accumulators to bpt ([slot][k], NACC x K) and, per ket, its accumulator
-// record summed over the tile's bras to kpt ([l][slot], K x NACC).
+// One two-sided pair tile: bras [k0, k0+64) x kets [l0, l1) x terms [p0, p1),
+// run by the pair_waves(N) waves of a workgroup (each takes alternate kets).
+// A pair contributes when bra k >= ket l, or when l >= lup (the upper pairs
+// of the last, partial bra tile's functions): the bra side of every such
+// pair and the ket side of those with k != l.  Writes the element partials
+// to hp/sp at [min(k, l)][max(k, l)], the tile's bra accumulators to bpt
+// ([slot][k], NACC x K) and, per ket, its accumulator record summed over the
+// tile's bras to kpt ([l][slot], K x NACC).
 template <int N>
 __device__ __forceinline__ void pair2_tile(const double *__restrict__ bra,
                                            const double *__restrict__ ket, int K, int nst, int k0,
-                                           int l0, int l1, int p0, int p1, const SysConst &sc,
-                                           const unsigned *__restrict__ tperm,
+                                           int l0, int l1, int lup, int p0, int p1,
+                                           const SysConst &sc, const unsigned *__restrict__ tperm,
                                            double *__restrict__ bpt, double *__restrict__ kpt,
                                            double *__restrict__ hp, double *__restrict__ sp,
                                            int *__restrict__ flag, double *acc_s,
@@ -557,7 +564,8 @@ __device__ __forceinline__ void pair2_tile(const double *__restrict__ bra,
   LdsSink sink{acc_s + wv * NA + lane};
   bool ok = true;
   for (int l = l0 + wv; l < l1; l += W) {
-    const bool valid = active && k >= l;
+    const bool upper = l >= lup;
+    const bool valid = active && (k >= l || upper);
     const double mb = valid ? 1.0 : 0.0;
     const double mk = (valid && k != l) ? 1.0 : 0.0;
     double hel = 0.0, sel = 0.0;
@@ -566,7 +574,7 @@ __device__ __forceinline__ void pair2_tile(const double *__restrict__ bra,
       ok &= pair_term2<N>(b, kl + (size_t)p * D::KS, sc, mb, mk, tperm[p * 64 + lane], hel, sel,
                           sink, kacc, lane);
     if (valid) {
-      const size_t o = (size_t)l * K + k;
+      const size_t o = upper ? (size_t)k * K + l : (size_t)l * K + k;
       hp[o] = hel;
       sp[o] = sel;
     }
@@ -590,6 +598,48 @@ __device__ __forceinline__ void pair2_tile(const double *__restrict__ bra,
     for (int u = 0; u < W; ++u) t += acc_s[u * NA + v * 64 + lane];
     o[(size_t)v * K] = t;
   }
+}
+
+// The partials that function k sums in a lower-triangle pass (tri_tiles):
+// as a bra, those of ket tiles [yb0, yb1) and [yu0, yu1) of its bra tile; as
+// a ket, those of bra tiles [x0, x1).
+struct TriRanges {
+  int yb0, yb1, yu0, yu1, x0, x1;
+};
+
+__host__ __device__ inline TriRanges tri_ranges(int K, int kt, int k) {
+  const int bx = k >> 6;
+  const int nkt = (K + kt - 1) / kt, nbt = (K + 63) / 64;
+  const int up = tri_up(K, kt);
+  TriRanges r;
+  r.yb0 = (64 * bx >= up) ? up / kt : 0;   // tile P alone: its own block
+  r.yb1 = (64 * bx + 63) / kt + 1 < nkt ? (64 * bx + 63) / kt + 1 : nkt;
+  r.yu0 = r.yu1 = 0;
+  if (64 * bx < up && up < K) {             // upper pairs with tile P's functions
+    r.yu0 = up / kt;
+    r.yu1 = nkt;
+  }
+  r.x0 = bx;
+  r.x1 = nbt;
+  if (up < K) {
+    if (k < up) r.x1 = nbt - 1;            // tile P takes no kets below its block
+    else r.x0 = 0;                         // every full tile takes tile P's functions
+  }
+  return r;
+}
+
+// Sum of the two-sided partials of function k, slot v: bra partials at
+// bpart[((y * ts + z) * nacc + v) * K + k], ket partials at
+// kpart[((x * ts + z) * K + k) * nacc + v], over tri_ranges.  Fixed order.
+__device__ __forceinline__ double tile_reduce2_one(const double *__restrict__ bpart,
+                                                   const double *__restrict__ kpart, int K,
+                                                   int kt, int ts, int nacc, int v, int k) {
+  const TriRanges r = tri_ranges(K, kt, k);
+  double s = 0.0;
+  for (int t = r.yb0 * ts; t < r.yb1 * ts; ++t) s += bpart[((size_t)t * nacc + v) * K + k];
+  for (int t = r.yu0 * ts; t < r.yu1 * ts; ++t) s += bpart[((size_t)t * nacc + v) * K + k];
+  for (int t = r.x0 * ts; t < r.x1 * ts; ++t) s += kpart[((size_t)t * K + k) * nacc + v];
+  return s;
 }
 
 }  // namespace necg

@@ -1,0 +1,143 @@
+// necg_two.hpp -- host-side setup of the two-sided pair form (shared by the
+// single-system context and the batch): whether a system's terms allow it,
+// and the per-(term, lane) target-slot words of the ket-side sums.
+#pragma once
+
+#include <algorithm>
+#include <vector>
+
+#include "../../include/necg.h"
+#include "necg_pair2.hpp"
+
+namespace necg {
+
+// The two-sided pair form applies when every term is a particle
+// permutation P (sml[p] a permutation matrix, sms[p] = kron(P^T, I3)) that
+// leaves masses and charges unchanged, and P^T is a term of the same weight.
+// Fills the per-(term, lane) words of target slots: for chunk c, byte c of
+// lane j's word is the slot of the ket's own accumulator record that receives
+// value 16 c + (j >> 2) of the two-sided term's ket-side values (X_H, X_S
+// interleaved per packed (a, b), Y_H, Y_S per (a, x), sigma_H, sigma_S), with
+// X' -> P X' P^T and Y' -> P Y' applied; 255 where there is no value.
+inline bool two_sided_table(const necg_system *sys, std::vector<unsigned> &tab) {
+  const int n = sys->n, nst = sys->nst, n3 = 3 * n;
+  if (n > 5) return false;
+  auto P = [&](int p, int i, int j) { return sys->sml[((size_t)p * n + i) * n + j]; };
+  std::vector<int> pinv((size_t)nst * n);
+  for (int p = 0; p < nst; ++p) {
+    for (int i = 0; i < n; ++i) {
+      int hits = 0, col = -1;
+      for (int j = 0; j < n; ++j) {
+        const double v = P(p, i, j);
+        if (v != 0.0 && v != 1.0) return false;
+        if (v == 1.0) {
+          ++hits;
+          col = j;
+        }
+      }
+      if (hits != 1) return false;
+      pinv[(size_t)p * n + col] = i;  // P[i][pi(i)] = 1, pinv = pi^-1
+    }
+    for (int j = 0; j < n; ++j) {
+      int hits = 0;
+      for (int i = 0; i < n; ++i) hits += P(p, i, j) == 1.0;
+      if (hits != 1) return false;
+    }
+    for (int r = 0; r < n3; ++r)
+      for (int q = 0; q < n3; ++q) {
+        const double want = (r % 3 == q % 3) ? P(p, q / 3, r / 3) : 0.0;
+        if (sys->sms[((size_t)p * n3 + r) * n3 + q] != want) return false;
+      }
+    // masses and charges unchanged: M[pi(i)][pi(j)] = M[i][j], q[pi(i)] = q[i]
+    for (int i = 0; i < n; ++i) {
+      int pii = -1;
+      for (int j = 0; j < n; ++j)
+        if (P(p, i, j) == 1.0) pii = j;
+      if (sys->ncv[pii + 1] != sys->ncv[i + 1]) return false;
+      for (int j = 0; j < n; ++j) {
+        int pij = -1;
+        for (int m = 0; m < n; ++m)
+          if (P(p, j, m) == 1.0) pij = m;
+        if (sys->mass[pii * n + pij] != sys->mass[i * n + j]) return false;
+      }
+    }
+    // P^T is a term with the same weight
+    bool found = false;
+    for (int q = 0; q < nst && !found; ++q) {
+      bool same = sys->scvv[q] == sys->scvv[p];
+      for (int i = 0; i < n && same; ++i)
+        for (int j = 0; j < n && same; ++j) same = P(q, i, j) == P(p, j, i);
+      found = same;
+    }
+    if (!found) return false;
+  }
+  const int ns = n * (n + 1) / 2;
+  const int a_xh = 0, a_yh = ns, a_xs = ns + n3, a_ys = 2 * ns + n3, a_sh = 2 * ns + 2 * n3;
+  const int nv = a_sh + 2;
+  if (nv > 64) return false;
+  tab.assign((size_t)nst * 64, 0u);
+  for (int p = 0; p < nst; ++p) {
+    const int *iv = &pinv[(size_t)p * n];
+    for (int lane = 0; lane < 64; ++lane) {
+      unsigned word = 0;
+      for (int c = 0; c < 4; ++c) {
+        const int s = 16 * c + (lane >> 2);
+        int t = 255;
+        if (s < 2 * ns) {
+          const int pk = s / 2;
+          int a = 0;
+          while ((a + 1) * (a + 2) / 2 <= pk) ++a;
+          const int b = pk - a * (a + 1) / 2;
+          t = ((s & 1) ? a_xs : a_xh) + sx(iv[a], iv[b]);
+        } else if (s < 2 * ns + 2 * n3) {
+          const int u = (s - 2 * ns) / 2;
+          t = (((s - 2 * ns) & 1) ? a_ys : a_yh) + 3 * iv[u / 3] + u % 3;
+        } else if (s < nv) {
+          t = s - 2 * ns - 2 * n3 + a_sh;
+        }
+        word |= (unsigned)t << (8 * c);
+      }
+      tab[(size_t)p * 64 + lane] = word;
+    }
+  }
+  return true;
+}
+
+// Lower-triangle tiles of the two-sided and element-only passes, in launch
+// order (full tiles first): emit(bx, by) for bra tile bx (64 functions) and
+// ket tile by (kt functions).  Bra tile bx covers its kets l <= its last
+// function.  When K is not a multiple of 64 and kt divides 64, the functions
+// of the last, partial bra tile P are paired with the full tiles' lanes
+// instead (upper pairs, kets >= tri_up(K, kt)), and tile P covers only its
+// own diagonal block: fewer idle lanes.
+template <class Emit>
+void tri_tiles(int K, int kt, Emit emit) {
+  const int nbt = (K + 63) / 64, nkt = (K + kt - 1) / kt;
+  const int up = tri_up(K, kt);
+  std::vector<int> part;
+  auto add = [&](int bx, int by) {
+    const int l0 = by * kt, k0 = 64 * bx;
+    const int lup = k0 < up ? up : K;
+    int l1 = std::min(K, l0 + kt);
+    if (l0 < lup) l1 = std::min(l1, k0 + 64);
+    if (l1 - l0 == kt && std::min(64, K - k0) == 64) emit(bx, by);
+    else { part.push_back(bx); part.push_back(by); }
+  };
+  for (int bx = 0; bx < nbt; ++bx) {
+    const int k0 = 64 * bx;
+    const int lo = (k0 >= up) ? up / kt : 0;           // tile P alone: its own block
+    const int nyl = std::min(nkt, (k0 + 63) / kt + 1);
+    for (int by = lo; by < nyl; ++by) add(bx, by);
+    if (k0 < up && up < K)                             // upper pairs with tile P's functions
+      for (int by = up / kt; by < nkt; ++by) add(bx, by);
+  }
+  for (size_t i = 0; i < part.size(); i += 2) emit(part[i], part[i + 1]);
+}
+
+inline long long tri_count(int K, int kt) {
+  long long n = 0;
+  tri_tiles(K, kt, [&](int, int) { ++n; });
+  return n;
+}
+
+}  // namespace necg

@@ -1,21 +1,79 @@
-"""Algorithmic work model of one energy+gradient evaluation.
+"""Algorithmic work of one energy+gradient evaluation.
 
-Counting rules: FMA = 2 FLOP; add, mul, div, sqrt, exp = 1.  Per ordered
-(pair, term) in the n-space form, the elements cost F_E(n) and one bra-side
-gradient F_G(n); per unordered pair: nst (F_E + 2 F_G) off the diagonal and
-nst (F_E + F_G) on it.  The table is the fixed model the throughput is
-reported against (LiH: 413,856 FLOP per off-diagonal pair); the kernels may
-execute fewer operations than the model counts."""
+Counting rules: FMA = 2 FLOP; add, subtract, multiply, divide, sqrt, exp = 1;
+a reciprocal square root is a sqrt and a divide (2).  Newton refinements of
+hardware seeds, selects and moves are not counted.
+
+Two models:
+
+* ``flops_per_eval_v0``: the reference's formulas written in n-space, per
+  ordered (pair, term) F_E(n) for the elements and F_G(n) for one bra-side
+  gradient; per unordered pair nst (F_E + 2 F_G) off the diagonal and
+  nst (F_E + F_G) on it (LiH: 413,856 FLOP per off-diagonal pair).
+* ``flops_per_eval``: the two-sided algorithm the pair kernel runs: per
+  unordered (pair, term) one factorisation, the elements, the Coulomb and
+  Stark derivatives, and both gradient halves (``two_sided_term``; LiH:
+  4,760 FLOP per pair-term, 114,240 per off-diagonal pair).  The identities
+  it uses (A' dif2 = -A dif1, B^-1 A' = I - B^-1 A, the collapsed kinetic and
+  Coulomb gradient products) make it 3.6x smaller than the v0 model for LiH,
+  so the v0 model overstates the work of the run and is not the roofline
+  basis.
+"""
 
 F_E = {2: 346, 3: 733, 5: 2100}
 F_G = {2: 975, 3: 2272, 5: 7572}
 
 
-def flops_per_eval(n: int, nst: int, K: int) -> float:
+def flops_per_eval_v0(n: int, nst: int, K: int) -> float:
     if n not in F_E:
-        raise KeyError(f"no FLOP model for n = {n}")
+        raise KeyError(f"no v0 FLOP model for n = {n}")
     off = K * (K - 1) // 2
     return float(nst * (off * (F_E[n] + 2 * F_G[n]) + K * (F_E[n] + F_G[n])))
+
+
+def two_sided_term(n: int) -> dict:
+    """FLOP of one unordered (bra, ket, term) evaluation of the two-sided
+    pair term, by phase, for n pseudo-particles (ns = n(n+1)/2 packed
+    entries, n3 = 3n shift entries, n(n+1)/2 Coulomb pairs of which n involve
+    the reference particle).  'ket' is the ket-side gradient, which the
+    diagonal pairs (k = l) do not need."""
+    ns, n3 = n * (n + 1) // 2, 3 * n
+    chol = ns                                            # B = A + A'
+    for j in range(n):                                   # Cholesky, 1/R_jj by rsq
+        chol += 2 * j + 2 + 1 + (n - 1 - j) * (2 * j + 1)
+    inv = sum(2 * (i - j) + 1 for j in range(n) for i in range(j + 1, n))   # R^-1
+    inv += sum((i + 1) * 2 * (n - i) for i in range(n))                     # B^-1 = R^-T R^-1
+    det = 3 * n                                          # det(B)^-3/2 from the pivots
+    elem = n3 + 2 * n3 * n + 2 * n3 + 6                  # E, st = B^-1 E, i3, ov
+    kin = n3 + 2 * n3 * n + 2 * n3 * n + 2 * n3          # dif1, a1 = A dif1, M a1, i3k
+    kin += 2 * ns + (ns - n) + 4                         # <AMA, B^-1>, tfac
+    ref_pair = 2 + 1 + 6 + 55 + 4 + 3 + 5 + 2            # (0, q): rsq, z, Boys F0/F0', sums
+    pp_pair = ref_pair + 3 + 3 + 4                       # (p, q): i2, dif, 6 rank-1 updates
+    coul = n * ref_pair + (ns - n) * pp_pair
+    stark = 2 * n + 5 + 3 + 2                            # zsum, alpha, H and S sums
+    shared = 2 * n3 * n + 2 * n + 3 * 2 * n3 * n         # Lb st, Stark column, A M a1, B^-1 F, B^-1 v
+    side = (n3 + 2 * n3                                  # dif, Phi from the shared parts
+            + 2 * ns + 2 * n * n * n                     # 12 A'MA' + La, times B^-1
+            + ns * (2 * n + 6 + 12 + 3 + 4 + 1)          # G, dd, dp, x0, X_H, X_S
+            + 3 + n3 * 6                                 # weights, Y_H, Y_S
+            + (2 * ns + 2 * n3 + 2))                     # accumulation of the values
+    return {"factorisation": chol + inv + det, "elements": elem, "kinetic": kin,
+            "coulomb": coul, "stark": stark, "shared_gradient": shared,
+            "bra": side + n3,                            # Phi = Phic + 8 (M a1 - w)
+            "ket": side}
+
+
+def two_sided_term_flops(n: int, diagonal: bool = False) -> int:
+    ph = two_sided_term(n)
+    return sum(ph.values()) - (ph["ket"] if diagonal else 0)
+
+
+def flops_per_eval(n: int, nst: int, K: int) -> float:
+    """FLOP of one energy+gradient evaluation by the two-sided algorithm:
+    nst terms for each of the K(K-1)/2 off-diagonal pairs (both gradient
+    halves) and the K diagonal pairs (bra half only)."""
+    off = K * (K - 1) // 2
+    return float(nst * (off * two_sided_term_flops(n) + K * two_sided_term_flops(n, True)))
 
 
 def pairs_per_eval(K: int) -> int:

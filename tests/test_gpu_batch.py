@@ -16,6 +16,19 @@ def _single(sysd, x):
     return Engine(sysd, "cuda:0").outprod(x)
 
 
+def _lih_without_an_inverse():
+    """LiH less one term that is not its own inverse: the two-sided pair form
+    does not apply, so a batch holding it runs both forms for n = 5."""
+    s = necg.lih_system()
+    p = next(q for q in range(s.nst) if not np.array_equal(s.sml[q], s.sml[q].T))
+    keep = [q for q in range(s.nst) if q != p]
+    t = necg.System(n=s.n, nst=len(keep), efs=s.efs, mass=s.mass.copy(), ncv=s.ncv.copy(),
+                    scvv_raw=s.scvv_raw[keep].copy(), sml=s.sml[keep].copy(),
+                    sms=s.sms[keep].copy(), name="LiH-23")
+    t.validate()
+    return t
+
+
 def _mixed_small():
     cases = []
     for name in ("lih_k12_s0", "he_k8_s3", "h2_k8_s4"):
@@ -26,6 +39,9 @@ def _mixed_small():
         s = system_by_name(name)
         iv, C = necg.generate_parameters(s, K, seed)
         cases.append((s, necg.pack_x(iv, C), None))
+    s = _lih_without_an_inverse()
+    iv, C = necg.generate_parameters(s, 90, 17)
+    cases.append((s, necg.pack_x(iv, C), None))
     return cases
 
 
