@@ -137,15 +137,14 @@ __global__ void __launch_bounds__(64 * pair_waves(N), NECG_MIN_WAVES) pair_tri_k
                      hpart + (size_t)z * K * K, spart + (size_t)z * K * K, flag, acc_s, lup);
 }
 
-// Two-sided partial sums per (function, slot).  Grid (ceil(K/256), nv), 256 lanes.
+// Two-sided partial sums per (function, slot).  Grid (ceil(K/64), nv),
+// 256 lanes: 64 functions per workgroup, four waves over the partials.
 __global__ void __launch_bounds__(256) tile_reduce2(const double *__restrict__ bpart,
                                                     const double *__restrict__ kpart, int K,
                                                     int kt, int ts, int nacc,
                                                     double *__restrict__ red) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  const int v = blockIdx.y;
-  if (k >= K) return;
-  red[(size_t)v * K + k] = tile_reduce2_one(bpart, kpart, K, kt, ts, nacc, v, k);
+  __shared__ double sh[256];
+  tile_reduce2_block(bpart, kpart, K, kt, ts, nacc, blockIdx.y, blockIdx.x * 64, red, sh);
 }
 
 // Full symmetric H and S and the per-chunk partials of H C and S C.
@@ -615,7 +614,7 @@ template <int N>
 int finish_grad2(necg_ctx *c, int K, int kt, int ts, const double *gbar, double *g,
                  hipStream_t st) {
   using D = Dim<N>;
-  hipLaunchKernelGGL(tile_reduce2, dim3((K + 255) / 256, D::NACC), dim3(256), 0, st, c->part,
+  hipLaunchKernelGGL(tile_reduce2, dim3((K + 63) / 64, D::NACC), dim3(256), 0, st, c->part,
                      c->kpart, K, kt, ts, D::NACC, c->red);
   hipLaunchKernelGGL((grad_final<N, MODE_SPLIT>), dim3((K + 255) / 256), dim3(256), 0, st, c->red,
                      K, c->bra, c->scal, c->HC, c->SC, gbar, g);

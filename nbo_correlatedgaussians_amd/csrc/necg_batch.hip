@@ -218,23 +218,29 @@ __global__ void __launch_bounds__(1024) b_quotient(const BatchSys *__restrict__ 
                  rh, rs);
 }
 
-// one lane per (system, slot, function)
-__global__ void b_tile_reduce(const BatchSys *__restrict__ sys, int nsys,
-                              const long long *__restrict__ rpre,
-                              const int32_t *__restrict__ active, const double *__restrict__ part,
-                              const double *__restrict__ kpart, double *__restrict__ red) {
-  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= rpre[nsys]) return;
-  const int s = seg_find(rpre, nsys, idx);
-  if (active && !active[s]) return;
+// one 256-lane workgroup per (system, slot, 64 functions): the two-sided
+// reduction over four waves, or (per-ordering systems) one lane per function
+__global__ void __launch_bounds__(256) b_tile_reduce(const BatchSys *__restrict__ sys, int nsys,
+                                                     const long long *__restrict__ r2pre,
+                                                     const int32_t *__restrict__ active,
+                                                     const double *__restrict__ part,
+                                                     const double *__restrict__ kpart,
+                                                     double *__restrict__ red) {
+  __shared__ double sh[256];
+  const long long blk = blockIdx.x;
+  const int s = seg_find(r2pre, nsys, blk);
+  if (active && !active[s]) return;  // uniform over the workgroup
   const BatchSys &b = sys[s];
-  const int r = (int)(idx - rpre[s]);
-  const int v = r / b.K, k = r - v * b.K;
-  if (b.two)
-    red[b.red_off + (size_t)v * b.K + k] = tile_reduce2_one(
-        part + b.part_off, kpart + b.kpart_off, b.K, b.kt, b.ts, b.nacc, v, k);
-  else
-    tile_reduce_one(part + b.part_off, b.ntiles, b.K, b.nacc, v, k, red + b.red_off);
+  const int r = (int)(blk - r2pre[s]);
+  const int nch = (b.K + 63) / 64;
+  const int v = r / nch, k0 = (r - v * nch) * 64;
+  if (b.two) {
+    tile_reduce2_block(part + b.part_off, kpart + b.kpart_off, b.K, b.kt, b.ts, b.nacc, v, k0,
+                       red + b.red_off, sh);
+  } else if (threadIdx.x < 64 && k0 + (int)threadIdx.x < b.K) {
+    tile_reduce_one(part + b.part_off, b.ntiles, b.K, b.nacc, v, k0 + threadIdx.x,
+                    red + b.red_off);
+  }
 }
 
 // one lane per (system, function)
@@ -316,9 +322,11 @@ int dim_nacc(int n) { int r = 0; SWITCH_N(n, r = Dim<NN>::NACC); return r; }
 struct necg_batch {
   int device = 0, nsys = 0, ncu = 0;
   std::vector<BatchSys> hsys;
-  std::vector<long long> fpre, kpre, hpre, rpre;  // host copies of the prefixes
+  // host copies of the prefixes: functions, (function, term) records,
+  // (row, chunk) workgroups, (slot, 64 functions) workgroups
+  std::vector<long long> fpre, kpre, hpre, r2pre;
   BatchSys *d_sys = nullptr;
-  long long *d_fpre = nullptr, *d_kpre = nullptr, *d_hpre = nullptr, *d_rpre = nullptr;
+  long long *d_fpre = nullptr, *d_kpre = nullptr, *d_hpre = nullptr, *d_r2pre = nullptr;
   double *sml = nullptr, *sms = nullptr, *scvv = nullptr;
   double *bra = nullptr, *ket = nullptr, *H = nullptr, *S = nullptr, *hpart = nullptr,
          *spart = nullptr, *part = nullptr, *red = nullptr, *HC = nullptr, *SC = nullptr,
@@ -492,7 +500,7 @@ int necg_batch_destroy(necg_batch *B) {
                       B->part, B->red, B->HC, B->SC, B->hcp, B->scp, B->scal, B->kpart};
     for (double *p : bufs)
       if (p) (void)hipFree(p);
-    long long *pre[] = {B->d_fpre, B->d_kpre, B->d_hpre, B->d_rpre};
+    long long *pre[] = {B->d_fpre, B->d_kpre, B->d_hpre, B->d_r2pre};
     for (long long *p : pre)
       if (p) (void)hipFree(p);
     if (B->d_sys) (void)hipFree(B->d_sys);
@@ -539,7 +547,7 @@ int necg_batch_create(const necg_system *systems, int32_t nsys, const int32_t *K
   B->fpre.assign(1, 0);
   B->kpre.assign(1, 0);
   B->hpre.assign(1, 0);
-  B->rpre.assign(1, 0);
+  B->r2pre.assign(1, 0);
   for (int s = 0; s < nsys; ++s) {
     const necg_system &y = systems[s];
     const int n = y.n, K = Ks[s];
@@ -587,7 +595,7 @@ int necg_batch_create(const necg_system *systems, int32_t nsys, const int32_t *K
     B->fpre.push_back(B->fpre.back() + K);
     B->kpre.push_back(B->kpre.back() + (long long)K * y.nst);
     B->hpre.push_back(B->hpre.back() + (long long)K * b.nchunk);
-    B->rpre.push_back(B->rpre.back() + (long long)b.nacc * K);
+    B->r2pre.push_back(B->r2pre.back() + (long long)b.nacc * ((K + 63) / 64));
   }
   B->nx = xo;
   int rc = NECG_OK;
@@ -600,7 +608,7 @@ int necg_batch_create(const necg_system *systems, int32_t nsys, const int32_t *K
       (rc = dalloc(&B->scal, (size_t)8 * nsys)) || (rc = dalloc(&B->d_sys, (size_t)nsys)) ||
       (rc = dalloc(&B->flag, 1)) || (rc = upload_prefix(&B->d_fpre, B->fpre)) ||
       (rc = upload_prefix(&B->d_kpre, B->kpre)) || (rc = upload_prefix(&B->d_hpre, B->hpre)) ||
-      (rc = upload_prefix(&B->d_rpre, B->rpre))) {
+      (rc = upload_prefix(&B->d_r2pre, B->r2pre))) {
     necg_batch_destroy(B);
     return rc;
   }
@@ -670,8 +678,8 @@ int necg_batch_outprod_masked(necg_batch *B, const int32_t *active, const double
                      B->d_hpre, active, B->hpart, B->spart, B->bra, B->H, B->S, B->hcp, B->scp);
   hipLaunchKernelGGL(b_quotient, dim3(n), dim3(1024), 0, st, B->d_sys, active, B->hcp, B->scp,
                      B->bra, B->HC, B->SC, B->scal, f, B->flag);
-  hipLaunchKernelGGL(b_tile_reduce, dim3(blocks(B->rpre[n], tb)), dim3(tb), 0, st, B->d_sys, n,
-                     B->d_rpre, active, B->part, B->kpart, B->red);
+  hipLaunchKernelGGL(b_tile_reduce, dim3((unsigned)B->r2pre[n]), dim3(256), 0, st, B->d_sys, n,
+                     B->d_r2pre, active, B->part, B->kpart, B->red);
   hipLaunchKernelGGL(b_grad_final, dim3(blocks(B->fpre[n], tb)), dim3(tb), 0, st, B->d_sys, n,
                      B->d_fpre, active, B->red, B->bra, B->scal, B->HC, B->SC, g);
   return hip_rc(hipGetLastError());

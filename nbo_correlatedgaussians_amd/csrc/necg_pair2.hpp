@@ -630,16 +630,36 @@ __host__ __device__ inline TriRanges tri_ranges(int K, int kt, int k) {
 
 // Sum of the two-sided partials of function k, slot v: bra partials at
 // bpart[((y * ts + z) * nacc + v) * K + k], ket partials at
-// kpart[((x * ts + z) * K + k) * nacc + v], over tri_ranges.  Fixed order.
-__device__ __forceinline__ double tile_reduce2_one(const double *__restrict__ bpart,
-                                                   const double *__restrict__ kpart, int K,
-                                                   int kt, int ts, int nacc, int v, int k) {
+// kpart[((x * ts + z) * K + k) * nacc + v], over tri_ranges.  Part q of np
+// (np lanes share one function) sums every np-th partial from the q-th on;
+// the caller adds the np part sums in order q = 0, 1, ...: a fixed order.
+__device__ __forceinline__ double tile_reduce2_part(const double *__restrict__ bpart,
+                                                    const double *__restrict__ kpart, int K,
+                                                    int kt, int ts, int nacc, int v, int k,
+                                                    int q, int np) {
   const TriRanges r = tri_ranges(K, kt, k);
   double s = 0.0;
-  for (int t = r.yb0 * ts; t < r.yb1 * ts; ++t) s += bpart[((size_t)t * nacc + v) * K + k];
-  for (int t = r.yu0 * ts; t < r.yu1 * ts; ++t) s += bpart[((size_t)t * nacc + v) * K + k];
-  for (int t = r.x0 * ts; t < r.x1 * ts; ++t) s += kpart[((size_t)t * K + k) * nacc + v];
+  for (int t = r.yb0 * ts + q; t < r.yb1 * ts; t += np)
+    s += bpart[((size_t)t * nacc + v) * K + k];
+  for (int t = r.yu0 * ts + q; t < r.yu1 * ts; t += np)
+    s += bpart[((size_t)t * nacc + v) * K + k];
+  for (int t = r.x0 * ts + q; t < r.x1 * ts; t += np)
+    s += kpart[((size_t)t * K + k) * nacc + v];
   return s;
+}
+
+// The reduction of slot v for the 64 functions [k0, k0 + 64) by the 4 waves
+// of a 256-lane workgroup: wave q sums part q, wave 0 adds the parts.
+__device__ __forceinline__ void tile_reduce2_block(const double *__restrict__ bpart,
+                                                   const double *__restrict__ kpart, int K,
+                                                   int kt, int ts, int nacc, int v, int k0,
+                                                   double *__restrict__ red, double *sh) {
+  const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
+  const int k = k0 + lane;
+  sh[threadIdx.x] = k < K ? tile_reduce2_part(bpart, kpart, K, kt, ts, nacc, v, k, q, 4) : 0.0;
+  __syncthreads();
+  if (q == 0 && k < K)
+    red[(size_t)v * K + k] = ((sh[lane] + sh[64 + lane]) + sh[128 + lane]) + sh[192 + lane];
 }
 
 }  // namespace necg
