@@ -483,10 +483,12 @@ int finish_grad(necg_ctx *c, int K, int ntiles, const double *gbar, double *g, h
 
 // Ket tile and term groups of the lower-triangle passes.  One workgroup per
 // CU at a time, its waves taking alternate kets, so a launch takes about
-// (rounds of ncu workgroups) x (kets per wave) ket-term sweeps; the ket tile
-// minimising that is chosen (measured on LiH K = 2048: the model orders nine
-// tile sizes within 2 %), then term groups while there are fewer than two
-// workgroups per CU.
+// (rounds of ncu workgroups) x (kets per wave + a per-tile overhead of 0.2
+// ket) ket-term sweeps; the ket tile minimising that is chosen (measured on
+// LiH K = 2048: the model orders nine tile sizes within 2 %; the overhead
+// term, from four repeated 4-vs-8 comparisons, also prices the per-tile
+// partials the reduction reads), then term groups while there are fewer
+// than two workgroups per CU.
 void choose_tiling2(necg_ctx *c, int K, int &kt, int &ts) {
   const int ncu = std::max(1, c->ncu);
   const int w = pair_waves(c->n);
@@ -494,12 +496,12 @@ void choose_tiling2(necg_ctx *c, int K, int &kt, int &ts) {
     kt = c->req_kt;
   } else {
     static const int cand[] = {2, 4, 8, 16, 32, 64};   // divisors of 64 (upper pairs)
-    long long best = -1;
+    double best = -1.0;
     kt = 8;
     for (int q : cand) {
       const long long n = tri_count(K, q);
-      const long long cost = ((n + ncu - 1) / ncu) * ((q + w - 1) / w);
-      if (best < 0 || cost < best) {
+      const double cost = (double)((n + ncu - 1) / ncu) * ((q + w - 1) / w + 0.2);
+      if (best < 0.0 || cost < best) {
         best = cost;
         kt = q;
       }

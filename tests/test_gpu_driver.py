@@ -80,6 +80,27 @@ def test_optimiser_trajectory_matches_oracle(oracle_lib, method):
     assert rg.energies[-1] < rg.energies[0]
 
 
+def test_lbfgsb_trajectory_on_generator_input(oracle_lib):
+    """Config 0 as specified (LiH K = 12 made by the reference's generator
+    from seed 0, C = 1/K): 10 L-BFGS-B iterations driven by the GPU outprod
+    follow the oracle-driven ones to 1e-9 relative in every energy.  (Truncated
+    Newton is not compared here: its gradient-difference Hessian products turn
+    a 1e-13 change of the gradient into an O(1) change of its first step on
+    this input.)"""
+    from nbo_correlatedgaussians_amd.driver import optimize
+    from nbo_correlatedgaussians_amd.engine import Engine
+    _, s, iv, C = necg.generated_lih_input(12, 0)
+    x0 = necg.pack_x(iv, C)
+    eng = Engine(s, "cuda:0")
+    o = oracle_lib.Oracle(s)
+    rg = optimize(eng.outprod, x0, method="L-BFGS-B", maxiter=10, siv=12 * s.blk, M=12)
+    ro = optimize(o.outprod, x0, method="L-BFGS-B", maxiter=10, siv=12 * s.blk, M=12)
+    assert rg.nit == ro.nit == 10
+    for a, b in zip(rg.energies, ro.energies):
+        assert abs(a - b) <= 1e-9 * max(1.0, abs(b)), (a, b)
+    assert rg.energies[-1] < rg.energies[0]
+
+
 def test_driver_run_small():
     from nbo_correlatedgaussians_amd.driver import run
     s = necg.he_system()
