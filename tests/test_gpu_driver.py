@@ -58,7 +58,7 @@ def test_kinetic_potential_and_energetics(oracle_lib):
 
 @pytest.mark.parametrize("method", ["L-BFGS-B", "TN"])
 def test_optimiser_trajectory_matches_oracle(oracle_lib, method):
-    """Config 0 (LiH K = 12, seed 0): the optimiser driven by the GPU
+    """LiH K = 12, seed 0: the optimiser driven by the GPU
     outprod follows the oracle-driven trajectory.  Truncated Newton forms
     Hessian-vector products from gradient differences over a ~sqrt(eps) step,
     which amplifies 1e-12 gradient differences, so only its first outer
@@ -81,8 +81,8 @@ def test_optimiser_trajectory_matches_oracle(oracle_lib, method):
 
 
 def test_lbfgsb_trajectory_on_generator_input(oracle_lib):
-    """Config 0 as specified (LiH K = 12 made by the reference's generator
-    from seed 0, C = 1/K): 10 L-BFGS-B iterations driven by the GPU outprod
+    """LiH K = 12 made by the reference's input generator from seed 0
+    (C = 1/K): 10 L-BFGS-B iterations driven by the GPU outprod
     follow the oracle-driven ones to 1e-9 relative in every energy.  (Truncated
     Newton is not compared here: its gradient-difference Hessian products turn
     a 1e-13 change of the gradient into an O(1) change of its first step on
