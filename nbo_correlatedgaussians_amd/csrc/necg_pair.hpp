@@ -60,17 +60,19 @@ struct Dim {
   static constexpr int N3 = 3 * N;            // shift block length
   static constexpr int BLK = NS + N3;         // parameters per basis function
   static constexpr int NP = N * (N + 1) / 2;  // Coulomb pairs incl. reference
-  // ket-table record for one (ket, term): A' | S' | A'S' | A'MA' | A'M | i2 pl wc sc
+  // ket-table record for one (ket, term): A' | S' | A'S' | A'MA' | i2 pl wc sc | A'M,
+  // padded to whole 128-byte lines; the two-sided term reads the fields before
+  // A'M only (4 lines for n = 5)
   static constexpr int K_AP = 0;
   static constexpr int K_SP = K_AP + NS;
   static constexpr int K_APSP = K_SP + N3;
   static constexpr int K_NP = K_APSP + N3;
-  static constexpr int K_APM = K_NP + NS;
-  static constexpr int K_I2 = K_APM + N * N;
+  static constexpr int K_I2 = K_NP + NS;
   static constexpr int K_PL = K_I2 + 1;
   static constexpr int K_WC = K_PL + 1;  // scvv_P * C_l
   static constexpr int K_SC = K_WC + 1;  // scvv_P
-  static constexpr int KS = (K_SC + 1 + 1) & ~1;
+  static constexpr int K_APM = K_SC + 1;
+  static constexpr int KS = (K_APM + N * N + 15) & ~15;
   // bra-table record: A | S | AS | i1 pk C | L (vech) | 1/L_ii | A M A | tr(M A)
   static constexpr int B_A = 0;
   static constexpr int B_S = B_A + NS;

@@ -124,3 +124,22 @@ def test_two_sided_refused_where_it_does_not_apply(oracle_lib):
         assert eng.pair_form() == (False, False)
         Eo, go = oracle_lib.Oracle(t).outprod(x)
         assert _close(E, g, Eo, go, 1e-12, 1e-9)
+
+
+@pytest.mark.parametrize("K", [1000, 4096])
+def test_two_sided_large_k(K):
+    """LiH at K = 1000 (the last bra tile's functions paired as upper kets)
+    and K = 4096 (the largest served system): the two forms agree, and the
+    gradient satisfies C . g_C = 0 (E is invariant under scaling C)."""
+    s = necg.lih_system()
+    iv, C = necg.generate_parameters(s, K, 31)
+    x = necg.pack_x(iv, C)
+    print(f"two-sided LiH K={K}", flush=True)
+    eng = _engine(s)
+    E2, g2 = eng.outprod(x)
+    assert eng.pair_form() == (True, True)
+    eng.set_pair_mode(2)
+    E1, g1 = eng.outprod(x)
+    assert _close(E2, g2, E1, g1, 1e-13, 1e-11)
+    gC = g2[K * s.blk:]
+    assert abs(np.dot(C, gC)) <= 1e-10 * np.linalg.norm(C) * max(np.linalg.norm(gC), 1.0)
