@@ -106,6 +106,7 @@ __global__ void __launch_bounds__(64 * pair_waves(N), NECG_MIN_WAVES) pair2_kern
   __shared__ double acc_s[PairLds<N, MODE_SPLIT>::ACC];
   __shared__ double kacc_s[pair_waves(N) * 64];
   __shared__ double bra_s[BraLds<N>::NF * 64];
+  __shared__ double kst_s[pair_waves(N) * 64];
   const int bx = tiles[2 * blockIdx.x], by = tiles[2 * blockIdx.x + 1];
   const int k0 = bx * 64, l0 = by * kt;
   const int up = tri_up(K, kt), lup = k0 < up ? up : K;
@@ -115,7 +116,7 @@ __global__ void __launch_bounds__(64 * pair_waves(N), NECG_MIN_WAVES) pair2_kern
   pair2_tile<N>(bra, ket, K, nst, k0, l0, l1, lup, p0, min(nst, p0 + tps), sc, tperm,
                 bpart + (size_t)(by * ts + z) * D::NACC * K,
                 kpart + (size_t)(bx * ts + z) * K * D::NACC, hpart + (size_t)z * K * K,
-                spart + (size_t)z * K * K, flag, acc_s, kacc_s, bra_s);
+                spart + (size_t)z * K * K, flag, acc_s, kacc_s, bra_s, kst_s);
 }
 
 // Element-only pass (MODE_ELEMENTS / MODE_TV) over the same lower-triangle
@@ -813,9 +814,7 @@ int necg_create(const necg_system *sys, int device, necg_ctx **out) {
   for (int i = 0; i < n; ++i)
     for (int j = 0; j <= i; ++j) c->sc.msym[sx(i, j)] = sys->mass[i * n + j];
   for (int i = 0; i <= n; ++i) c->sc.q[i] = sys->ncv[i];
-  int pi = 0;
-  for (int p = 0; p <= n; ++p)
-    for (int q = p + 1; q <= n; ++q) c->sc.cpair[pi++] = sys->ncv[p] * sys->ncv[q];
+  set_pair_charges(c->sc, sys->ncv, n);
   DeviceGuard g(device);
   int rc = NECG_OK;
   const size_t nsml = (size_t)nst * n * n, nsms = (size_t)nst * 9 * n * n;

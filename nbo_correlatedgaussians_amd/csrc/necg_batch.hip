@@ -145,6 +145,7 @@ __global__ void __launch_bounds__(256, 1) b_pair2_all(
   __shared__ double acc_s[PairLds<5, MODE_SPLIT>::ACC];
   __shared__ double kacc_s[4 * 64];
   __shared__ double bra_s[BraLds<5>::NF * 64];
+  __shared__ double kst_s[4 * 64];
   for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
     const Item w = items[it];
     if (active && !active[w.sys]) continue;  // uniform over the workgroup
@@ -161,7 +162,7 @@ __global__ void __launch_bounds__(256, 1) b_pair2_all(
 #define NECG_B2(NN)                                                                          \
   case NN:                                                                                   \
     pair2_tile<NN>(br, kt, K, b.nst, w.k0, w.l0, w.l1, lup, w.p0, w.p1, b.sc, tp, bp, kp, hp, \
-                   sp, flag, acc_s, kacc_s, bra_s);                                          \
+                   sp, flag, acc_s, kacc_s, bra_s, kst_s);                                   \
     break;
       NECG_B2(1)
       NECG_B2(2)
@@ -559,9 +560,7 @@ int necg_batch_create(const necg_system *systems, int32_t nsys, const int32_t *K
     for (int i = 0; i < n; ++i)
       for (int j = 0; j <= i; ++j) b.sc.msym[sx(i, j)] = y.mass[i * n + j];
     for (int i = 0; i <= n; ++i) b.sc.q[i] = y.ncv[i];
-    int pi = 0;
-    for (int p = 0; p <= n; ++p)
-      for (int q = p + 1; q <= n; ++q) b.sc.cpair[pi++] = y.ncv[p] * y.ncv[q];
+    set_pair_charges(b.sc, y.ncv, n);
     b.n = n;
     b.nst = y.nst;
     b.K = K;

@@ -103,7 +103,24 @@ struct SysConst {
   double msym[kMaxN * (kMaxN + 1) / 2];   // packed symmetric copy of mass
   double q[kMaxN + 1];                    // ncv
   double cpair[kMaxN * (kMaxN + 1) / 2];  // q_p q_q in pair order
+  double cpair2[kMaxN * (kMaxN + 1) / 2]; // 2 q_p q_q / sqrt(pi) in pair order
 };
+
+// Position of particle pair (p, q), 0 <= p < q <= n (0 the fixed particle),
+// in the pair order p = 0, 1, ...; q = p + 1, ..., n.
+__host__ __device__ constexpr int pair_index(int p, int q, int n) {
+  return p * n - p * (p - 1) / 2 + (q - p - 1);
+}
+
+// The pair constants of SysConst from the charges ncv[0..n].
+inline void set_pair_charges(SysConst &sc, const double *ncv, int n) {
+  for (int p = 0; p <= n; ++p)
+    for (int q = p + 1; q <= n; ++q) {
+      const double c = ncv[p] * ncv[q];
+      sc.cpair[pair_index(p, q, n)] = c;
+      sc.cpair2[pair_index(p, q, n)] = c * (2.0 * 0.56418958354775628695);
+    }
+}
 
 // 1/x: hardware seed + two Newton steps (relative error ~1 ulp for the
 // positive normal arguments that reach here).
@@ -115,15 +132,15 @@ __device__ __forceinline__ double rcp(double x) {
   return __builtin_fma(r, e, r);
 }
 
-// 1/sqrt(x) for positive normal x: hardware seed + two Newton steps
-// y <- y + y (1/2 - x y^2 / 2).
+// 1/sqrt(x) for positive normal x: the hardware seed y (relative error
+// below 2^-25) and one third-order step from the series of (1 - e)^-1/2,
+// e = 1 - x y^2: y (1 + e/2 + 3 e^2/8).  The truncation error is O(e^3),
+// far below rounding; over 4M log-uniform x in [e^-20, e^20] the result is
+// within 0.994 ulp of 1/sqrt(x) (two Newton steps: 0.99 ulp, one: 20 ulp).
 __device__ __forceinline__ double rsq(double x) {
-  double y = __builtin_amdgcn_rsq(x);
-  const double hx = 0.5 * x;
-  double r = __builtin_fma(-hx * y, y, 0.5);
-  y = __builtin_fma(y, r, y);
-  r = __builtin_fma(-hx * y, y, 0.5);
-  return __builtin_fma(y, r, y);
+  const double y = __builtin_amdgcn_rsq(x);
+  const double e = __builtin_fma(-x * y, y, 1.0);
+  return __builtin_fma(y, e * __builtin_fma(0.375, e, 0.5), y);
 }
 
 constexpr double kSqrtPiHalf = 0.88622692545275801365;   // sqrt(pi)/2

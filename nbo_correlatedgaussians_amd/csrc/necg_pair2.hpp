@@ -80,17 +80,17 @@ __device__ __forceinline__ void fill_bra_lds(double *dst, const double *__restri
 // placements tried (LiH K = 2048: 7.52 ms against 7.74 ms with one per phase).
 __device__ __forceinline__ void phase_barrier() { asm volatile("" ::: "memory"); }
 
-// F0 and dF0/dz as boys_f0 (python/utils.py:78-106), written so that the
+// F0 and 2 dF0/dz as boys_f0 (python/utils.py:78-106), written so that the
 // 15 evaluations of a term form one straight-line block: the Pade part is
 // evaluated at min(z, 16.3578) (finite, and not used above the cut), each
 // Horner pass yields the polynomial and its derivative together (only the
 // reference's 11 constants appear, as addends), and the branch is a select
-// of finished values.  v = (P Q^3)^-1/2 gives F0 = P Q v and
-// dF0 = (Q P' - P Q') v / 2; above the cut v = z^-1/2, F0 = sqrt(pi)/2 v,
-// dF0 = -F0 v^2 / 2.
-__device__ __forceinline__ void boys_f0_line(double z, double &f, double &fp) {
+// of factors.  v = (P Q^3)^-1/2 below the cut gives F0 = P Q v and
+// 2 dF0 = (Q P' - P Q') v; above it v = z^-1/2, F0 = sqrt(pi)/2 v and
+// 2 dF0 = -F0 v^2.
+__device__ __forceinline__ void boys_f0_line(double z, double &f, double &f2p) {
   const bool big = z > kZcut;
-  const double zc = big ? kZcut : z;
+  const double zc = __builtin_fmin(z, kZcut);
   double P = kC1, dP = kC1;
   P = __builtin_fma(P, zc, kC2);
   dP = __builtin_fma(dP, zc, P);
@@ -115,11 +115,10 @@ __device__ __forceinline__ void boys_f0_line(double z, double &f, double &fp) {
   Q = __builtin_fma(Q, zc, 1.0);
   const double PQ = P * Q;
   const double v = rsq(big ? z : PQ * (Q * Q));
-  const double fa = kSqrtPiHalf * v;
-  double fb = 0.5 * __builtin_fma(Q, dP, -P * dQ) * v;
-  asm volatile("" : "+v"(fb));   // formed on every lane: no branch around it
-  f = big ? fa : PQ * v;
-  fp = big ? -0.5 * fa * (v * v) : fb;
+  f = (big ? kSqrtPiHalf : PQ) * v;
+  double num = __builtin_fma(Q, dP, -P * dQ);
+  asm volatile("" : "+v"(num));   // formed on every lane: no branch around it
+  f2p = (big ? -f * v : num) * v;
 }
 
 // First ket of the upper pairs of a lower-triangle pass: when K is not a
@@ -138,6 +137,9 @@ struct Two {
   static constexpr int NV = Dim<N>::NACC;
   static constexpr int NCH = (NV + 15) / 16;
   static constexpr int KV = NCH * 16;
+  // ket-record fields the two-sided term reads (A', S', A'S', A'MA', i2, pl,
+  // scvv C, scvv): the record's leading fields, one per lane when staged
+  static constexpr int KF = Dim<N>::K_SC + 1;
 };
 
 __device__ __forceinline__ double lane_swap32_add(double x, double y) {
@@ -327,13 +329,13 @@ __device__ __forceinline__ bool pair_term2(const BraLds<N> &b, const double *__r
   const double tfac = 2.0 * i3k + 3.0 * (b.f(BraLds<N>::F_TRMA) - ab);
 
   // Coulomb (python/matel.py:162-197; gradients python/gradel.py:266-337).
-  double csum = 0.0;
+  // Per particle pair (p, q) (p = 0 the fixed particle): with
+  // h = 2 c_pq ri^3 / sqrt(pi), the coefficients of d(i2p)/dB^-1 are
+  // ca = f h + 2 F0' h z (La) and cb = 2 F0' h (Lb).  The diagonal entry of
+  // particle a sums the pairs that contain a (the first of them, (0, a),
+  // sets it); the off-diagonal entry (a, b) is minus the pair's value.
+  double csum2 = 0.0;  // twice the Coulomb sum
   double La[NS], Lb[NS];
-#pragma unroll
-  for (int t = 0; t < NS; ++t) {
-    La[t] = 0.0;
-    Lb[t] = 0.0;
-  }
   {
     int pi = 0;
 #pragma unroll
@@ -348,31 +350,31 @@ __device__ __forceinline__ bool pair_term2(const BraLds<N> &b, const double *__r
           dy = st[3 * bq + 1];
           dz = st[3 * bq + 2];
         } else {
-          i2p = Bi[sx(a, a)] + Bi[sx(bq, bq)] - 2.0 * Bi[sx(a, bq)];
+          i2p = __builtin_fma(-2.0, Bi[sx(a, bq)], Bi[sx(a, a)] + Bi[sx(bq, bq)]);
           dx = st[3 * a] - st[3 * bq];
           dy = st[3 * a + 1] - st[3 * bq + 1];
           dz = st[3 * a + 2] - st[3 * bq + 2];
         }
         const double ri = rsq(i2p);
         const double inv2 = ri * ri;
-        const double z = (dx * dx + dy * dy + dz * dz) * inv2;
-        double f, fp;
-        boys_f0_line(z, f, fp);
-        const double c = sc_.cpair[pi];
-        const double cr0 = c * ri * kInvSqrtPi;
-        csum = __builtin_fma(cr0, f, csum);
-        const double cb = 4.0 * fp * cr0 * inv2;
-        const double ca = __builtin_fma(cb, z, 2.0 * f * cr0 * inv2);
+        const double z = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx)) * inv2;
+        double f, f2p;
+        boys_f0_line(z, f, f2p);
+        const double cr = sc_.cpair2[pi] * ri;   // 2 c / sqrt(pi i2p)
+        csum2 = __builtin_fma(cr, f, csum2);
+        const double h = cr * inv2;
+        const double cb = f2p * h;
+        const double ca = __builtin_fma(cb, z, f * h);
         if (p == 0) {
-          La[sx(bq, bq)] += ca;
-          Lb[sx(bq, bq)] += cb;
+          La[sx(bq, bq)] = ca;
+          Lb[sx(bq, bq)] = cb;
         } else {
           La[sx(a, a)] += ca;
           La[sx(bq, bq)] += ca;
-          La[sx(a, bq)] -= ca;
+          La[sx(bq, a)] = -ca;
           Lb[sx(a, a)] += cb;
           Lb[sx(bq, bq)] += cb;
-          Lb[sx(a, bq)] -= cb;
+          Lb[sx(bq, a)] = -cb;
         }
       }
   }
@@ -380,7 +382,7 @@ __device__ __forceinline__ bool pair_term2(const BraLds<N> &b, const double *__r
   double zsum = 0.0;
 #pragma unroll
   for (int i = 0; i < N; ++i) zsum = __builtin_fma(sc_.q[i + 1], st[3 * i + 2], zsum);
-  const double alpha = 2.0 * tfac + 2.0 * csum - sc_.efs * zsum;
+  const double alpha = 2.0 * tfac + csum2 - sc_.efs * zsum;
   hel = __builtin_fma(scv * ov, alpha, hel);
   sel = __builtin_fma(scv, ov, sel);
 
@@ -532,6 +534,11 @@ __device__ __forceinline__ bool pair_term2(const BraLds<N> &b, const double *__r
 
 // One two-sided pair tile: bras [k0, k0+64) x kets [l0, l1) x terms [p0, p1),
 // run by the pair_waves(N) waves of a workgroup (each takes alternate kets).
+// The ket fields of each (ket, term) reach the term through the wave's LDS
+// buffer kst_s (64 doubles per wave): the record of the next (ket, term) is
+// loaded, one field per lane, while the current term runs and stored to the
+// buffer after it, so the term's reads are LDS broadcasts with no wait on
+// scalar memory.
 // A pair contributes when bra k >= ket l, or when l >= lup (the upper pairs
 // of the last, partial bra tile's functions): the bra side of every such
 // pair and the ket side of those with k != l.  Writes the element partials
@@ -546,9 +553,11 @@ __device__ __forceinline__ void pair2_tile(const double *__restrict__ bra,
                                            double *__restrict__ bpt, double *__restrict__ kpt,
                                            double *__restrict__ hp, double *__restrict__ sp,
                                            int *__restrict__ flag, double *acc_s,
-                                           double *kacc_s, double *bra_s) {
+                                           double *kacc_s, double *bra_s, double *kst_s) {
   using D = Dim<N>;
   constexpr int W = pair_waves(N);
+  constexpr int KF = Two<N>::KF;
+  static_assert(KF <= 64, "one staged ket field per lane");
   constexpr int NA = D::NACC * 64;
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -562,6 +571,8 @@ __device__ __forceinline__ void pair2_tile(const double *__restrict__ bra,
   __syncthreads();
   const BraLds<N> b{bra_s + lane};
   LdsSink sink{acc_s + wv * NA + lane};
+  double *kst = kst_s + wv * 64;
+  if (l0 + wv < l1 && lane < KF) kst[lane] = ket[((size_t)(l0 + wv) * nst + p0) * D::KS + lane];
   bool ok = true;
   for (int l = l0 + wv; l < l1; l += W) {
     const bool upper = l >= lup;
@@ -569,10 +580,16 @@ __device__ __forceinline__ void pair2_tile(const double *__restrict__ bra,
     const double mb = valid ? 1.0 : 0.0;
     const double mk = (valid && k != l) ? 1.0 : 0.0;
     double hel = 0.0, sel = 0.0;
-    const double *kl = ket + (size_t)l * nst * D::KS;
-    for (int p = p0; p < p1; ++p)
-      ok &= pair_term2<N>(b, kl + (size_t)p * D::KS, sc, mb, mk, tperm[p * 64 + lane], hel, sel,
-                          sink, kacc, lane);
+    for (int p = p0; p < p1; ++p) {
+      // the record after (l, p): (l, p + 1), else the wave's next ket at p0
+      const int pn = p + 1 < p1 ? p + 1 : p0;
+      const int ln = p + 1 < p1 ? l : l + W;
+      double nx = 0.0;
+      if (ln < l1 && lane < KF) nx = ket[((size_t)ln * nst + pn) * D::KS + lane];
+      ok &= pair_term2<N>(b, kst, sc, mb, mk, tperm[p * 64 + lane], hel, sel, sink, kacc, lane);
+      // after every read of the term (the wave's LDS operations complete in order)
+      if (lane < KF) kst[lane] = nx;
+    }
     if (valid) {
       const size_t o = upper ? (size_t)k * K + l : (size_t)l * K + k;
       hp[o] = hel;

@@ -13,9 +13,9 @@ Two models:
 * ``flops_per_eval``: the two-sided algorithm the pair kernel runs: per
   unordered (pair, term) one factorisation, the elements, the Coulomb and
   Stark derivatives, and both gradient halves (``two_sided_term``; LiH:
-  4,760 FLOP per pair-term, 114,240 per off-diagonal pair).  The identities
+  4,609 FLOP per pair-term, 110,616 per off-diagonal pair).  The identities
   it uses (A' dif2 = -A dif1, B^-1 A' = I - B^-1 A, the collapsed kinetic and
-  Coulomb gradient products) make it 3.6x smaller than the v0 model for LiH,
+  Coulomb gradient products) make it 3.7x smaller than the v0 model for LiH,
   so the v0 model overstates the work of the run and is not the roofline
   basis.
 """
@@ -47,10 +47,10 @@ def two_sided_term(n: int) -> dict:
     elem = n3 + 2 * n3 * n + 2 * n3 + 6                  # E, st = B^-1 E, i3, ov
     kin = n3 + 2 * n3 * n + 2 * n3 * n + 2 * n3          # dif1, a1 = A dif1, M a1, i3k
     kin += 2 * ns + (ns - n) + 4                         # <AMA, B^-1>, tfac
-    ref_pair = 2 + 1 + 6 + 55 + 4 + 3 + 5 + 2            # (0, q): rsq, z, Boys F0/F0', sums
-    pp_pair = ref_pair + 3 + 3 + 4                       # (p, q): i2, dif, 6 rank-1 updates
+    ref_pair = 2 + 1 + 6 + 51 + 8                        # (0, q): rsq, 1/i2, z, Boys F0/2F0', ca cb
+    pp_pair = ref_pair + 3 + 3 + 4                       # (p, q): i2, dif, 4 diagonal updates
     coul = n * ref_pair + (ns - n) * pp_pair
-    stark = 2 * n + 5 + 3 + 2                            # zsum, alpha, H and S sums
+    stark = 2 * n + 4 + 3 + 2                            # zsum, alpha, H and S sums
     shared = 2 * n3 * n + 2 * n + 3 * 2 * n3 * n         # Lb st, Stark column, A M a1, B^-1 F, B^-1 v
     side = (n3 + 2 * n3                                  # dif, Phi from the shared parts
             + 2 * ns + 2 * n * n * n                     # 12 A'MA' + La, times B^-1
