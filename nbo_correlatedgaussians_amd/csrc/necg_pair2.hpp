@@ -73,13 +73,6 @@ __device__ __forceinline__ void fill_bra_lds(double *dst, const double *__restri
   dst[64 * f + lane] = rec[src];
 }
 
-// Compiler barrier between the phases of a term: memory operations are not
-// moved across it, so the gradient halves read the bra fields they need right
-// where they need them (no instruction is emitted).  Two barriers, before the
-// shared gradient pieces and before the ket side, measured fastest among the
-// placements tried (LiH K = 2048: 7.52 ms against 7.74 ms with one per phase).
-__device__ __forceinline__ void phase_barrier() { asm volatile("" ::: "memory"); }
-
 // F0 and 2 dF0/dz as boys_f0 (python/utils.py:78-106), written so that the
 // 15 evaluations of a term form one straight-line block: the Pade part is
 // evaluated at min(z, 16.3578) (finite, and not used above the cut), each
@@ -171,12 +164,14 @@ __device__ __forceinline__ double lane_swap16_add(double x, double y) {
 }
 
 // v from the DPP partner lane (CTRL: 0x140 row mirror, 0x141 half-row
-// mirror, quad permutations below 0x100); every lane has a partner.
+// mirror, quad permutations below 0x100); every lane has a partner and
+// every row and bank is enabled, so no lane keeps a previous value (the
+// move needs no initialised destination).
 template <int CTRL>
 __device__ __forceinline__ double lane_dpp(double v) {
   const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
-  const unsigned lo = __builtin_amdgcn_update_dpp(0u, (unsigned)u, CTRL, 0xf, 0xf, false);
-  const unsigned hi = __builtin_amdgcn_update_dpp(0u, (unsigned)(u >> 32), CTRL, 0xf, 0xf, false);
+  const unsigned lo = __builtin_amdgcn_mov_dpp((int)(unsigned)u, CTRL, 0xf, 0xf, false);
+  const unsigned hi = __builtin_amdgcn_mov_dpp((int)(unsigned)(u >> 32), CTRL, 0xf, 0xf, false);
   return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
 }
 
@@ -386,7 +381,6 @@ __device__ __forceinline__ bool pair_term2(const BraLds<N> &b, const double *__r
   hel = __builtin_fma(scv * ov, alpha, hel);
   sel = __builtin_fma(scv, ov, sel);
 
-  phase_barrier();
   // Shared gradient pieces: Phic = B^-1 (Lb st - efs q e_z^T), w = B^-1 A M a1.
   double Phic[N3], w[N3];
   {
@@ -469,7 +463,6 @@ __device__ __forceinline__ bool pair_term2(const BraLds<N> &b, const double *__r
     sink.add(D::A_SS, wo);
   }
 
-  phase_barrier();
   // Ket side: Phi' = Phic - 8 w, G' = B^-1 (12 A M A + La) B^-1, dif2 = st - S'.
   {
     const double wk = mk * scv * b.f(BraLds<N>::F_C);   // scvv_P C_k
