@@ -281,6 +281,17 @@ def properties_leg(eng, xd, K, reps=5):
             "dipole_au": [float(v) for v in o[-3:]]}
 
 
+def fp64_fma_rate(iters=200000):
+    """The FP64 FMA rate this GPU sustains (necg_fp64_fma_rate: eight
+    independent FMA chains per lane, two waves per SIMD), in TFLOP/s."""
+    import ctypes as ct
+    from nbo_correlatedgaussians_amd import _lib
+    lib = _lib.load()
+    r = ct.c_double(0.0)
+    _lib.raise_for(lib.necg_fp64_fma_rate(0, iters, ct.byref(r)), "necg_fp64_fma_rate")
+    return r.value
+
+
 def traffic_probe(K, seed):
     """For the rocprofv3 counter passes: a 256 MiB device copy (the
     calibration of FETCH_SIZE / WRITE_SIZE) followed by two outprod calls."""
@@ -407,6 +418,7 @@ def main():
     achieved = fl / (pair_ms / 1000.0) / 1e12
     kt, ts, w = eng.tiling()
     ex_fl = recorded_executed_flop() if (K == 2048 and args.seed == 1 and two_sided) else None
+    fma_peak = fp64_fma_rate()
     out = {
         "metric": "ECG pair energy+grad evals/s (FP64, % of peak); served p50/p99 latency",
         "value": value,
@@ -442,6 +454,8 @@ def main():
                      "executed_frac": (ex_fl / (pair_ms / 1000.0) / 1e12 / PEAK_FP64_TFLOPS
                                        if ex_fl else None),
                      "executed_source": "profiles/pair2_kernel_lih_k2048_fp64_counters.md",
+                     "fma_probe_tflops": fma_peak,
+                     "frac_of_fma_probe": achieved / fma_peak if fma_peak else None,
                      "tiling": {"ket_tile": kt, "term_split": ts, "waves": w}},
         "energy": float(E.item()),
     }

@@ -134,6 +134,13 @@ int necg_set_pair_mode(necg_ctx *ctx, int32_t mode);
  * *last = 1 if the last gradient pair pass used it.  Host pointers. */
 int necg_pair_form(necg_ctx *ctx, int32_t *available, int32_t *last);
 
+/* The FP64 vector FMA rate device `device` sustains, in TFLOP/s (*tflops,
+ * host pointer): eight independent FMA chains per lane, `iters` steps each,
+ * eight 256-lane workgroups per CU, best of three timed launches.  A
+ * measurement aid for setting the pair kernels' rates beside the device's own
+ * peak; no counterpart in the reference. */
+int necg_fp64_fma_rate(int32_t device, int32_t iters, double *tflops);
+
 int necg_kinetic_potential(necg_ctx *ctx, int32_t K, const double *x, double *T, double *V,
                            void *stream);
 

@@ -25,7 +25,7 @@ NECG_ERR_STALE = 8
 
 EXPORTS = ("necg_create", "necg_destroy", "necg_outprod", "necg_energy",
            "necg_kinetic_potential", "necg_dipole", "necg_set_dipole_mode", "necg_device_visible",
-           "necg_set_pair_mode", "necg_pair_form",
+           "necg_set_pair_mode", "necg_pair_form", "necg_fp64_fma_rate",
            "necg_energy_fwd", "necg_energy_bwd",
            "necg_energy_vjp", "necg_matrices", "necg_matel", "necg_gradel",
            "necg_dhds", "necg_status", "necg_set_tiling", "necg_get_tiling",
@@ -98,6 +98,7 @@ def load() -> ct.CDLL:
         "necg_set_dipole_mode": [P, I],
         "necg_set_pair_mode": [P, I],
         "necg_pair_form": [P, ct.POINTER(I), ct.POINTER(I)],
+        "necg_fp64_fma_rate": [I, I, ct.POINTER(D)],
         "necg_device_visible": [P],
         "necg_energy_fwd": [P, I, P, P, ct.POINTER(ct.c_int64), P],
         "necg_energy_bwd": [P, I, ct.c_int64, P, P, P],

@@ -12,7 +12,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 SRCS = [os.path.join(HERE, "csrc", f) for f in ("necg.hip", "necg_batch.hip", "necg_pool.hip",
-                                                   "necg_props.hip")]
+                                                   "necg_props.hip", "necg_probe.hip")]
 OUT = os.path.join(HERE, "libnecg_hip.so")
 ARCH = os.environ.get("NECG_OFFLOAD_ARCH", "gfx950")
 

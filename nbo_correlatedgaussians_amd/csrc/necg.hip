@@ -95,7 +95,7 @@ __global__ void __launch_bounds__(64 * pair_waves(N), NECG_MIN_WAVES) pair_kerne
 // <= bra tile end; blockIdx.y = term group z of gridDim.y.
 // bpart layout: [(ket tile * ts + z) * NACC + v] * K + k;
 // kpart layout: [((bra tile * ts + z) * K + l) * NACC + v].
-template <int N>
+template <int N, bool R1>
 __global__ void __launch_bounds__(64 * pair_waves(N), NECG_MIN_WAVES) pair2_kernel(
     const double *__restrict__ bra, const double *__restrict__ ket, int K, int nst, int kt,
     int tps, SysConst sc, const int *__restrict__ tiles, const unsigned *__restrict__ tperm,
@@ -113,7 +113,7 @@ __global__ void __launch_bounds__(64 * pair_waves(N), NECG_MIN_WAVES) pair2_kern
   const int l1 = l0 < lup ? min(min(K, l0 + kt), k0 + 64) : min(K, l0 + kt);
   const int z = blockIdx.y, ts = gridDim.y;
   const int p0 = z * tps;
-  pair2_tile<N>(bra, ket, K, nst, k0, l0, l1, lup, p0, min(nst, p0 + tps), sc, tperm,
+  pair2_tile<N, R1>(bra, ket, K, nst, k0, l0, l1, lup, p0, min(nst, p0 + tps), sc, tperm,
                 bpart + (size_t)(by * ts + z) * D::NACC * K,
                 kpart + (size_t)(bx * ts + z) * K * D::NACC, hpart + (size_t)z * K * K,
                 spart + (size_t)z * K * K, flag, acc_s, kacc_s, bra_s, kst_s);
@@ -581,9 +581,14 @@ int run_pairs2(necg_ctx *c, int K, hipStream_t st, int &kt, int &ts) {
   if ((rc = ensure_hpart(c, (size_t)ts * K * K))) return rc;
   hipEvent_t e1;
   if ((rc = timing_begin(c, st, e1))) return rc;
-  hipLaunchKernelGGL((pair2_kernel<N>), dim3(c->ntri, ts), dim3(64 * pair_waves(N)), 0, st,
-                     c->bra, c->ket, K, c->nst, kt, tps, c->sc, c->d_tiles, c->d_tperm, c->part,
-                     c->kpart, c->hpart, c->spart, c->d_flag);
+  if (c->sc.mrank1)
+    hipLaunchKernelGGL((pair2_kernel<N, true>), dim3(c->ntri, ts), dim3(64 * pair_waves(N)), 0,
+                       st, c->bra, c->ket, K, c->nst, kt, tps, c->sc, c->d_tiles, c->d_tperm,
+                       c->part, c->kpart, c->hpart, c->spart, c->d_flag);
+  else
+    hipLaunchKernelGGL((pair2_kernel<N, false>), dim3(c->ntri, ts), dim3(64 * pair_waves(N)), 0,
+                       st, c->bra, c->ket, K, c->nst, kt, tps, c->sc, c->d_tiles, c->d_tperm,
+                       c->part, c->kpart, c->hpart, c->spart, c->d_flag);
   if (c->timing) HIP_TRY(hipEventRecord(e1, st));
   c->last_kt = kt;
   c->last_ts = ts;
@@ -810,9 +815,7 @@ int necg_create(const necg_system *sys, int device, necg_ctx **out) {
   c->sc.n = n;
   c->sc.nst = nst;
   c->sc.efs = sys->efs;
-  for (int i = 0; i < n * n; ++i) c->sc.mass[i] = sys->mass[i];
-  for (int i = 0; i < n; ++i)
-    for (int j = 0; j <= i; ++j) c->sc.msym[sx(i, j)] = sys->mass[i * n + j];
+  set_mass(c->sc, sys->mass, n);
   for (int i = 0; i <= n; ++i) c->sc.q[i] = sys->ncv[i];
   set_pair_charges(c->sc, sys->ncv, n);
   DeviceGuard g(device);

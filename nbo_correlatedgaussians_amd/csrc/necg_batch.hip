@@ -161,8 +161,12 @@ __global__ void __launch_bounds__(256, 1) b_pair2_all(
     switch (b.n) {
 #define NECG_B2(NN)                                                                          \
   case NN:                                                                                   \
-    pair2_tile<NN>(br, kt, K, b.nst, w.k0, w.l0, w.l1, lup, w.p0, w.p1, b.sc, tp, bp, kp, hp, \
-                   sp, flag, acc_s, kacc_s, bra_s, kst_s);                                   \
+    if (b.sc.mrank1)                                                                         \
+      pair2_tile<NN, true>(br, kt, K, b.nst, w.k0, w.l0, w.l1, lup, w.p0, w.p1, b.sc, tp, bp, \
+                           kp, hp, sp, flag, acc_s, kacc_s, bra_s, kst_s);                   \
+    else                                                                                     \
+      pair2_tile<NN, false>(br, kt, K, b.nst, w.k0, w.l0, w.l1, lup, w.p0, w.p1, b.sc, tp,   \
+                            bp, kp, hp, sp, flag, acc_s, kacc_s, bra_s, kst_s);              \
     break;
       NECG_B2(1)
       NECG_B2(2)
@@ -556,9 +560,7 @@ int necg_batch_create(const necg_system *systems, int32_t nsys, const int32_t *K
     b.sc.n = n;
     b.sc.nst = y.nst;
     b.sc.efs = y.efs;
-    for (int i = 0; i < n * n; ++i) b.sc.mass[i] = y.mass[i];
-    for (int i = 0; i < n; ++i)
-      for (int j = 0; j <= i; ++j) b.sc.msym[sx(i, j)] = y.mass[i * n + j];
+    set_mass(b.sc, y.mass, n);
     for (int i = 0; i <= n; ++i) b.sc.q[i] = y.ncv[i];
     set_pair_charges(b.sc, y.ncv, n);
     b.n = n;

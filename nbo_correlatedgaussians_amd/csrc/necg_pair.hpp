@@ -104,7 +104,25 @@ struct SysConst {
   double q[kMaxN + 1];                    // ncv
   double cpair[kMaxN * (kMaxN + 1) / 2];  // q_p q_q in pair order
   double cpair2[kMaxN * (kMaxN + 1) / 2]; // 2 q_p q_q / sqrt(pi) in pair order
+  // mass = diag(mdiag) + mc 1 1^T when every off-diagonal entry equals mc
+  // (relative coordinates about a reference particle); then mrank1 = 1
+  int mrank1;
+  double mc;
+  double mdiag[kMaxN];
 };
+
+// The mass constants of SysConst from the row-major n x n mass matrix.
+inline void set_mass(SysConst &sc, const double *mass, int n) {
+  for (int i = 0; i < n * n; ++i) sc.mass[i] = mass[i];
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j <= i; ++j) sc.msym[sx(i, j)] = mass[i * n + j];
+  sc.mc = n > 1 ? mass[1] : 0.0;
+  sc.mrank1 = 1;
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j)
+      if (i != j && mass[i * n + j] != sc.mc) sc.mrank1 = 0;
+  for (int i = 0; i < n; ++i) sc.mdiag[i] = mass[i * n + i] - sc.mc;
+}
 
 // Position of particle pair (p, q), 0 <= p < q <= n (0 the fixed particle),
 // in the pair order p = 0, 1, ...; q = p + 1, ..., n.

@@ -207,8 +207,10 @@ __device__ __forceinline__ double wave_sum16(const double *v, int lane) {
 // the ket's accumulator record for the values it holds after each chunk sum.
 // Adds sc*ov*alpha to hel and sc*ov to sel, the bra-side gradient (weight
 // scvv_P C_l) to the lane's LDS slots and the ket-side gradient (weight
-// scvv_P C_k, summed over the wave) to the ket accumulator kacc.
-template <int N>
+// scvv_P C_k, summed over the wave) to the ket accumulator kacc.  R1: the
+// mass matrix is diag(mdiag) + mc 1 1^T (SysConst::mrank1), so M a1 takes
+// one column sum instead of a full product.
+template <int N, bool R1>
 __device__ __forceinline__ bool pair_term2(const BraLds<N> &b, const double *__restrict__ kd,
                                            const SysConst &sc_, double mb, double mk,
                                            unsigned pw, double &hel, double &sel, LdsSink &sink,
@@ -304,16 +306,35 @@ __device__ __forceinline__ bool pair_term2(const BraLds<N> &b, const double *__r
           v = __builtin_fma(b.A(sx(i, j)), st[3 * j + x] - b.S(3 * j + x), v);
         a1[3 * i + x] = v;
       }
-#pragma unroll
-    for (int i = 0; i < N; ++i)
+    if constexpr (R1) {
+      double cs[3];
 #pragma unroll
       for (int x = 0; x < 3; ++x) {
-        double t = 0.0;
+        double t = a1[x];
 #pragma unroll
-        for (int j = 0; j < N; ++j) t = __builtin_fma(sc_.msym[sx(i, j)], a1[3 * j + x], t);
-        Ma1[3 * i + x] = t;
-        i3k = __builtin_fma(-a1[3 * i + x], t, i3k);
+        for (int j = 1; j < N; ++j) t += a1[3 * j + x];
+        cs[x] = sc_.mc * t;
       }
+#pragma unroll
+      for (int i = 0; i < N; ++i)
+#pragma unroll
+        for (int x = 0; x < 3; ++x) {
+          const double t = __builtin_fma(sc_.mdiag[i], a1[3 * i + x], cs[x]);
+          Ma1[3 * i + x] = t;
+          i3k = __builtin_fma(-a1[3 * i + x], t, i3k);
+        }
+    } else {
+#pragma unroll
+      for (int i = 0; i < N; ++i)
+#pragma unroll
+        for (int x = 0; x < 3; ++x) {
+          double t = 0.0;
+#pragma unroll
+          for (int j = 0; j < N; ++j) t = __builtin_fma(sc_.msym[sx(i, j)], a1[3 * j + x], t);
+          Ma1[3 * i + x] = t;
+          i3k = __builtin_fma(-a1[3 * i + x], t, i3k);
+        }
+    }
   }
   double ab = 0.0;  // <AMA, B^-1> over the full symmetric matrices
 #pragma unroll
@@ -538,7 +559,7 @@ __device__ __forceinline__ bool pair_term2(const BraLds<N> &b, const double *__r
 // to hp/sp at [min(k, l)][max(k, l)], the tile's bra accumulators to bpt
 // ([slot][k], NACC x K) and, per ket, its accumulator record summed over the
 // tile's bras to kpt ([l][slot], K x NACC).
-template <int N>
+template <int N, bool R1>
 __device__ __forceinline__ void pair2_tile(const double *__restrict__ bra,
                                            const double *__restrict__ ket, int K, int nst, int k0,
                                            int l0, int l1, int lup, int p0, int p1,
@@ -579,7 +600,7 @@ __device__ __forceinline__ void pair2_tile(const double *__restrict__ bra,
       const int ln = p + 1 < p1 ? l : l + W;
       double nx = 0.0;
       if (ln < l1 && lane < KF) nx = ket[((size_t)ln * nst + pn) * D::KS + lane];
-      ok &= pair_term2<N>(b, kst, sc, mb, mk, tperm[p * 64 + lane], hel, sel, sink, kacc, lane);
+      ok &= pair_term2<N, R1>(b, kst, sc, mb, mk, tperm[p * 64 + lane], hel, sel, sink, kacc, lane);
       // after every read of the term (the wave's LDS operations complete in order)
       if (lane < KF) kst[lane] = nx;
     }

@@ -126,6 +126,32 @@ def test_two_sided_refused_where_it_does_not_apply(oracle_lib):
         assert _close(E, g, Eo, go, 1e-12, 1e-9)
 
 
+def test_two_sided_general_mass_matrix(oracle_lib):
+    """LiH with the nucleus-electron mass couplings changed (still invariant
+    under the electron permutations, so the two-sided form applies) but no
+    longer equal to the electron-electron ones: the term takes the general
+    M a1 product instead of the diagonal-plus-constant form."""
+    s = necg.lih_system()
+    m = s.mass.copy()
+    m[0, 1:] *= 1.7
+    m[1:, 0] *= 1.7
+    t = necg.System(n=s.n, nst=s.nst, efs=s.efs, mass=m, ncv=s.ncv.copy(),
+                    scvv_raw=s.scvv_raw.copy(), sml=s.sml.copy(), sms=s.sms.copy(),
+                    name="LiH-coupled")
+    t.validate()
+    iv, C = necg.generate_parameters(t, 70, 12)
+    x = necg.pack_x(iv, C)
+    print("two-sided LiH-coupled K=70", flush=True)
+    eng = _engine(t)
+    E2, g2 = eng.outprod(x)
+    assert eng.pair_form() == (True, True)
+    eng.set_pair_mode(2)
+    E1, g1 = eng.outprod(x)
+    assert _close(E2, g2, E1, g1, 1e-13, 1e-11)
+    Eo, go = oracle_lib.Oracle(t).outprod(x)
+    assert _close(E2, g2, Eo, go, 1e-12, 1e-9)
+
+
 @pytest.mark.parametrize("K", [1000, 4096])
 def test_two_sided_large_k(K):
     """LiH at K = 1000 (the last bra tile's functions paired as upper kets)
