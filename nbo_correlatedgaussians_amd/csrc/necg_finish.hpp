@@ -104,8 +104,8 @@ __device__ __forceinline__ void make_bra(const double *__restrict__ p, double c,
 
 // Ket record of function l (bra record b) under term P with ket transform T
 // (n x n) and shift transform Sm (3n x 3n): A' = T^T A T, S' = Sm s
-// (python/matel.py:76-87), A'S', A'MA', A'M, i2, the ket prefactor, and the
-// weights scvv_P C_l and scvv_P.
+// (python/matel.py:76-87), A'S', A'MA', A'M, tr(M A'), i2, the ket prefactor,
+// and the weights scvv_P C_l and scvv_P.
 template <int N>
 __device__ __forceinline__ void make_ket(const double *__restrict__ b,
                                          const double *__restrict__ T,
@@ -166,6 +166,10 @@ __device__ __forceinline__ void make_ket(const double *__restrict__ b,
       ApM[i][j] = s;
       r[D::K_APM + i * N + j] = s;
     }
+  double trm = 0.0;
+#pragma unroll
+  for (int i = 0; i < N; ++i) trm += ApM[i][i];
+  r[D::K_TRMP] = trm;
 #pragma unroll
   for (int i = 0; i < N; ++i)
 #pragma unroll

@@ -60,9 +60,9 @@ struct Dim {
   static constexpr int N3 = 3 * N;            // shift block length
   static constexpr int BLK = NS + N3;         // parameters per basis function
   static constexpr int NP = N * (N + 1) / 2;  // Coulomb pairs incl. reference
-  // ket-table record for one (ket, term): A' | S' | A'S' | A'MA' | i2 pl wc sc | A'M,
-  // padded to whole 128-byte lines; the two-sided term reads the fields before
-  // A'M only (4 lines for n = 5)
+  // ket-table record for one (ket, term): A' | S' | A'S' | A'MA' | i2 pl wc sc | A'M |
+  // tr(M A'), padded to whole 128-byte lines; the two-sided term reads the
+  // fields before A'M only (4 lines for n = 5)
   static constexpr int K_AP = 0;
   static constexpr int K_SP = K_AP + NS;
   static constexpr int K_APSP = K_SP + N3;
@@ -72,7 +72,8 @@ struct Dim {
   static constexpr int K_WC = K_PL + 1;  // scvv_P * C_l
   static constexpr int K_SC = K_WC + 1;  // scvv_P
   static constexpr int K_APM = K_SC + 1;
-  static constexpr int KS = (K_APM + N * N + 15) & ~15;
+  static constexpr int K_TRMP = K_APM + N * N;  // tr(M A')
+  static constexpr int KS = (K_TRMP + 1 + 15) & ~15;
   // bra-table record: A | S | AS | i1 pk C | L (vech) | 1/L_ii | A M A | tr(M A)
   static constexpr int B_A = 0;
   static constexpr int B_S = B_A + NS;
@@ -164,23 +165,56 @@ __device__ __forceinline__ double rsq(double x) {
 constexpr double kSqrtPiHalf = 0.88622692545275801365;   // sqrt(pi)/2
 constexpr double kInvSqrtPi = 0.56418958354775628695;    // 1/sqrt(pi)
 
-// Boys F0 with the reference's Pade form and its derivative
-// (python/utils.py:78-106): F0 = sqrt(P5/Q6) for z <= 16.3578, else
-// sqrt(pi/(4z)); dF0 = (Q P' - P Q') / (2 F0 Q^2), or -F0/(2z).
-// One reciprocal square root serves both branches and both outputs:
-// with v = (P Q^3)^-1/2, F0 = P Q v and dF0 = (Q P' - P Q') v / 2; with
-// v = z^-1/2, F0 = sqrt(pi)/2 v and dF0 = -F0 v^2 / 2.  Branch-free.
-__device__ __forceinline__ void boys_f0(double z, double &f, double &fp) {
-  const double P = __builtin_fma(__builtin_fma(__builtin_fma(__builtin_fma(__builtin_fma(kC1, z, kC2), z, kC3), z, kC4), z, kC5), z, 1.0);
-  const double Q = __builtin_fma(__builtin_fma(__builtin_fma(__builtin_fma(__builtin_fma(__builtin_fma(kC6, z, kC7), z, kC8), z, kC9), z, kC10), z, kC11), z, 1.0);
-  const double dP = __builtin_fma(__builtin_fma(__builtin_fma(__builtin_fma(5.0 * kC1, z, 4.0 * kC2), z, 3.0 * kC3), z, 2.0 * kC4), z, kC5);
-  const double dQ = __builtin_fma(__builtin_fma(__builtin_fma(__builtin_fma(__builtin_fma(6.0 * kC6, z, 5.0 * kC7), z, 4.0 * kC8), z, 3.0 * kC9), z, 2.0 * kC10), z, kC11);
+// F0 and 2 dF0/dz (python/utils.py:78-106), written so that the
+// 15 evaluations of a term form one straight-line block: the Pade part is
+// evaluated at min(z, 16.3578) (finite, and not used above the cut), each
+// Horner pass yields the polynomial and its derivative together (only the
+// reference's 11 constants appear, as addends), and the branch is a select
+// of factors.  v = (P Q^3)^-1/2 below the cut gives F0 = P Q v and
+// 2 dF0 = (Q P' - P Q') v; above it v = z^-1/2, F0 = sqrt(pi)/2 v and
+// 2 dF0 = -F0 v^2.
+__device__ __forceinline__ void boys_f0_line(double z, double &f, double &f2p) {
   const bool big = z > kZcut;
+  const double zc = __builtin_fmin(z, kZcut);
+  double P = kC1, dP = kC1;
+  P = __builtin_fma(P, zc, kC2);
+  dP = __builtin_fma(dP, zc, P);
+  P = __builtin_fma(P, zc, kC3);
+  dP = __builtin_fma(dP, zc, P);
+  P = __builtin_fma(P, zc, kC4);
+  dP = __builtin_fma(dP, zc, P);
+  P = __builtin_fma(P, zc, kC5);
+  dP = __builtin_fma(dP, zc, P);
+  P = __builtin_fma(P, zc, 1.0);
+  double Q = kC6, dQ = kC6;
+  Q = __builtin_fma(Q, zc, kC7);
+  dQ = __builtin_fma(dQ, zc, Q);
+  Q = __builtin_fma(Q, zc, kC8);
+  dQ = __builtin_fma(dQ, zc, Q);
+  Q = __builtin_fma(Q, zc, kC9);
+  dQ = __builtin_fma(dQ, zc, Q);
+  Q = __builtin_fma(Q, zc, kC10);
+  dQ = __builtin_fma(dQ, zc, Q);
+  Q = __builtin_fma(Q, zc, kC11);
+  dQ = __builtin_fma(dQ, zc, Q);
+  Q = __builtin_fma(Q, zc, 1.0);
   const double PQ = P * Q;
   const double v = rsq(big ? z : PQ * (Q * Q));
-  const double fa = kSqrtPiHalf * v;
-  f = big ? fa : PQ * v;
-  fp = big ? -0.5 * fa * (v * v) : 0.5 * __builtin_fma(Q, dP, -P * dQ) * v;
+  f = (big ? kSqrtPiHalf : PQ) * v;
+  double num = __builtin_fma(Q, dP, -P * dQ);
+  asm volatile("" : "+v"(num));   // formed on every lane: no branch around it
+  f2p = (big ? -f * v : num) * v;
+}
+
+// F0 alone (python/utils.py:78-90) for the element modes: with
+// v = (P Q^3)^-1/2 below the cut F0 = P Q v, above it F0 = sqrt(pi)/2 z^-1/2.
+__device__ __forceinline__ double boys_f0_value(double z) {
+  const bool big = z > kZcut;
+  const double zc = __builtin_fmin(z, kZcut);
+  const double P = __builtin_fma(__builtin_fma(__builtin_fma(__builtin_fma(__builtin_fma(kC1, zc, kC2), zc, kC3), zc, kC4), zc, kC5), zc, 1.0);
+  const double Q = __builtin_fma(__builtin_fma(__builtin_fma(__builtin_fma(__builtin_fma(__builtin_fma(kC6, zc, kC7), zc, kC8), zc, kC9), zc, kC10), zc, kC11), zc, 1.0);
+  const double PQ = P * Q;
+  return (big ? kSqrtPiHalf : PQ) * rsq(big ? z : PQ * (Q * Q));
 }
 
 // Bra-side data of one basis function, held in registers.
@@ -355,27 +389,23 @@ __device__ __forceinline__ bool pair_term(const Bra &b, const double *__restrict
         u8[3 * i + x] = 8.0 * v;
       }
   }
-  double T2 = 0.0;  // 3 tr(M A B^-1 A') = 3 sum_ij (A'M A)_ij B^-1_ji
+  // 3 tr(M A B^-1 A') = 3 (tr(M A') - <A'MA', B^-1>), since A B^-1 = I - A' B^-1
+  double ab = 0.0;
 #pragma unroll
   for (int i = 0; i < N; ++i)
 #pragma unroll
-    for (int j = 0; j < N; ++j) {
-      double t = 0.0;
-#pragma unroll
-      for (int m = 0; m < N; ++m) t = __builtin_fma(ApM[i * N + m], b.A(sx(m, j)), t);
-      T2 = __builtin_fma(t, Bi[sx(j, i)], T2);
-    }
-  T2 *= 3.0;
-  const double tfac = 2.0 * i3k + T2;
+    for (int j = 0; j <= i; ++j)
+      ab = __builtin_fma((i == j ? 1.0 : 2.0) * Np[sx(i, j)], Bi[sx(i, j)], ab);
+  const double tfac = 2.0 * i3k + 3.0 * (kd[D::K_TRMP] - ab);
 
   // Coulomb (python/matel.py:162-197; gradients python/gradel.py:266-337).
-  double csum = 0.0;  // sum_c c f r0
+  // Per particle pair (p, q): with h = 2 c_pq ri^3 / sqrt(pi), the
+  // coefficients of d(i2p)/dB^-1 are ca = f h + 2 F0' h z (La) and
+  // cb = 2 F0' h (Lb); the diagonal entry of particle a sums the pairs that
+  // contain a (the first, (0, a), sets it), the off-diagonal entry (a, b) is
+  // minus the pair's value.
+  double csum2 = 0.0;  // twice sum_c c f r0
   double La[NS], Lb[NS];
-#pragma unroll
-  for (int t = 0; t < NS; ++t) {
-    La[t] = 0.0;
-    Lb[t] = 0.0;
-  }
   (void)La;
   (void)Lb;
   {
@@ -392,33 +422,35 @@ __device__ __forceinline__ bool pair_term(const Bra &b, const double *__restrict
           dy = st[3 * bq + 1];
           dz = st[3 * bq + 2];
         } else {
-          i2p = Bi[sx(a, a)] + Bi[sx(bq, bq)] - 2.0 * Bi[sx(a, bq)];
+          i2p = __builtin_fma(-2.0, Bi[sx(a, bq)], Bi[sx(a, a)] + Bi[sx(bq, bq)]);
           dx = st[3 * a] - st[3 * bq];
           dy = st[3 * a + 1] - st[3 * bq + 1];
           dz = st[3 * a + 2] - st[3 * bq + 2];
         }
         const double ri = rsq(i2p);  // 1/sqrt(i2p)
         const double inv2 = ri * ri;
-        const double z = (dx * dx + dy * dy + dz * dz) * inv2;
-        double f, fp;
-        boys_f0(z, f, fp);
-        const double c = sc_.cpair[pi];
-        const double cr0 = c * ri * kInvSqrtPi;  // c / sqrt(pi i2p)
-        csum = __builtin_fma(cr0, f, csum);
+        const double z = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx)) * inv2;
+        const double cr = sc_.cpair2[pi] * ri;  // 2 c / sqrt(pi i2p)
         if (mode_has_grad(MODE)) {
-          const double cb = 4.0 * fp * cr0 * inv2;
-          const double ca = __builtin_fma(cb, z, 2.0 * f * cr0 * inv2);
+          double f, f2p;
+          boys_f0_line(z, f, f2p);
+          csum2 = __builtin_fma(cr, f, csum2);
+          const double h = cr * inv2;
+          const double cb = f2p * h;
+          const double ca = __builtin_fma(cb, z, f * h);
           if (p == 0) {
-            La[sx(bq, bq)] += ca;
-            Lb[sx(bq, bq)] += cb;
+            La[sx(bq, bq)] = ca;
+            Lb[sx(bq, bq)] = cb;
           } else {
             La[sx(a, a)] += ca;
             La[sx(bq, bq)] += ca;
-            La[sx(a, bq)] -= ca;
+            La[sx(bq, a)] = -ca;
             Lb[sx(a, a)] += cb;
             Lb[sx(bq, bq)] += cb;
-            Lb[sx(a, bq)] -= cb;
+            Lb[sx(bq, a)] = -cb;
           }
+        } else {
+          csum2 = __builtin_fma(cr, boys_f0_value(z), csum2);
         }
       }
   }
@@ -426,11 +458,11 @@ __device__ __forceinline__ bool pair_term(const Bra &b, const double *__restrict
   double zsum = 0.0;
 #pragma unroll
   for (int i = 0; i < N; ++i) zsum = __builtin_fma(sc_.q[i + 1], st[3 * i + 2], zsum);
-  const double alpha = 2.0 * tfac + 2.0 * csum - sc_.efs * zsum;
+  const double alpha = 2.0 * tfac + csum2 - sc_.efs * zsum;
   if (MODE == MODE_TV) {
     // Kinetic += scvv tk, Potential += scvv ert (the Stark term is in H only)
     hel = __builtin_fma(scv * ov, 2.0 * tfac, hel);
-    sel = __builtin_fma(scv * ov, 2.0 * csum, sel);
+    sel = __builtin_fma(scv * ov, csum2, sel);
   } else {
     hel = __builtin_fma(scv * ov, alpha, hel);
     sel = __builtin_fma(scv, ov, sel);
@@ -438,7 +470,7 @@ __device__ __forceinline__ bool pair_term(const Bra &b, const double *__restrict
   if (parts) {
     parts->ov = ov;
     parts->tk = 2.0 * ov * tfac;
-    parts->ert = 2.0 * ov * csum;
+    parts->ert = ov * csum2;
     parts->zht = ov * zsum * sc_.efs;
   }
   if (!mode_has_grad(MODE)) return ok;

@@ -73,47 +73,6 @@ __device__ __forceinline__ void fill_bra_lds(double *dst, const double *__restri
   dst[64 * f + lane] = rec[src];
 }
 
-// F0 and 2 dF0/dz as boys_f0 (python/utils.py:78-106), written so that the
-// 15 evaluations of a term form one straight-line block: the Pade part is
-// evaluated at min(z, 16.3578) (finite, and not used above the cut), each
-// Horner pass yields the polynomial and its derivative together (only the
-// reference's 11 constants appear, as addends), and the branch is a select
-// of factors.  v = (P Q^3)^-1/2 below the cut gives F0 = P Q v and
-// 2 dF0 = (Q P' - P Q') v; above it v = z^-1/2, F0 = sqrt(pi)/2 v and
-// 2 dF0 = -F0 v^2.
-__device__ __forceinline__ void boys_f0_line(double z, double &f, double &f2p) {
-  const bool big = z > kZcut;
-  const double zc = __builtin_fmin(z, kZcut);
-  double P = kC1, dP = kC1;
-  P = __builtin_fma(P, zc, kC2);
-  dP = __builtin_fma(dP, zc, P);
-  P = __builtin_fma(P, zc, kC3);
-  dP = __builtin_fma(dP, zc, P);
-  P = __builtin_fma(P, zc, kC4);
-  dP = __builtin_fma(dP, zc, P);
-  P = __builtin_fma(P, zc, kC5);
-  dP = __builtin_fma(dP, zc, P);
-  P = __builtin_fma(P, zc, 1.0);
-  double Q = kC6, dQ = kC6;
-  Q = __builtin_fma(Q, zc, kC7);
-  dQ = __builtin_fma(dQ, zc, Q);
-  Q = __builtin_fma(Q, zc, kC8);
-  dQ = __builtin_fma(dQ, zc, Q);
-  Q = __builtin_fma(Q, zc, kC9);
-  dQ = __builtin_fma(dQ, zc, Q);
-  Q = __builtin_fma(Q, zc, kC10);
-  dQ = __builtin_fma(dQ, zc, Q);
-  Q = __builtin_fma(Q, zc, kC11);
-  dQ = __builtin_fma(dQ, zc, Q);
-  Q = __builtin_fma(Q, zc, 1.0);
-  const double PQ = P * Q;
-  const double v = rsq(big ? z : PQ * (Q * Q));
-  f = (big ? kSqrtPiHalf : PQ) * v;
-  double num = __builtin_fma(Q, dP, -P * dQ);
-  asm volatile("" : "+v"(num));   // formed on every lane: no branch around it
-  f2p = (big ? -f * v : num) * v;
-}
-
 // First ket of the upper pairs of a lower-triangle pass: when K is not a
 // multiple of 64 and kt divides 64, the functions of the last bra tile
 // P = (K-1)/64 are taken as kets of the full bra tiles (pairs k < l); else K

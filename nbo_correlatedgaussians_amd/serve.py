@@ -179,7 +179,7 @@ class Server:
         gradient partials (about K/16 tiles of NACC x K)."""
         n, nst = system.n, system.nst
         nacc = n * (n + 1) + 6 * n + 2
-        ks = (n * (n + 1) + 6 * n + 4 + n * n + 15) // 16 * 16   # ket record, whole lines
+        ks = (n * (n + 1) + 6 * n + 4 + n * n + 1 + 15) // 16 * 16   # ket record, whole lines
         return 8 * (4 * K * K + K * nst * ks + max(1, K // 16) * nacc * K)
 
     def close(self):
