@@ -74,7 +74,7 @@ __global__ void prep_ket(const double *__restrict__ bra, int K, int nst,
 // Lane = bra k.  blockIdx.y = ket tile, blockIdx.z = term group.
 // part layout: part[((tile * NACC) + v) * K + k], tile = y * TS + z.
 // hpart/spart layout: [z][l][k] for bra k >= ket l.
-template <int N, int MODE>
+template <int N, int MODE, bool R1>
 __global__ void __launch_bounds__(64 * pair_waves(N), NECG_MIN_WAVES) pair_kernel(
     const double *__restrict__ bra, const double *__restrict__ ket, int K, int nst, int kt,
     int tps, SysConst sc, const double *__restrict__ scal, double *__restrict__ part,
@@ -84,7 +84,7 @@ __global__ void __launch_bounds__(64 * pair_waves(N), NECG_MIN_WAVES) pair_kerne
   const int l0 = blockIdx.y * kt;
   const int p0 = blockIdx.z * tps;
   const size_t tile = (size_t)blockIdx.y * gridDim.z + blockIdx.z;
-  pair_tile<N, MODE>(bra, ket, K, nst, blockIdx.x * 64, l0, min(K, l0 + kt), p0,
+  pair_tile<N, MODE, R1>(bra, ket, K, nst, blockIdx.x * 64, l0, min(K, l0 + kt), p0,
                      min(nst, p0 + tps), sc, (MODE == MODE_COMBINED) ? scal[0] : 0.0,
                      part + tile * D::NACC * K, hpart + (size_t)blockIdx.z * K * K,
                      spart + (size_t)blockIdx.z * K * K, flag, acc_s);
@@ -121,7 +121,7 @@ __global__ void __launch_bounds__(64 * pair_waves(N), NECG_MIN_WAVES) pair2_kern
 
 // Element-only pass (MODE_ELEMENTS / MODE_TV) over the same lower-triangle
 // tile list: pairs with bra k >= ket l only.
-template <int N, int MODE>
+template <int N, int MODE, bool R1>
 __global__ void __launch_bounds__(64 * pair_waves(N), NECG_MIN_WAVES) pair_tri_kernel(
     const double *__restrict__ bra, const double *__restrict__ ket, int K, int nst, int kt,
     int tps, SysConst sc, const int *__restrict__ tiles, double *__restrict__ hpart,
@@ -134,7 +134,7 @@ __global__ void __launch_bounds__(64 * pair_waves(N), NECG_MIN_WAVES) pair_tri_k
   const int l1 = l0 < lup ? min(min(K, l0 + kt), k0 + 64) : min(K, l0 + kt);
   const int z = blockIdx.y;
   const int p0 = z * tps;
-  pair_tile<N, MODE>(bra, ket, K, nst, k0, l0, l1, p0, min(nst, p0 + tps), sc, 0.0, nullptr,
+  pair_tile<N, MODE, R1>(bra, ket, K, nst, k0, l0, l1, p0, min(nst, p0 + tps), sc, 0.0, nullptr,
                      hpart + (size_t)z * K * K, spart + (size_t)z * K * K, flag, acc_s, lup);
 }
 
@@ -443,8 +443,14 @@ int run_pairs(necg_ctx *c, int K, hipStream_t st, int &ntiles, int &ts_out) {
     c->nev++;
     HIP_TRY(hipEventRecord(e0, st));
   }
-  hipLaunchKernelGGL((pair_kernel<N, MODE>), grid, dim3(threads), 0, st, c->bra, c->ket, K,
-                     c->nst, kt, tps, c->sc, c->scal, c->part, c->hpart, c->spart, c->d_flag);
+  if (c->sc.mrank1)
+    hipLaunchKernelGGL((pair_kernel<N, MODE, true>), grid, dim3(threads), 0, st, c->bra, c->ket,
+                       K, c->nst, kt, tps, c->sc, c->scal, c->part, c->hpart, c->spart,
+                       c->d_flag);
+  else
+    hipLaunchKernelGGL((pair_kernel<N, MODE, false>), grid, dim3(threads), 0, st, c->bra, c->ket,
+                       K, c->nst, kt, tps, c->sc, c->scal, c->part, c->hpart, c->spart,
+                       c->d_flag);
   if (c->timing) HIP_TRY(hipEventRecord(e1, st));
   c->last_kt = kt;
   c->last_ts = ts;
@@ -608,9 +614,14 @@ int run_tri(necg_ctx *c, int K, hipStream_t st, int &ts) {
   if ((rc = ensure_hpart(c, (size_t)ts * K * K))) return rc;
   hipEvent_t e1;
   if ((rc = timing_begin(c, st, e1))) return rc;
-  hipLaunchKernelGGL((pair_tri_kernel<N, MODE>), dim3(c->ntri, ts), dim3(64 * pair_waves(N)), 0,
-                     st, c->bra, c->ket, K, c->nst, kt, tps, c->sc, c->d_tiles, c->hpart,
-                     c->spart, c->d_flag);
+  if (c->sc.mrank1)
+    hipLaunchKernelGGL((pair_tri_kernel<N, MODE, true>), dim3(c->ntri, ts),
+                       dim3(64 * pair_waves(N)), 0, st, c->bra, c->ket, K, c->nst, kt, tps, c->sc,
+                       c->d_tiles, c->hpart, c->spart, c->d_flag);
+  else
+    hipLaunchKernelGGL((pair_tri_kernel<N, MODE, false>), dim3(c->ntri, ts),
+                       dim3(64 * pair_waves(N)), 0, st, c->bra, c->ket, K, c->nst, kt, tps, c->sc,
+                       c->d_tiles, c->hpart, c->spart, c->d_flag);
   if (c->timing) HIP_TRY(hipEventRecord(e1, st));
   c->last_kt = kt;
   c->last_ts = ts;

@@ -281,7 +281,7 @@ __host__ __device__ constexpr bool mode_has_grad(int mode) {
 // A dif1 + A' dif2 = B st - E = 0), so the kinetic quadratic is
 // <A dif1, M A' dif2> = -<a1, M a1> with a1 = A dif1, and the vector
 // A dif1 - A' dif2 of the gradient is 2 a1.
-template <int N, int MODE, class Bra, class Sink>
+template <int N, int MODE, bool R1 = false, class Bra, class Sink>
 __device__ __forceinline__ bool pair_term(const Bra &b, const double *__restrict__ kd,
                                           const SysConst &sc_, double eshift, double &hel,
                                           double &sel, Sink &sink, Parts *parts) {
@@ -375,14 +375,24 @@ __device__ __forceinline__ bool pair_term(const Bra &b, const double *__restrict
         for (int j = 0; j < N; ++j) v = __builtin_fma(b.A(sx(i, j)), d1[3 * j + x], v);
         a1[3 * i + x] = v;
       }
+    double cs[3];  // R1: the constant part of M a1
+#pragma unroll
+    for (int x = 0; x < 3; ++x) {
+      double t = a1[x];
+#pragma unroll
+      for (int j = 1; j < N; ++j) t += a1[3 * j + x];
+      cs[x] = sc_.mc * t;
+    }
+    (void)cs;
 #pragma unroll
     for (int i = 0; i < N; ++i)
 #pragma unroll
       for (int x = 0; x < 3; ++x) {
         double t = 0.0, v = 0.0;
+        if constexpr (R1) t = __builtin_fma(sc_.mdiag[i], a1[3 * i + x], cs[x]);
 #pragma unroll
         for (int j = 0; j < N; ++j) {
-          t = __builtin_fma(sc_.msym[sx(i, j)], a1[3 * j + x], t);
+          if constexpr (!R1) t = __builtin_fma(sc_.msym[sx(i, j)], a1[3 * j + x], t);
           if (mode_has_grad(MODE)) v = __builtin_fma(ApM[i * N + j], a1[3 * j + x], v);
         }
         i3k = __builtin_fma(-a1[3 * i + x], t, i3k);
@@ -570,7 +580,7 @@ struct PairLds {
 // tile's summed accumulators to pt ([slot][k], NACC x K).  Kets l >= lup
 // (element modes of the lower-triangle pass: the last, partial bra tile's
 // functions) pair with every bra of the tile, written at [k][l].
-template <int N, int MODE>
+template <int N, int MODE, bool R1 = false>
 __device__ __forceinline__ void pair_tile(const double *__restrict__ bra,
                                           const double *__restrict__ ket, int K, int nst, int k0,
                                           int l0, int l1, int p0, int p1, const SysConst &sc,
@@ -603,7 +613,7 @@ __device__ __forceinline__ void pair_tile(const double *__restrict__ bra,
     double hel = 0.0, sel = 0.0;
     const double *kl = ket + (size_t)l * nst * D::KS;
     for (int p = p0; p < p1; ++p)
-      ok &= pair_term<N, MODE>(b, kl + (size_t)p * D::KS, sc, eshift, hel, sel, sink, nullptr);
+      ok &= pair_term<N, MODE, R1>(b, kl + (size_t)p * D::KS, sc, eshift, hel, sel, sink, nullptr);
     if (MODE != MODE_COMBINED && active && (k >= l || l >= lup)) {
       const size_t o = (l >= lup && k < l) ? (size_t)k * K + l : (size_t)l * K + k;
       hp[o] = hel;

@@ -85,9 +85,15 @@ __device__ __forceinline__ void trial_row(const PoolSys &b, int k, const double 
   RegSink<N> none;
   double h = 0.0, s = 0.0;
   bool ok = true;
-  for (int p = 0; p < b.nst; ++p)
-    ok &= pair_term<N, MODE_ELEMENTS>(br, w + D::BS + (size_t)p * D::KS, b.sc, 0.0, h, s, none,
-                                      nullptr);
+  if (b.sc.mrank1) {
+    for (int p = 0; p < b.nst; ++p)
+      ok &= pair_term<N, MODE_ELEMENTS, true>(br, w + D::BS + (size_t)p * D::KS, b.sc, 0.0, h, s,
+                                              none, nullptr);
+  } else {
+    for (int p = 0; p < b.nst; ++p)
+      ok &= pair_term<N, MODE_ELEMENTS, false>(br, w + D::BS + (size_t)p * D::KS, b.sc, 0.0, h, s,
+                                               none, nullptr);
+  }
   if (!ok) atomicOr(flag, (int)F_NOT_PD);
   *rh = h;
   *rs = s;

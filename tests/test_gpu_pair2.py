@@ -150,6 +150,8 @@ def test_two_sided_general_mass_matrix(oracle_lib):
     assert _close(E2, g2, E1, g1, 1e-13, 1e-11)
     Eo, go = oracle_lib.Oracle(t).outprod(x)
     assert _close(E2, g2, Eo, go, 1e-12, 1e-9)
+    # the element pass takes the general product too
+    assert abs(eng.energy(x) - E2) <= 1e-13 * max(abs(E2), 1.0)
 
 
 @pytest.mark.parametrize("K", [1000, 4096])
