@@ -35,28 +35,54 @@ namespace {
 
 // ------------------------------------------------------------------ prep ----
 
-// One lane per basis function k: the bra record.
+// One lane per basis function k: the bra record.  The workgroup's 64
+// parameter blocks are read, and its 64 records written, as contiguous runs
+// through LDS (coalesced).
 template <int N>
-__global__ void prep_bra(const double *__restrict__ x, int K, int with_c, SysConst sc,
-                         double *__restrict__ bra, int *__restrict__ flag) {
+__global__ void __launch_bounds__(64) prep_bra(const double *__restrict__ x, int K, int with_c,
+                                               SysConst sc, double *__restrict__ bra,
+                                               int *__restrict__ flag) {
   using D = Dim<N>;
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= K) return;
-  make_bra<N>(x + (size_t)k * D::BLK, with_c ? x[(size_t)K * D::BLK + k] : 0.0, with_c != 0,
-              sc, bra + (size_t)k * D::BS, flag);
+  constexpr int PS = D::BLK + 1, RS = D::BS + 1;  // odd row strides (LDS banks)
+  __shared__ double xin[64 * PS], rec[64 * RS];
+  const int k0 = blockIdx.x * 64, k = k0 + (int)threadIdx.x, nf = min(64, K - k0);
+  const double *xs = x + (size_t)k0 * D::BLK;
+  for (int i = threadIdx.x; i < nf * D::BLK; i += 64) xin[(i / D::BLK) * PS + i % D::BLK] = xs[i];
+  double *r = rec + threadIdx.x * RS;
+  for (int f = 0; f < D::BS; ++f) r[f] = 0.0;
+  __syncthreads();
+  if (k < K)
+    make_bra<N>(xin + threadIdx.x * PS, with_c ? x[(size_t)K * D::BLK + k] : 0.0, with_c != 0,
+                sc, r, flag);
+  __syncthreads();
+  double *out = bra + (size_t)k0 * D::BS;
+  for (int i = threadIdx.x; i < nf * D::BS; i += 64) out[i] = rec[(i / D::BS) * RS + i % D::BS];
 }
 
-// One lane per (ket l, term p): the transformed ket record.
+// One lane per (ket l, term p): the transformed ket record, built in LDS and
+// written out by the wave as one contiguous run of 64 records (coalesced).
 template <int N>
-__global__ void prep_ket(const double *__restrict__ bra, int K, int nst,
-                         const double *__restrict__ sml, const double *__restrict__ sms,
-                         const double *__restrict__ scvv, SysConst sc, double *__restrict__ ket) {
+__global__ void __launch_bounds__(64) prep_ket(const double *__restrict__ bra, int K, int nst,
+                                               const double *__restrict__ sml,
+                                               const double *__restrict__ sms,
+                                               const double *__restrict__ scvv, SysConst sc,
+                                               double *__restrict__ ket) {
   using D = Dim<N>;
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= K * nst) return;
-  const int l = idx / nst, p = idx - l * nst;
-  make_ket<N>(bra + (size_t)l * D::BS, sml + (size_t)p * N * N, sms + (size_t)p * D::N3 * D::N3,
-              scvv[p], sc, ket + (size_t)idx * D::KS);
+  constexpr int RS = D::KS + 1;  // odd row stride: the lanes' rows start in different banks
+  __shared__ double rec[64 * RS];
+  const int nk = K * nst;
+  const int idx0 = blockIdx.x * 64, idx = idx0 + (int)threadIdx.x;
+  double *r = rec + threadIdx.x * RS;
+  if (idx < nk) {
+    const int l = idx / nst, p = idx - l * nst;
+    make_ket<N>(bra + (size_t)l * D::BS, sml + (size_t)p * N * N,
+                sms + (size_t)p * D::N3 * D::N3, scvv[p], sc, r);
+  }
+  for (int f = D::K_TRMP + 1; f < D::KS; ++f) r[f] = 0.0;  // padding
+  __syncthreads();
+  const int n = min(64, nk - idx0) * D::KS;
+  double *out = ket + (size_t)idx0 * D::KS;
+  for (int i = threadIdx.x; i < n; i += 64) out[i] = rec[(i / D::KS) * RS + i % D::KS];
 }
 
 // ------------------------------------------------------------------ pair ----
@@ -148,14 +174,46 @@ __global__ void __launch_bounds__(256) tile_reduce2(const double *__restrict__ b
   tile_reduce2_block(bpart, kpart, K, kt, ts, nacc, blockIdx.y, blockIdx.x * 64, red, sh);
 }
 
-// Full symmetric H and S and the per-chunk partials of H C and S C.
-// Grid (ceil(K/256), K), 256 lanes.
+// Full symmetric H and S and the per-chunk (64 columns) partials of H C and
+// S C.  One 256-lane workgroup per block pair (I, J), I >= J: blockIdx.x
+// enumerates them row by row, (0,0), (1,0), (1,1), (2,0), ...
 __global__ void __launch_bounds__(256) hs_finish(
     const double *__restrict__ hpart, const double *__restrict__ spart, int K, int ts,
     const double *__restrict__ bra, int bs, int bc, double *__restrict__ H,
     double *__restrict__ S, double *__restrict__ hcp, double *__restrict__ scp) {
-  __shared__ double red[8];
-  hs_row_chunk(hpart, spart, K, ts, bra, bs, bc, blockIdx.y, blockIdx.x, H, S, hcp, scp, red);
+  __shared__ double bh[64 * 65], bsl[64 * 65], cc[128];
+  const int t = blockIdx.x;
+  int I = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+  while ((I + 1) * (I + 2) / 2 <= t) ++I;
+  while (I * (I + 1) / 2 > t) --I;
+  const int J = t - I * (I + 1) / 2;
+  hs_tile(hpart, spart, K, ts, bra, bs, bc, I, J, H, S, hcp, scp, bh, bsl, cc);
+}
+
+// H C and S C from the 64-column chunk partials: 64 rows per 256-lane
+// workgroup, wave q summing the chunks c = q mod 4, the four parts added in
+// order.
+__global__ void __launch_bounds__(256) hc_sum(const double *__restrict__ hcp,
+                                              const double *__restrict__ scp, int nchunk, int K,
+                                              double *__restrict__ HC, double *__restrict__ SC) {
+  __shared__ double ph[256], ps[256];
+  const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
+  const int i = blockIdx.x * 64 + lane;
+  double a = 0.0, b = 0.0;
+  if (i < K) {
+#pragma unroll 4
+    for (int c = q; c < nchunk; c += 4) {
+      a += hcp[(size_t)c * K + i];
+      b += scp[(size_t)c * K + i];
+    }
+  }
+  ph[threadIdx.x] = a;
+  ps[threadIdx.x] = b;
+  __syncthreads();
+  if (q == 0 && i < K) {
+    HC[i] = ((ph[lane] + ph[64 + lane]) + ph[128 + lane]) + ph[192 + lane];
+    SC[i] = ((ps[lane] + ps[64 + lane]) + ps[128 + lane]) + ps[192 + lane];
+  }
 }
 
 __global__ void __launch_bounds__(1024) quotient(
@@ -174,13 +232,27 @@ __global__ void __launch_bounds__(256) tile_reduce(const double *__restrict__ pa
   if (k < K) tile_reduce_one(part, ntiles, K, nacc, blockIdx.y, k, red);
 }
 
+// One lane per function; the 64 blocks of a workgroup are formed in LDS and
+// written out as one contiguous run (coalesced).
 template <int N, int MODE>
-__global__ void grad_final(const double *__restrict__ red, int K,
-                           const double *__restrict__ bra, const double *__restrict__ scal,
-                           const double *__restrict__ HC, const double *__restrict__ SC,
-                           const double *__restrict__ gbar, double *__restrict__ g) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < K) grad_one<N, MODE>(red, K, k, bra, scal, HC, SC, gbar ? gbar[0] : 1.0, g);
+__global__ void __launch_bounds__(64) grad_final(const double *__restrict__ red, int K,
+                                                 const double *__restrict__ bra,
+                                                 const double *__restrict__ scal,
+                                                 const double *__restrict__ HC,
+                                                 const double *__restrict__ SC,
+                                                 const double *__restrict__ gbar,
+                                                 double *__restrict__ g) {
+  using D = Dim<N>;
+  constexpr int RS = D::BLK + 1;  // odd row stride: the lanes' rows start in different banks
+  __shared__ double go[64 * RS];
+  const int k0 = blockIdx.x * 64, k = k0 + (int)threadIdx.x;
+  if (k < K)
+    grad_one<N, MODE>(red, K, k, bra, scal, HC, SC, gbar ? gbar[0] : 1.0, g,
+                      go + threadIdx.x * RS);
+  __syncthreads();
+  const int n = min(64, K - k0) * D::BLK;
+  double *out = g + (size_t)k0 * D::BLK;
+  for (int i = threadIdx.x; i < n; i += 64) out[i] = go[(i / D::BLK) * RS + i % D::BLK];
 }
 
 // ----------------------------------------------------------- queries -----
@@ -349,7 +421,7 @@ int ensure_K(necg_ctx *c, int K) {
   if ((rc = dev_alloc(&c->HC, k))) return rc;
   if ((rc = dev_alloc(&c->SC, k))) return rc;
   if ((rc = dev_alloc(&c->scal, 8))) return rc;
-  const size_t nchunk = (k + 255) / 256;
+  const size_t nchunk = (k + 63) / 64;  // H C / S C partials per 64 columns
   if ((rc = dev_alloc(&c->hcp, nchunk * k))) return rc;
   if ((rc = dev_alloc(&c->scp, nchunk * k))) return rc;
   if ((rc = dev_alloc(&c->red, (size_t)Dim<kMaxN>::NACC * k))) return rc;
@@ -400,12 +472,11 @@ void choose_tiling(necg_ctx *c, int K, int &kt, int &ts, int &w) {
 
 template <int N>
 int run_prep(necg_ctx *c, int K, const double *x, int with_c, hipStream_t st) {
-  const int tb = 256;
-  hipLaunchKernelGGL((prep_bra<N>), dim3((K + tb - 1) / tb), dim3(tb), 0, st, x, K, with_c,
-                     c->sc, c->bra, c->d_flag);
+  hipLaunchKernelGGL((prep_bra<N>), dim3((K + 63) / 64), dim3(64), 0, st, x, K, with_c, c->sc,
+                     c->bra, c->d_flag);
   const int nk = K * c->nst;
-  hipLaunchKernelGGL((prep_ket<N>), dim3((nk + tb - 1) / tb), dim3(tb), 0, st, c->bra, K,
-                     c->nst, c->d_sml, c->d_sms, c->d_scvv, c->sc, c->ket);
+  hipLaunchKernelGGL((prep_ket<N>), dim3((nk + 63) / 64), dim3(64), 0, st, c->bra, K, c->nst,
+                     c->d_sml, c->d_sms, c->d_scvv, c->sc, c->ket);
   return hip_status(hipGetLastError());
 }
 
@@ -463,15 +534,19 @@ int run_pairs(necg_ctx *c, int K, hipStream_t st, int &ntiles, int &ts_out) {
 int finish_hs(necg_ctx *c, int K, int ts, hipStream_t st, double *H = nullptr,
               double *S = nullptr) {
   const int bs = bra_stride(c->n), bc = bra_c_off(c->n);
-  hipLaunchKernelGGL(hs_finish, dim3((K + 255) / 256, K), dim3(256), 0, st, c->hpart, c->spart,
+  const int nb = (K + 63) / 64;
+  hipLaunchKernelGGL(hs_finish, dim3(nb * (nb + 1) / 2), dim3(256), 0, st, c->hpart, c->spart,
                      K, ts, c->bra, bs, bc, H ? H : c->H, S ? S : c->S, c->hcp, c->scp);
   return hip_status(hipGetLastError());
 }
 
 int finish_quotient(necg_ctx *c, int K, double *f, hipStream_t st) {
   const int bs = bra_stride(c->n), bc = bra_c_off(c->n);
-  hipLaunchKernelGGL(quotient, dim3(1), dim3(1024), 0, st, c->hcp, c->scp, (K + 255) / 256,
-                     c->bra, bs, bc, K, c->HC, c->SC, c->scal, f, c->d_flag);
+  hipLaunchKernelGGL(hc_sum, dim3((K + 63) / 64), dim3(256), 0, st, c->hcp, c->scp,
+                     (K + 63) / 64, K, c->HC, c->SC);
+  // the quotient over the summed vectors (one chunk: H C and S C themselves)
+  hipLaunchKernelGGL(quotient, dim3(1), dim3(1024), 0, st, c->HC, c->SC, 1, c->bra, bs, bc, K,
+                     nullptr, nullptr, c->scal, f, c->d_flag);
   return hip_status(hipGetLastError());
 }
 
@@ -481,7 +556,7 @@ int finish_grad(necg_ctx *c, int K, int ntiles, const double *gbar, double *g, h
   const int nv = (MODE == MODE_SPLIT) ? D::NACC : D::A_SH + 1;  // MODE 1 uses slots < A_XS and A_SH
   hipLaunchKernelGGL(tile_reduce, dim3((K + 255) / 256, nv), dim3(256), 0, st, c->part, ntiles, K,
                      D::NACC, c->red);
-  hipLaunchKernelGGL((grad_final<N, MODE>), dim3((K + 255) / 256), dim3(256), 0, st, c->red, K,
+  hipLaunchKernelGGL((grad_final<N, MODE>), dim3((K + 63) / 64), dim3(64), 0, st, c->red, K,
                      c->bra, c->scal, c->HC, c->SC, gbar, g);
   return hip_status(hipGetLastError());
 }
@@ -635,7 +710,7 @@ int finish_grad2(necg_ctx *c, int K, int kt, int ts, const double *gbar, double 
   using D = Dim<N>;
   hipLaunchKernelGGL(tile_reduce2, dim3((K + 63) / 64, D::NACC), dim3(256), 0, st, c->part,
                      c->kpart, K, kt, ts, D::NACC, c->red);
-  hipLaunchKernelGGL((grad_final<N, MODE_SPLIT>), dim3((K + 255) / 256), dim3(256), 0, st, c->red,
+  hipLaunchKernelGGL((grad_final<N, MODE_SPLIT>), dim3((K + 63) / 64), dim3(64), 0, st, c->red,
                      K, c->bra, c->scal, c->HC, c->SC, gbar, g);
   return hip_status(hipGetLastError());
 }

@@ -234,8 +234,79 @@ __device__ __forceinline__ void hs_row_chunk(const double *__restrict__ hpart,
   }
 }
 
-// HC, SC from the chunk partials, then C^T H C, C^T S C, E = cHc / cSc and
-// 1/cSc (python/outprod.py:65-68).  One workgroup of 1024 lanes per system;
+// Full symmetric H and S for one pair of 64 x 64 blocks (I, J), I >= J, and
+// their H C / S C chunk partials, by a 256-lane workgroup.  The term-group
+// partials hold element (i, j) at [min(i, j)][max(i, j)] (the reference
+// mirrors the lower triangle, python/outprod.py:57-60), so block [J][I] of
+// the partials is read row by row (coalesced), summed over the term groups,
+// written to H[J][I] as read and, transposed through LDS, to H[I][J].
+// hcp[c * K + i] = sum over the 64 columns j of chunk c of H[i][j] C_j; each
+// (chunk, row) has one writer, and every sum runs in a fixed order.
+// bh, bsl: 64 x 65 LDS blocks; cc: 128 LDS doubles.
+__device__ __forceinline__ void hs_tile(const double *__restrict__ hpart,
+                                        const double *__restrict__ spart, int K, int ts,
+                                        const double *__restrict__ bra, int bs, int bc, int I,
+                                        int J, double *__restrict__ H, double *__restrict__ S,
+                                        double *__restrict__ hcp, double *__restrict__ scp,
+                                        double *bh, double *bsl, double *cc) {
+  constexpr int LD = 65;
+  const int tid = threadIdx.x, c = tid & 63, r0 = tid >> 6;
+  const int i0 = 64 * I, j0 = 64 * J;
+  const bool diag = I == J;
+  if (tid < 128) {
+    const int j = (tid < 64 ? i0 : j0) + (tid & 63);
+    cc[tid] = j < K ? bra[(size_t)j * bs + bc] : 0.0;
+  }
+#pragma unroll 4
+  for (int r = r0; r < 64; r += 4) {
+    const int row = j0 + r, col = i0 + c;
+    double h = 0.0, s = 0.0;
+    if (row < K && col < K && (!diag || r <= c)) {
+      for (int z = 0; z < ts; ++z) {
+        const size_t o = ((size_t)z * K + row) * K + col;
+        h += hpart[o];
+        s += spart[o];
+      }
+    }
+    bh[r * LD + c] = h;
+    bsl[r * LD + c] = s;
+  }
+  __syncthreads();
+#pragma unroll 4
+  for (int r = r0; r < 64; r += 4) {
+    const int row = j0 + r, col = i0 + c;
+    if (row < K && col < K) {
+      const int a = diag ? min(r, c) : r, b = diag ? max(r, c) : c;
+      H[(size_t)row * K + col] = bh[a * LD + b];
+      S[(size_t)row * K + col] = bsl[a * LD + b];
+    }
+    const int row2 = i0 + r, col2 = j0 + c;
+    if (!diag && row2 < K && col2 < K) {
+      H[(size_t)row2 * K + col2] = bh[c * LD + r];
+      S[(size_t)row2 * K + col2] = bsl[c * LD + r];
+    }
+  }
+  // lanes 0-127: rows j0 + r over chunk I (H, then S); lanes 128-255: rows
+  // i0 + r over chunk J (off the diagonal only)
+  const int q = tid >> 6, r = tid & 63;
+  const double *blk = (q & 1) ? bsl : bh;
+  double t = 0.0;
+  if (q < 2) {
+    if (j0 + r < K) {
+      for (int k = 0; k < 64; ++k) {
+        const int a = diag ? min(r, k) : r, b = diag ? max(r, k) : k;
+        t = __builtin_fma(blk[a * LD + b], cc[k], t);
+      }
+      ((q & 1) ? scp : hcp)[(size_t)I * K + j0 + r] = t;
+    }
+  } else if (!diag && i0 + r < K) {
+    for (int k = 0; k < 64; ++k) t = __builtin_fma(blk[k * LD + r], cc[64 + k], t);
+    ((q & 1) ? scp : hcp)[(size_t)J * K + i0 + r] = t;
+  }
+}
+
+// HC, SC from the chunk partials (not stored where HC, SC are null), then
+// C^T H C, C^T S C, E = cHc / cSc and 1/cSc (python/outprod.py:65-68).  One workgroup of 1024 lanes per system;
 // rh/rs are 1024-entry LDS arrays.
 __device__ __forceinline__ void quotient_block(const double *__restrict__ hcp,
                                                const double *__restrict__ scp, int nchunk,
@@ -247,12 +318,13 @@ __device__ __forceinline__ void quotient_block(const double *__restrict__ hcp,
   double h = 0.0, s = 0.0;
   for (int i = threadIdx.x; i < K; i += blockDim.x) {
     double a = 0.0, b = 0.0;
+#pragma unroll 8
     for (int c = 0; c < nchunk; ++c) {
       a += hcp[(size_t)c * K + i];
       b += scp[(size_t)c * K + i];
     }
-    HC[i] = a;
-    SC[i] = b;
+    if (HC) HC[i] = a;
+    if (SC) SC[i] = b;
     const double ci = bra[(size_t)i * bs + bc];
     h = __builtin_fma(ci, a, h);
     s = __builtin_fma(ci, b, s);
@@ -294,14 +366,15 @@ __device__ __forceinline__ void tile_reduce_one(const double *__restrict__ part,
 // a = 2 C_k [vech(X_H L) + 1.5 sigma_H diag(1/L) | A Y_H]
 // (python/make_grad.py:61-79); g_C = 2 norm (HC - E SC) (:85).  gb scales
 // the whole gradient (vector-Jacobian product).  g points at the system's x
-// layout.
+// layout; the function's parameter block goes to gk when given (a staging
+// row), else to its place in g.
 template <int N, int MODE>
 __device__ __forceinline__ void grad_one(const double *__restrict__ red, int K, int k,
                                          const double *__restrict__ bra,
                                          const double *__restrict__ scal,
                                          const double *__restrict__ HC,
                                          const double *__restrict__ SC, double gb,
-                                         double *__restrict__ g) {
+                                         double *__restrict__ g, double *gk = nullptr) {
   using D = Dim<N>;
   double a[D::NACC];
 #pragma unroll
@@ -313,7 +386,7 @@ __device__ __forceinline__ void grad_one(const double *__restrict__ red, int K, 
   double oh[D::BLK];
   grad_L<N>(a + D::A_XH, a[D::A_SH], r + D::B_L, r + D::B_LD, oh);
   grad_S<N>(r + D::B_A, a + D::A_YH, oh + D::NS);
-  double *gk = g + (size_t)k * D::BLK;
+  if (!gk) gk = g + (size_t)k * D::BLK;
   if (MODE == MODE_SPLIT) {
     double os[D::BLK];
     grad_L<N>(a + D::A_XS, a[D::A_SS], r + D::B_L, r + D::B_LD, os);
