@@ -15,8 +15,8 @@
 // derivative w.r.t. (A', S') has the bra-side form with A <-> A',
 // dif1 <-> dif2 and 12 A'MA' <-> 12 A M A in Gam.  With a1 = A dif1
 // (A' dif2 = -a1) and Fc = Lb st - efs q e_z^T, B^-1 A' = I - B^-1 A gives
-//   Phi_bra = B^-1 (8 A'M a1 + Fc) = B^-1 Fc + 8 (M a1 - w),
-//   Phi_ket = B^-1 (-8 A M a1 + Fc) = B^-1 Fc - 8 w,       w = B^-1 A M a1,
+//   Phi_ket = B^-1 (Fc - 8 A M a1),
+//   Phi_bra = B^-1 (Fc + 8 A'M a1) = Phi_ket + 8 M a1,
 // and the kinetic trace 3 tr(M A B^-1 A') = 3 (tr(M A) - <A M A, B^-1>).
 // The ket-side (X', Y') are in the frame of A' = P^T A_l P, S' = P^T S_l; in
 // the ket's own frame they are P X' P^T and P Y', a relabelling of slots that
@@ -334,7 +334,7 @@ __device__ __forceinline__ bool pair_term2(const BraLds<N> &b, const double *__r
         const double inv2 = ri * ri;
         const double z = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx)) * inv2;
         double f, f2p;
-        boys_f0_line(z, f, f2p);
+        boys_f0_line<false>(z, f, f2p);
         const double cr = sc_.cpair2[pi] * ri;   // 2 c / sqrt(pi i2p)
         csum2 = __builtin_fma(cr, f, csum2);
         const double h = cr * inv2;
@@ -361,10 +361,11 @@ __device__ __forceinline__ bool pair_term2(const BraLds<N> &b, const double *__r
   hel = __builtin_fma(scv * ov, alpha, hel);
   sel = __builtin_fma(scv, ov, sel);
 
-  // Shared gradient pieces: Phic = B^-1 (Lb st - efs q e_z^T), w = B^-1 A M a1.
-  double Phic[N3], w[N3];
+  // Shared gradient piece: Phk = B^-1 (Lb st - efs q e_z^T - 8 A M a1), the
+  // ket side's Phi; the bra side's is Phk + 8 M a1.
+  double Phk[N3];
   {
-    double Fc[N3], v[N3];
+    double G[N3];
 #pragma unroll
     for (int i = 0; i < N; ++i)
 #pragma unroll
@@ -376,30 +377,25 @@ __device__ __forceinline__ bool pair_term2(const BraLds<N> &b, const double *__r
           u = __builtin_fma(b.A(sx(i, m)), Ma1[3 * m + x], u);
         }
         if (x == 2) t = __builtin_fma(-sc_.efs, sc_.q[i + 1], t);
-        Fc[3 * i + x] = t;
-        v[3 * i + x] = u;
+        G[3 * i + x] = __builtin_fma(-8.0, u, t);
       }
 #pragma unroll
     for (int i = 0; i < N; ++i)
 #pragma unroll
       for (int x = 0; x < 3; ++x) {
-        double t = 0.0, u = 0.0;
+        double t = 0.0;
 #pragma unroll
-        for (int m = 0; m < N; ++m) {
-          t = __builtin_fma(Bi[sx(i, m)], Fc[3 * m + x], t);
-          u = __builtin_fma(Bi[sx(i, m)], v[3 * m + x], u);
-        }
-        Phic[3 * i + x] = t;
-        w[3 * i + x] = u;
+        for (int m = 0; m < N; ++m) t = __builtin_fma(Bi[sx(i, m)], G[3 * m + x], t);
+        Phk[3 * i + x] = t;
       }
   }
 
-  // Bra side: Phi = Phic + 8 (M a1 - w), G = B^-1 (12 A'MA' + La) B^-1.
+  // Bra side: Phi = Phk + 8 M a1, G = B^-1 (12 A'MA' + La) B^-1.
   {
     double Phi[N3], d1[N3];
 #pragma unroll
     for (int t = 0; t < N3; ++t) {
-      Phi[t] = __builtin_fma(8.0, Ma1[t] - w[t], Phic[t]);
+      Phi[t] = __builtin_fma(8.0, Ma1[t], Phk[t]);
       d1[t] = st[t] - b.S(t);
     }
     double Tm[N * N];
@@ -443,16 +439,13 @@ __device__ __forceinline__ bool pair_term2(const BraLds<N> &b, const double *__r
     sink.add(D::A_SS, wo);
   }
 
-  // Ket side: Phi' = Phic - 8 w, G' = B^-1 (12 A M A + La) B^-1, dif2 = st - S'.
+  // Ket side: Phi' = Phk, G' = B^-1 (12 A M A + La) B^-1, dif2 = st - S'.
   {
     const double wk = mk * scv * b.f(BraLds<N>::F_C);   // scvv_P C_k
     double kv[Two<N>::KV];
-    double Phk[N3], d2[N3];
+    double d2[N3];
 #pragma unroll
-    for (int t = 0; t < N3; ++t) {
-      Phk[t] = __builtin_fma(-8.0, w[t], Phic[t]);
-      d2[t] = st[t] - Sp[t];
-    }
+    for (int t = 0; t < N3; ++t) d2[t] = st[t] - Sp[t];
     double Tm[N * N];
 #pragma unroll
     for (int i = 0; i < N; ++i)
