@@ -371,7 +371,8 @@ def main():
     import nbo_correlatedgaussians_amd as necg
     from nbo_correlatedgaussians_amd.engine import Engine
     from nbo_correlatedgaussians_amd.flops import (flops_per_eval, flops_per_eval_v0,
-                                                   pairs_per_eval, two_sided_term_flops)
+                                                   mass_is_rank1, pairs_per_eval,
+                                                   two_sided_term_flops)
 
     sysd = necg.lih_system()
     K = args.K
@@ -412,7 +413,8 @@ def main():
     pairs = pairs_per_eval(K)
     value = pairs / (ms_step / 1000.0)
     two_sided = eng.pair_form()[1]
-    fl = flops_per_eval(sysd.n, sysd.nst, K)
+    rank1 = mass_is_rank1(sysd.mass.tolist())
+    fl = flops_per_eval(sysd.n, sysd.nst, K, rank1)
     fl_v0 = flops_per_eval_v0(sysd.n, sysd.nst, K)
     pair_ms = ms_pair / max(launches, 1)
     achieved = fl / (pair_ms / 1000.0) / 1e12
@@ -445,8 +447,10 @@ def main():
                      "traffic": (recorded_traffic()
                                  if (K == 2048 and args.seed == 1 and two_sided) else None),
                      "traffic_source": "profiles/pair2_kernel_lih_k2048_traffic.md",
-                     "flop_model": (f"two-sided n-space op count, {two_sided_term_flops(sysd.n)} "
-                                    f"FLOP per (pair, term) (flops.two_sided_term)"),
+                     "flop_model": (f"two-sided n-space op count, "
+                                    f"{two_sided_term_flops(sysd.n, rank1=rank1)} FLOP per "
+                                    f"(pair, term) (flops.two_sided_term"
+                                    f"{', diagonal-plus-constant mass' if rank1 else ''})"),
                      "flop_per_launch": fl, "pair_kernel_ms": pair_ms,
                      "v0_model_flop_per_launch": fl_v0,
                      "v0_model_tflops": fl_v0 / (pair_ms / 1000.0) / 1e12,

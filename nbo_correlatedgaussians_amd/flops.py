@@ -12,10 +12,11 @@ Two models:
   nst (F_E + F_G) on it (LiH: 413,856 FLOP per off-diagonal pair).
 * ``flops_per_eval``: the two-sided algorithm the pair kernel runs: per
   unordered (pair, term) one factorisation, the elements, the Coulomb and
-  Stark derivatives, and both gradient halves (``two_sided_term``; LiH:
-  4,609 FLOP per pair-term, 110,616 per off-diagonal pair).  The identities
+  Stark derivatives, and both gradient halves (``two_sided_term``; LiH,
+  whose mass matrix is diagonal plus a constant: 4,339 FLOP per pair-term,
+  104,136 per off-diagonal pair).  The identities
   it uses (A' dif2 = -A dif1, B^-1 A' = I - B^-1 A, the collapsed kinetic and
-  Coulomb gradient products) make it 3.7x smaller than the v0 model for LiH,
+  Coulomb gradient products) make it 4.0x smaller than the v0 model for LiH,
   so the v0 model overstates the work of the run and is not the roofline
   basis.
 """
@@ -31,12 +32,24 @@ def flops_per_eval_v0(n: int, nst: int, K: int) -> float:
     return float(nst * (off * (F_E[n] + 2 * F_G[n]) + K * (F_E[n] + F_G[n])))
 
 
-def two_sided_term(n: int) -> dict:
+def mass_is_rank1(mass) -> bool:
+    """True when every off-diagonal entry of the mass matrix is the same (the
+    matrix is diagonal plus a constant; the kernels then form M a1 from one
+    column sum)."""
+    n = len(mass)
+    if n < 2:
+        return True
+    c = mass[0][1]
+    return all(mass[i][j] == c for i in range(n) for j in range(n) if i != j)
+
+
+def two_sided_term(n: int, rank1: bool = False) -> dict:
     """FLOP of one unordered (bra, ket, term) evaluation of the two-sided
     pair term, by phase, for n pseudo-particles (ns = n(n+1)/2 packed
     entries, n3 = 3n shift entries, n(n+1)/2 Coulomb pairs of which n involve
     the reference particle).  'ket' is the ket-side gradient, which the
-    diagonal pairs (k = l) do not need."""
+    diagonal pairs (k = l) do not need.  rank1: the mass matrix is diagonal
+    plus a constant, and M a1 costs a column sum, 3 products and n3 FMAs."""
     ns, n3 = n * (n + 1) // 2, 3 * n
     chol = ns                                            # B = A + A'
     for j in range(n):                                   # Cholesky, 1/R_jj by rsq
@@ -45,35 +58,38 @@ def two_sided_term(n: int) -> dict:
     inv += sum((i + 1) * 2 * (n - i) for i in range(n))                     # B^-1 = R^-T R^-1
     det = 3 * n                                          # det(B)^-3/2 from the pivots
     elem = n3 + 2 * n3 * n + 2 * n3 + 6                  # E, st = B^-1 E, i3, ov
-    kin = n3 + 2 * n3 * n + 2 * n3 * n + 2 * n3          # dif1, a1 = A dif1, M a1, i3k
+    ma1 = 3 * (n - 1) + 3 + 2 * n3 if rank1 else 2 * n3 * n
+    kin = n3 + 2 * n3 * n + ma1 + 2 * n3                 # dif1, a1 = A dif1, M a1, i3k
     kin += 2 * ns + (ns - n) + 4                         # <AMA, B^-1>, tfac
     ref_pair = 2 + 1 + 6 + 51 + 8                        # (0, q): rsq, 1/i2, z, Boys F0/2F0', ca cb
     pp_pair = ref_pair + 3 + 3 + 4                       # (p, q): i2, dif, 4 diagonal updates
     coul = n * ref_pair + (ns - n) * pp_pair
     stark = 2 * n + 4 + 3 + 2                            # zsum, alpha, H and S sums
-    shared = 2 * n3 * n + 2 * n + 3 * 2 * n3 * n         # Lb st, Stark column, A M a1, B^-1 F, B^-1 v
-    side = (n3 + 2 * n3                                  # dif, Phi from the shared parts
+    shared = 2 * n3 * n + 2 * n + 2 * n3 * n + 2 * n3 + 2 * n3 * n   # Lb st, Stark column, A M a1,
+    #                                                      F - 8 A M a1, B^-1 F (the ket side's Phi)
+    side = (n3                                           # dif
             + 2 * ns + 2 * n * n * n                     # 12 A'MA' + La, times B^-1
             + ns * (2 * n + 6 + 12 + 3 + 4 + 1)          # G, dd, dp, x0, X_H, X_S
             + 3 + n3 * 6                                 # weights, Y_H, Y_S
             + (2 * ns + 2 * n3 + 2))                     # accumulation of the values
     return {"factorisation": chol + inv + det, "elements": elem, "kinetic": kin,
             "coulomb": coul, "stark": stark, "shared_gradient": shared,
-            "bra": side + n3,                            # Phi = Phic + 8 (M a1 - w)
+            "bra": side + 2 * n3,                        # Phi = Phi_ket + 8 M a1
             "ket": side}
 
 
-def two_sided_term_flops(n: int, diagonal: bool = False) -> int:
-    ph = two_sided_term(n)
+def two_sided_term_flops(n: int, diagonal: bool = False, rank1: bool = False) -> int:
+    ph = two_sided_term(n, rank1)
     return sum(ph.values()) - (ph["ket"] if diagonal else 0)
 
 
-def flops_per_eval(n: int, nst: int, K: int) -> float:
+def flops_per_eval(n: int, nst: int, K: int, rank1: bool = False) -> float:
     """FLOP of one energy+gradient evaluation by the two-sided algorithm:
     nst terms for each of the K(K-1)/2 off-diagonal pairs (both gradient
     halves) and the K diagonal pairs (bra half only)."""
     off = K * (K - 1) // 2
-    return float(nst * (off * two_sided_term_flops(n) + K * two_sided_term_flops(n, True)))
+    return float(nst * (off * two_sided_term_flops(n, rank1=rank1)
+                        + K * two_sided_term_flops(n, True, rank1)))
 
 
 def pairs_per_eval(K: int) -> int:

@@ -117,3 +117,25 @@ def test_generator_template_statistics():
     assert np.all(b[:, diag_idx] > 0)
     assert np.allclose(b.mean(axis=0), mean, atol=4 * sd.max() / np.sqrt(20000) + 1e-12)
     assert np.allclose(b.std(axis=0), np.where(sd < 1e-10, 1e-6, sd), rtol=0.05)
+
+
+def test_mass_rank1_detection_and_op_count():
+    """The op count follows the instance the kernels pick: LiH's, He's and
+    H2's mass matrices are diagonal plus a constant; a matrix with unequal
+    off-diagonal entries is not."""
+    import numpy as np
+    import nbo_correlatedgaussians_amd as necg
+    from nbo_correlatedgaussians_amd.flops import (mass_is_rank1, two_sided_term,
+                                                   two_sided_term_flops)
+    for s in (necg.lih_system(), necg.he_system(), necg.h2_system()):
+        assert mass_is_rank1(s.mass.tolist()), s.name
+    m = necg.lih_system().mass.copy()
+    m[0, 1:] *= 1.7
+    m[1:, 0] *= 1.7
+    assert not mass_is_rank1(m.tolist())
+    full, r1 = two_sided_term(5), two_sided_term(5, True)
+    # only the kinetic phase differs: M a1 as 75 FMAs against 12 adds, 3 products, 15 FMAs
+    assert {k for k in full if full[k] != r1[k]} == {"kinetic"}
+    assert full["kinetic"] - r1["kinetic"] == 150 - 45
+    assert two_sided_term_flops(5, rank1=True) == 4339
+    assert two_sided_term_flops(5, True, True) == 4339 - r1["ket"]
