@@ -35,28 +35,17 @@ namespace {
 
 // ------------------------------------------------------------------ prep ----
 
-// One lane per basis function k: the bra record.  The workgroup's 64
-// parameter blocks are read, and its 64 records written, as contiguous runs
-// through LDS (coalesced).
+// One lane per basis function k: the bra record.  (Staging the parameter
+// blocks and records through LDS, as prep_ket does, measured slower here:
+// the work per lane, not the access pattern, sets this kernel's time.)
 template <int N>
-__global__ void __launch_bounds__(64) prep_bra(const double *__restrict__ x, int K, int with_c,
-                                               SysConst sc, double *__restrict__ bra,
-                                               int *__restrict__ flag) {
+__global__ void prep_bra(const double *__restrict__ x, int K, int with_c, SysConst sc,
+                         double *__restrict__ bra, int *__restrict__ flag) {
   using D = Dim<N>;
-  constexpr int PS = D::BLK + 1, RS = D::BS + 1;  // odd row strides (LDS banks)
-  __shared__ double xin[64 * PS], rec[64 * RS];
-  const int k0 = blockIdx.x * 64, k = k0 + (int)threadIdx.x, nf = min(64, K - k0);
-  const double *xs = x + (size_t)k0 * D::BLK;
-  for (int i = threadIdx.x; i < nf * D::BLK; i += 64) xin[(i / D::BLK) * PS + i % D::BLK] = xs[i];
-  double *r = rec + threadIdx.x * RS;
-  for (int f = 0; f < D::BS; ++f) r[f] = 0.0;
-  __syncthreads();
-  if (k < K)
-    make_bra<N>(xin + threadIdx.x * PS, with_c ? x[(size_t)K * D::BLK + k] : 0.0, with_c != 0,
-                sc, r, flag);
-  __syncthreads();
-  double *out = bra + (size_t)k0 * D::BS;
-  for (int i = threadIdx.x; i < nf * D::BS; i += 64) out[i] = rec[(i / D::BS) * RS + i % D::BS];
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= K) return;
+  make_bra<N>(x + (size_t)k * D::BLK, with_c ? x[(size_t)K * D::BLK + k] : 0.0, with_c != 0,
+              sc, bra + (size_t)k * D::BS, flag);
 }
 
 // One lane per (ket l, term p): the transformed ket record, built in LDS and
@@ -472,8 +461,8 @@ void choose_tiling(necg_ctx *c, int K, int &kt, int &ts, int &w) {
 
 template <int N>
 int run_prep(necg_ctx *c, int K, const double *x, int with_c, hipStream_t st) {
-  hipLaunchKernelGGL((prep_bra<N>), dim3((K + 63) / 64), dim3(64), 0, st, x, K, with_c, c->sc,
-                     c->bra, c->d_flag);
+  hipLaunchKernelGGL((prep_bra<N>), dim3((K + 255) / 256), dim3(256), 0, st, x, K, with_c,
+                     c->sc, c->bra, c->d_flag);
   const int nk = K * c->nst;
   hipLaunchKernelGGL((prep_ket<N>), dim3((nk + 63) / 64), dim3(64), 0, st, c->bra, K, c->nst,
                      c->d_sml, c->d_sms, c->d_scvv, c->sc, c->ket);
