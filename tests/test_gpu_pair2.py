@@ -171,3 +171,19 @@ def test_two_sided_large_k(K):
     assert _close(E2, g2, E1, g1, 1e-13, 1e-11)
     gC = g2[K * s.blk:]
     assert abs(np.dot(C, gC)) <= 1e-10 * np.linalg.norm(C) * max(np.linalg.norm(gC), 1.0)
+
+
+def test_fp64_fma_rate_probe():
+    """The FMA rate probe reports a rate below MI355X's published FP64 peak
+    (78.6 TFLOP/s) and well above any rate the pair kernels reach when the
+    device is healthy, and agrees with itself from run to run."""
+    import ctypes as ct
+    lib = _lib.load()
+    rates = []
+    for _ in range(2):
+        r = ct.c_double(0.0)
+        assert lib.necg_fp64_fma_rate(0, 50000, ct.byref(r)) == _lib.NECG_OK
+        rates.append(r.value)
+    print(f"fp64 fma rate {rates}", flush=True)
+    assert all(20.0 < v < 78.6 * 1.02 for v in rates), rates
+    assert abs(rates[0] - rates[1]) <= 0.15 * max(rates)   # clocks settle between runs
