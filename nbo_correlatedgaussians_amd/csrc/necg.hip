@@ -96,13 +96,15 @@ __global__ void __launch_bounds__(64 * pair_waves(N), NECG_MIN_WAVES) pair_kerne
     double *__restrict__ hpart, double *__restrict__ spart, int *__restrict__ flag) {
   using D = Dim<N>;
   __shared__ double acc_s[PairLds<N, MODE>::ACC];
+  __shared__ double kst_s[pair_waves(N) * D::KS];
   const int l0 = blockIdx.y * kt;
   const int p0 = blockIdx.z * tps;
   const size_t tile = (size_t)blockIdx.y * gridDim.z + blockIdx.z;
-  pair_tile<N, MODE, R1>(bra, ket, K, nst, blockIdx.x * 64, l0, min(K, l0 + kt), p0,
-                     min(nst, p0 + tps), sc, (MODE == MODE_COMBINED) ? scal[0] : 0.0,
-                     part + tile * D::NACC * K, hpart + (size_t)blockIdx.z * K * K,
-                     spart + (size_t)blockIdx.z * K * K, flag, acc_s);
+  pair_tile<N, MODE, R1, true>(bra, ket, K, nst, blockIdx.x * 64, l0, min(K, l0 + kt), p0,
+                               min(nst, p0 + tps), sc, (MODE == MODE_COMBINED) ? scal[0] : 0.0,
+                               part + tile * D::NACC * K, hpart + (size_t)blockIdx.z * K * K,
+                               spart + (size_t)blockIdx.z * K * K, flag, acc_s, 0x7fffffff,
+                               kst_s);
 }
 
 // Two-sided pair pass over the lower triangle.  blockIdx.x indexes the tile
@@ -143,14 +145,16 @@ __global__ void __launch_bounds__(64 * pair_waves(N), NECG_MIN_WAVES) pair_tri_k
     double *__restrict__ spart, int *__restrict__ flag) {
   static_assert(!mode_has_grad(MODE), "element modes only");
   __shared__ double acc_s[PairLds<N, MODE>::ACC];
+  __shared__ double kst_s[pair_waves(N) * Dim<N>::KS];
   const int bx = tiles[2 * blockIdx.x], by = tiles[2 * blockIdx.x + 1];
   const int k0 = bx * 64, l0 = by * kt;
   const int up = tri_up(K, kt), lup = k0 < up ? up : K;
   const int l1 = l0 < lup ? min(min(K, l0 + kt), k0 + 64) : min(K, l0 + kt);
   const int z = blockIdx.y;
   const int p0 = z * tps;
-  pair_tile<N, MODE, R1>(bra, ket, K, nst, k0, l0, l1, p0, min(nst, p0 + tps), sc, 0.0, nullptr,
-                     hpart + (size_t)z * K * K, spart + (size_t)z * K * K, flag, acc_s, lup);
+  pair_tile<N, MODE, R1, true>(bra, ket, K, nst, k0, l0, l1, p0, min(nst, p0 + tps), sc, 0.0,
+                               nullptr, hpart + (size_t)z * K * K, spart + (size_t)z * K * K,
+                               flag, acc_s, lup, kst_s);
 }
 
 // Two-sided partial sums per (function, slot).  Grid (ceil(K/64), nv),

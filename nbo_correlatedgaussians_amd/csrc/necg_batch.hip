@@ -117,16 +117,17 @@ __global__ void __launch_bounds__(64 * pair_waves(N), 1) b_pair(
     double *__restrict__ spart, int *__restrict__ flag) {
   using D = Dim<N>;
   __shared__ double acc_s[PairLds<N, MODE_SPLIT>::ACC];
+  __shared__ double kst_s[pair_waves(N) * D::KS];
   for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
     const Item w = items[it];
     if (active && !active[w.sys]) continue;  // uniform over the workgroup
     const BatchSys &b = sys[w.sys];
     const int K = b.K;
     const int z = w.tile % b.ts;
-    pair_tile<N, MODE_SPLIT>(bra + b.bra_off, ket + b.ket_off, K, b.nst, w.k0, w.l0, w.l1, w.p0,
-                             w.p1, b.sc, 0.0, part + b.part_off + (size_t)w.tile * D::NACC * K,
-                             hpart + b.hp_off + (size_t)z * K * K,
-                             spart + b.hp_off + (size_t)z * K * K, flag, acc_s);
+    pair_tile<N, MODE_SPLIT, false, true>(
+        bra + b.bra_off, ket + b.ket_off, K, b.nst, w.k0, w.l0, w.l1, w.p0, w.p1, b.sc, 0.0,
+        part + b.part_off + (size_t)w.tile * D::NACC * K, hpart + b.hp_off + (size_t)z * K * K,
+        spart + b.hp_off + (size_t)z * K * K, flag, acc_s, 0x7fffffff, kst_s);
     __syncthreads();  // the next item overwrites the LDS tables
   }
 }
