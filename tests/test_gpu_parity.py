@@ -333,3 +333,28 @@ def test_outprod_on_generator_input_vs_reference_golden():
     E, g = Engine(s, "cuda:0").outprod(necg.pack_x(iv, C))
     assert close_E(E, float(gd["m12_s0_E"])), (E, float(gd["m12_s0_E"]))
     assert close_g(g, gd["m12_s0_g"])
+
+
+@pytest.mark.parametrize("K", [2, 63, 64, 65, 127, 129])
+def test_block_boundaries_vs_oracle(oracle_lib, K):
+    """LiH at K around the 64-function blocks of the record, mirror and
+    gradient kernels (partial last blocks, one-block matrices): both pair
+    forms, the element pass and H, S against the oracle."""
+    s = necg.lih_system()
+    iv, C = necg.generate_parameters(s, K, 40 + K)
+    x = necg.pack_x(iv, C)
+    print(f"LiH K={K} block boundaries", flush=True)
+    o = oracle_lib.Oracle(s)
+    Eo, go, Ho, So = o.outprod(x, return_hs=True)
+    eng = engine_for("LiH")
+    for mode in (1, 2):
+        eng.set_pair_mode(mode)
+        E, g = eng.outprod(x)
+        assert close_E(E, Eo), (mode, E, Eo)
+        assert close_g(g, go), mode
+    eng.set_pair_mode(0)
+    E, H, S = eng.energy(x, with_hs=True)
+    assert close_E(E, Eo)
+    assert np.max(np.abs(H - Ho)) <= 1e-12 * np.max(np.abs(Ho))
+    assert np.max(np.abs(S - So)) <= 1e-12 * np.max(np.abs(So))
+    assert np.array_equal(H, H.T) and np.array_equal(S, S.T)
