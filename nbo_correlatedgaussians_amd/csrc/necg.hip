@@ -56,22 +56,8 @@ __global__ void __launch_bounds__(64) prep_ket(const double *__restrict__ bra, i
                                                const double *__restrict__ sms,
                                                const double *__restrict__ scvv, SysConst sc,
                                                double *__restrict__ ket) {
-  using D = Dim<N>;
-  constexpr int RS = D::KS + 1;  // odd row stride: the lanes' rows start in different banks
-  __shared__ double rec[64 * RS];
-  const int nk = K * nst;
-  const int idx0 = blockIdx.x * 64, idx = idx0 + (int)threadIdx.x;
-  double *r = rec + threadIdx.x * RS;
-  if (idx < nk) {
-    const int l = idx / nst, p = idx - l * nst;
-    make_ket<N>(bra + (size_t)l * D::BS, sml + (size_t)p * N * N,
-                sms + (size_t)p * D::N3 * D::N3, scvv[p], sc, r);
-  }
-  for (int f = D::K_TRMP + 1; f < D::KS; ++f) r[f] = 0.0;  // padding
-  __syncthreads();
-  const int n = min(64, nk - idx0) * D::KS;
-  double *out = ket + (size_t)idx0 * D::KS;
-  for (int i = threadIdx.x; i < n; i += 64) out[i] = rec[(i / D::KS) * RS + i % D::KS];
+  __shared__ double rec[64 * (Dim<N>::KS + 1)];
+  ket_records_block<N>(bra, K, nst, sml, sms, scvv, sc, blockIdx.x * 64, ket, rec);
 }
 
 // ------------------------------------------------------------------ pair ----
@@ -190,23 +176,7 @@ __global__ void __launch_bounds__(256) hc_sum(const double *__restrict__ hcp,
                                               const double *__restrict__ scp, int nchunk, int K,
                                               double *__restrict__ HC, double *__restrict__ SC) {
   __shared__ double ph[256], ps[256];
-  const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
-  const int i = blockIdx.x * 64 + lane;
-  double a = 0.0, b = 0.0;
-  if (i < K) {
-#pragma unroll 4
-    for (int c = q; c < nchunk; c += 4) {
-      a += hcp[(size_t)c * K + i];
-      b += scp[(size_t)c * K + i];
-    }
-  }
-  ph[threadIdx.x] = a;
-  ps[threadIdx.x] = b;
-  __syncthreads();
-  if (q == 0 && i < K) {
-    HC[i] = ((ph[lane] + ph[64 + lane]) + ph[128 + lane]) + ph[192 + lane];
-    SC[i] = ((ps[lane] + ps[64 + lane]) + ps[128 + lane]) + ps[192 + lane];
-  }
+  hc_rows(hcp, scp, nchunk, K, blockIdx.x * 64, HC, SC, ph, ps);
 }
 
 __global__ void __launch_bounds__(1024) quotient(
@@ -235,17 +205,8 @@ __global__ void __launch_bounds__(64) grad_final(const double *__restrict__ red,
                                                  const double *__restrict__ SC,
                                                  const double *__restrict__ gbar,
                                                  double *__restrict__ g) {
-  using D = Dim<N>;
-  constexpr int RS = D::BLK + 1;  // odd row stride: the lanes' rows start in different banks
-  __shared__ double go[64 * RS];
-  const int k0 = blockIdx.x * 64, k = k0 + (int)threadIdx.x;
-  if (k < K)
-    grad_one<N, MODE>(red, K, k, bra, scal, HC, SC, gbar ? gbar[0] : 1.0, g,
-                      go + threadIdx.x * RS);
-  __syncthreads();
-  const int n = min(64, K - k0) * D::BLK;
-  double *out = g + (size_t)k0 * D::BLK;
-  for (int i = threadIdx.x; i < n; i += 64) out[i] = go[(i / D::BLK) * RS + i % D::BLK];
+  __shared__ double go[64 * (Dim<N>::BLK + 1)];
+  grad_block<N, MODE>(red, K, blockIdx.x * 64, bra, scal, HC, SC, gbar ? gbar[0] : 1.0, g, go);
 }
 
 // ----------------------------------------------------------- queries -----

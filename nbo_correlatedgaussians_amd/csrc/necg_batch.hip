@@ -88,23 +88,25 @@ __global__ void b_prep_bra(const BatchSys *__restrict__ sys, int nsys,
                               true, b.sc, bra + b.bra_off + (size_t)k * Dim<NN>::BS, flag)));
 }
 
-// one lane per (system, ket, term)
-__global__ void b_prep_ket(const BatchSys *__restrict__ sys, int nsys,
-                           const long long *__restrict__ kpre, const int32_t *__restrict__ active,
-                           const double *__restrict__ sml, const double *__restrict__ sms,
-                           const double *__restrict__ scvv, const double *__restrict__ bra,
-                           double *__restrict__ ket) {
-  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= kpre[nsys]) return;
-  const int s = seg_find(kpre, nsys, idx);
-  if (active && !active[s]) return;
+// one 64-lane workgroup per (system, 64 (ket, term) records): the records
+// built in LDS and written out as one contiguous run
+__global__ void __launch_bounds__(64) b_prep_ket(const BatchSys *__restrict__ sys, int nsys,
+                                                 const long long *__restrict__ kwpre,
+                                                 const int32_t *__restrict__ active,
+                                                 const double *__restrict__ sml,
+                                                 const double *__restrict__ sms,
+                                                 const double *__restrict__ scvv,
+                                                 const double *__restrict__ bra,
+                                                 double *__restrict__ ket) {
+  __shared__ double rec[64 * (Dim<kMaxN>::KS + 1)];
+  const long long blk = blockIdx.x;
+  const int s = seg_find(kwpre, nsys, blk);
+  if (active && !active[s]) return;  // uniform over the workgroup
   const BatchSys &b = sys[s];
-  const int li = (int)(idx - kpre[s]);
-  const int l = li / b.nst, p = li - l * b.nst;
-  SWITCH_N(b.n, (make_ket<NN>(bra + b.bra_off + (size_t)l * Dim<NN>::BS,
-                              sml + b.sml_off + (size_t)p * NN * NN,
-                              sms + b.sms_off + (size_t)p * 9 * NN * NN, scvv[b.scvv_off + p],
-                              b.sc, ket + b.ket_off + (size_t)li * Dim<NN>::KS)));
+  const int idx0 = (int)(blk - kwpre[s]) * 64;
+  SWITCH_N(b.n, (ket_records_block<NN>(bra + b.bra_off, b.K, b.nst, sml + b.sml_off,
+                                       sms + b.sms_off, scvv + b.scvv_off, b.sc, idx0,
+                                       ket + b.ket_off, rec)));
 }
 
 // Persistent pair kernel for one particle count: each workgroup walks the
@@ -181,7 +183,7 @@ __global__ void __launch_bounds__(256, 1) b_pair2_all(
   }
 }
 
-// one workgroup of 256 lanes per (system, row i, 256-column chunk)
+// one 256-lane workgroup per (system, pair of 64 x 64 blocks I >= J)
 __global__ void __launch_bounds__(256) b_hs_finish(const BatchSys *__restrict__ sys, int nsys,
                                                    const long long *__restrict__ hpre,
                                                    const int32_t *__restrict__ active,
@@ -191,26 +193,45 @@ __global__ void __launch_bounds__(256) b_hs_finish(const BatchSys *__restrict__ 
                                                    double *__restrict__ H, double *__restrict__ S,
                                                    double *__restrict__ hcp,
                                                    double *__restrict__ scp) {
-  __shared__ double red[8];
+  __shared__ double bh[64 * 65], bsl[64 * 65], cc[128];
   const long long blk = blockIdx.x;
   const int s = seg_find(hpre, nsys, blk);
   if (active && !active[s]) return;  // uniform over the workgroup
   const BatchSys &b = sys[s];
-  const int r = (int)(blk - hpre[s]);
-  const int i = r / b.nchunk, chunk = r - i * b.nchunk;
+  const int t = (int)(blk - hpre[s]);
+  int I = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+  while ((I + 1) * (I + 2) / 2 <= t) ++I;
+  while (I * (I + 1) / 2 > t) --I;
+  const int J = t - I * (I + 1) / 2;
   int bs = 0, bc = 0;
   SWITCH_N(b.n, (bs = Dim<NN>::BS, bc = Dim<NN>::B_C));
-  hs_row_chunk(hpart + b.hp_off, spart + b.hp_off, b.K, b.ts, bra + b.bra_off, bs, bc, i, chunk,
-               H + b.hs_off, S + b.hs_off, hcp + b.hcp_off, scp + b.hcp_off, red);
+  hs_tile(hpart + b.hp_off, spart + b.hp_off, b.K, b.ts, bra + b.bra_off, bs, bc, I, J,
+          H + b.hs_off, S + b.hs_off, hcp + b.hcp_off, scp + b.hcp_off, bh, bsl, cc);
+}
+
+// one 256-lane workgroup per (system, 64 rows): H C and S C from the chunk
+// partials
+__global__ void __launch_bounds__(256) b_hc_sum(const BatchSys *__restrict__ sys, int nsys,
+                                                const long long *__restrict__ fwpre,
+                                                const int32_t *__restrict__ active,
+                                                const double *__restrict__ hcp,
+                                                const double *__restrict__ scp,
+                                                double *__restrict__ HC, double *__restrict__ SC) {
+  __shared__ double ph[256], ps[256];
+  const long long blk = blockIdx.x;
+  const int s = seg_find(fwpre, nsys, blk);
+  if (active && !active[s]) return;  // uniform over the workgroup
+  const BatchSys &b = sys[s];
+  hc_rows(hcp + b.hcp_off, scp + b.hcp_off, b.nchunk, b.K, (int)(blk - fwpre[s]) * 64,
+          HC + b.vec_off, SC + b.vec_off, ph, ps);
 }
 
 // one workgroup of 1024 lanes per system
 __global__ void __launch_bounds__(1024) b_quotient(const BatchSys *__restrict__ sys,
                                                    const int32_t *__restrict__ active,
-                                                   const double *__restrict__ hcp,
-                                                   const double *__restrict__ scp,
                                                    const double *__restrict__ bra,
-                                                   double *__restrict__ HC, double *__restrict__ SC,
+                                                   const double *__restrict__ HC,
+                                                   const double *__restrict__ SC,
                                                    double *__restrict__ scal,
                                                    double *__restrict__ f, int *__restrict__ flag) {
   __shared__ double rh[1024], rs[1024];
@@ -219,9 +240,9 @@ __global__ void __launch_bounds__(1024) b_quotient(const BatchSys *__restrict__ 
   const BatchSys &b = sys[s];
   int bs = 0, bc = 0;
   SWITCH_N(b.n, (bs = Dim<NN>::BS, bc = Dim<NN>::B_C));
-  quotient_block(hcp + b.hcp_off, scp + b.hcp_off, b.nchunk, bra + b.bra_off, bs, bc, b.K,
-                 HC + b.vec_off, SC + b.vec_off, scal + 8 * (size_t)s, f ? f + s : nullptr, flag,
-                 rh, rs);
+  // over the summed H C and S C (b_hc_sum): one chunk, nothing stored back
+  quotient_block(HC + b.vec_off, SC + b.vec_off, 1, bra + b.bra_off, bs, bc, b.K, nullptr,
+                 nullptr, scal + 8 * (size_t)s, f ? f + s : nullptr, flag, rh, rs);
 }
 
 // one 256-lane workgroup per (system, slot, 64 functions): the two-sided
@@ -249,22 +270,25 @@ __global__ void __launch_bounds__(256) b_tile_reduce(const BatchSys *__restrict_
   }
 }
 
-// one lane per (system, function)
-__global__ void b_grad_final(const BatchSys *__restrict__ sys, int nsys,
-                             const long long *__restrict__ fpre,
-                             const int32_t *__restrict__ active, const double *__restrict__ red,
-                             const double *__restrict__ bra, const double *__restrict__ scal,
-                             const double *__restrict__ HC, const double *__restrict__ SC,
-                             double *__restrict__ g) {
-  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= fpre[nsys]) return;
-  const int s = seg_find(fpre, nsys, idx);
-  if (active && !active[s]) return;
+// one 64-lane workgroup per (system, 64 functions)
+__global__ void __launch_bounds__(64) b_grad_final(const BatchSys *__restrict__ sys, int nsys,
+                                                   const long long *__restrict__ fwpre,
+                                                   const int32_t *__restrict__ active,
+                                                   const double *__restrict__ red,
+                                                   const double *__restrict__ bra,
+                                                   const double *__restrict__ scal,
+                                                   const double *__restrict__ HC,
+                                                   const double *__restrict__ SC,
+                                                   double *__restrict__ g) {
+  __shared__ double go[64 * (Dim<kMaxN>::BLK + 1)];
+  const long long blk = blockIdx.x;
+  const int s = seg_find(fwpre, nsys, blk);
+  if (active && !active[s]) return;  // uniform over the workgroup
   const BatchSys &b = sys[s];
-  const int k = (int)(idx - fpre[s]);
-  SWITCH_N(b.n, (grad_one<NN, MODE_SPLIT>(red + b.red_off, b.K, k, bra + b.bra_off,
-                                          scal + 8 * (size_t)s, HC + b.vec_off, SC + b.vec_off,
-                                          1.0, g + b.x_off)));
+  const int k0 = (int)(blk - fwpre[s]) * 64;
+  SWITCH_N(b.n, (grad_block<NN, MODE_SPLIT>(red + b.red_off, b.K, k0, bra + b.bra_off,
+                                            scal + 8 * (size_t)s, HC + b.vec_off,
+                                            SC + b.vec_off, 1.0, g + b.x_off, go)));
 }
 
 template <int N>
@@ -328,11 +352,12 @@ int dim_nacc(int n) { int r = 0; SWITCH_N(n, r = Dim<NN>::NACC); return r; }
 struct necg_batch {
   int device = 0, nsys = 0, ncu = 0;
   std::vector<BatchSys> hsys;
-  // host copies of the prefixes: functions, (function, term) records,
-  // (row, chunk) workgroups, (slot, 64 functions) workgroups
-  std::vector<long long> fpre, kpre, hpre, r2pre;
+  // host copies of the prefixes: functions, 64-function groups, 64-record
+  // groups, (64 x 64 block pair) workgroups, (slot, 64 functions) workgroups
+  std::vector<long long> fpre, fwpre, kwpre, hpre, r2pre;
   BatchSys *d_sys = nullptr;
-  long long *d_fpre = nullptr, *d_kpre = nullptr, *d_hpre = nullptr, *d_r2pre = nullptr;
+  long long *d_fpre = nullptr, *d_fwpre = nullptr, *d_kwpre = nullptr, *d_hpre = nullptr,
+            *d_r2pre = nullptr;
   double *sml = nullptr, *sms = nullptr, *scvv = nullptr;
   double *bra = nullptr, *ket = nullptr, *H = nullptr, *S = nullptr, *hpart = nullptr,
          *spart = nullptr, *part = nullptr, *red = nullptr, *HC = nullptr, *SC = nullptr,
@@ -506,7 +531,7 @@ int necg_batch_destroy(necg_batch *B) {
                       B->part, B->red, B->HC, B->SC, B->hcp, B->scp, B->scal, B->kpart};
     for (double *p : bufs)
       if (p) (void)hipFree(p);
-    long long *pre[] = {B->d_fpre, B->d_kpre, B->d_hpre, B->d_r2pre};
+    long long *pre[] = {B->d_fpre, B->d_fwpre, B->d_kwpre, B->d_hpre, B->d_r2pre};
     for (long long *p : pre)
       if (p) (void)hipFree(p);
     if (B->d_sys) (void)hipFree(B->d_sys);
@@ -551,7 +576,8 @@ int necg_batch_create(const necg_system *systems, int32_t nsys, const int32_t *K
   std::vector<unsigned> h_tperm;
   long long xo = 0, bo = 0, ko = 0, hso = 0, ro = 0, vo = 0, hco = 0;
   B->fpre.assign(1, 0);
-  B->kpre.assign(1, 0);
+  B->fwpre.assign(1, 0);
+  B->kwpre.assign(1, 0);
   B->hpre.assign(1, 0);
   B->r2pre.assign(1, 0);
   for (int s = 0; s < nsys; ++s) {
@@ -568,7 +594,7 @@ int necg_batch_create(const necg_system *systems, int32_t nsys, const int32_t *K
     b.nst = y.nst;
     b.K = K;
     b.nacc = dim_nacc(n);
-    b.nchunk = (K + 255) / 256;
+    b.nchunk = (K + 63) / 64;  // H C / S C partials per 64 columns
     b.x_off = xo;
     xo += (long long)K * (dim_blk(n) + 1);
     b.bra_off = bo;
@@ -595,8 +621,9 @@ int necg_batch_create(const necg_system *systems, int32_t nsys, const int32_t *K
     h_tperm.insert(h_tperm.end(), tab.begin(), tab.end());
     B->hsys.push_back(b);
     B->fpre.push_back(B->fpre.back() + K);
-    B->kpre.push_back(B->kpre.back() + (long long)K * y.nst);
-    B->hpre.push_back(B->hpre.back() + (long long)K * b.nchunk);
+    B->fwpre.push_back(B->fwpre.back() + (K + 63) / 64);
+    B->kwpre.push_back(B->kwpre.back() + ((long long)K * y.nst + 63) / 64);
+    B->hpre.push_back(B->hpre.back() + (long long)b.nchunk * (b.nchunk + 1) / 2);
     B->r2pre.push_back(B->r2pre.back() + (long long)b.nacc * ((K + 63) / 64));
   }
   B->nx = xo;
@@ -609,7 +636,8 @@ int necg_batch_create(const necg_system *systems, int32_t nsys, const int32_t *K
       (rc = dalloc(&B->hcp, (size_t)hco)) || (rc = dalloc(&B->scp, (size_t)hco)) ||
       (rc = dalloc(&B->scal, (size_t)8 * nsys)) || (rc = dalloc(&B->d_sys, (size_t)nsys)) ||
       (rc = dalloc(&B->flag, 1)) || (rc = upload_prefix(&B->d_fpre, B->fpre)) ||
-      (rc = upload_prefix(&B->d_kpre, B->kpre)) || (rc = upload_prefix(&B->d_hpre, B->hpre)) ||
+      (rc = upload_prefix(&B->d_fwpre, B->fwpre)) ||
+      (rc = upload_prefix(&B->d_kwpre, B->kwpre)) || (rc = upload_prefix(&B->d_hpre, B->hpre)) ||
       (rc = upload_prefix(&B->d_r2pre, B->r2pre))) {
     necg_batch_destroy(B);
     return rc;
@@ -665,8 +693,8 @@ int necg_batch_outprod_masked(necg_batch *B, const int32_t *active, const double
   auto blocks = [](long long count, int t) { return (unsigned)((count + t - 1) / t); };
   hipLaunchKernelGGL(b_prep_bra, dim3(blocks(B->fpre[n], tb)), dim3(tb), 0, st, B->d_sys, n,
                      B->d_fpre, active, x, B->bra, B->flag);
-  hipLaunchKernelGGL(b_prep_ket, dim3(blocks(B->kpre[n], tb)), dim3(tb), 0, st, B->d_sys, n,
-                     B->d_kpre, active, B->sml, B->sms, B->scvv, B->bra, B->ket);
+  hipLaunchKernelGGL(b_prep_ket, dim3((unsigned)B->kwpre[n]), dim3(64), 0, st, B->d_sys, n,
+                     B->d_kwpre, active, B->sml, B->sms, B->scvv, B->bra, B->ket);
   for (const auto &c : B->cls) {
     if (c.two) {
       launch_pair2(c.grid, st, B->d_sys, c.d_items, c.nitems, active, B->bra, B->ket, B->tperm,
@@ -678,12 +706,14 @@ int necg_batch_outprod_masked(necg_batch *B, const int32_t *active, const double
   }
   hipLaunchKernelGGL(b_hs_finish, dim3((unsigned)B->hpre[n]), dim3(256), 0, st, B->d_sys, n,
                      B->d_hpre, active, B->hpart, B->spart, B->bra, B->H, B->S, B->hcp, B->scp);
-  hipLaunchKernelGGL(b_quotient, dim3(n), dim3(1024), 0, st, B->d_sys, active, B->hcp, B->scp,
-                     B->bra, B->HC, B->SC, B->scal, f, B->flag);
+  hipLaunchKernelGGL(b_hc_sum, dim3((unsigned)B->fwpre[n]), dim3(256), 0, st, B->d_sys, n,
+                     B->d_fwpre, active, B->hcp, B->scp, B->HC, B->SC);
+  hipLaunchKernelGGL(b_quotient, dim3(n), dim3(1024), 0, st, B->d_sys, active, B->bra, B->HC,
+                     B->SC, B->scal, f, B->flag);
   hipLaunchKernelGGL(b_tile_reduce, dim3((unsigned)B->r2pre[n]), dim3(256), 0, st, B->d_sys, n,
                      B->d_r2pre, active, B->part, B->kpart, B->red);
-  hipLaunchKernelGGL(b_grad_final, dim3(blocks(B->fpre[n], tb)), dim3(tb), 0, st, B->d_sys, n,
-                     B->d_fpre, active, B->red, B->bra, B->scal, B->HC, B->SC, g);
+  hipLaunchKernelGGL(b_grad_final, dim3((unsigned)B->fwpre[n]), dim3(64), 0, st, B->d_sys, n,
+                     B->d_fwpre, active, B->red, B->bra, B->scal, B->HC, B->SC, g);
   return hip_rc(hipGetLastError());
 }
 

@@ -305,6 +305,58 @@ __device__ __forceinline__ void hs_tile(const double *__restrict__ hpart,
   }
 }
 
+// The 64 ket records [idx0, idx0 + 64) of a system (one lane per (ket l,
+// term p), idx = l nst + p), built in the LDS rows rec (64 x (KS + 1)) and
+// written out by the workgroup of 64 lanes as one contiguous run (coalesced).
+template <int N>
+__device__ __forceinline__ void ket_records_block(const double *__restrict__ bra, int K,
+                                                  int nst, const double *__restrict__ sml,
+                                                  const double *__restrict__ sms,
+                                                  const double *__restrict__ scvv,
+                                                  const SysConst &sc, int idx0,
+                                                  double *__restrict__ ket, double *rec) {
+  using D = Dim<N>;
+  constexpr int RS = D::KS + 1;  // odd row stride: the lanes' rows start in different banks
+  const int nk = K * nst, idx = idx0 + (int)threadIdx.x;
+  double *r = rec + threadIdx.x * RS;
+  if (idx < nk) {
+    const int l = idx / nst, p = idx - l * nst;
+    make_ket<N>(bra + (size_t)l * D::BS, sml + (size_t)p * N * N,
+                sms + (size_t)p * D::N3 * D::N3, scvv[p], sc, r);
+  }
+  for (int f = D::K_TRMP + 1; f < D::KS; ++f) r[f] = 0.0;  // padding
+  __syncthreads();
+  const int n = min(64, nk - idx0) * D::KS;
+  double *out = ket + (size_t)idx0 * D::KS;
+  for (int i = threadIdx.x; i < n; i += 64) out[i] = rec[(i / D::KS) * RS + i % D::KS];
+}
+
+// H C and S C for the 64 rows [i0, i0 + 64) from the 64-column chunk
+// partials, by a 256-lane workgroup: wave q sums the chunks c = q mod 4, the
+// four parts are added in order.  ph, ps: 256 LDS doubles each.
+__device__ __forceinline__ void hc_rows(const double *__restrict__ hcp,
+                                        const double *__restrict__ scp, int nchunk, int K,
+                                        int i0, double *__restrict__ HC, double *__restrict__ SC,
+                                        double *ph, double *ps) {
+  const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
+  const int i = i0 + lane;
+  double a = 0.0, b = 0.0;
+  if (i < K) {
+#pragma unroll 4
+    for (int c = q; c < nchunk; c += 4) {
+      a += hcp[(size_t)c * K + i];
+      b += scp[(size_t)c * K + i];
+    }
+  }
+  ph[threadIdx.x] = a;
+  ps[threadIdx.x] = b;
+  __syncthreads();
+  if (q == 0 && i < K) {
+    HC[i] = ((ph[lane] + ph[64 + lane]) + ph[128 + lane]) + ph[192 + lane];
+    SC[i] = ((ps[lane] + ps[64 + lane]) + ps[128 + lane]) + ps[192 + lane];
+  }
+}
+
 // HC, SC from the chunk partials (not stored where HC, SC are null), then
 // C^T H C, C^T S C, E = cHc / cSc and 1/cSc (python/outprod.py:65-68).  One workgroup of 1024 lanes per system;
 // rh/rs are 1024-entry LDS arrays.
@@ -398,6 +450,26 @@ __device__ __forceinline__ void grad_one(const double *__restrict__ red, int K, 
     for (int v = 0; v < D::BLK; ++v) gk[v] = fac * oh[v];
   }
   g[(size_t)K * D::BLK + k] = gb * 2.0 * nrm * (HC[k] - E * SC[k]);
+}
+
+// The gradient blocks of the 64 functions [k0, k0 + 64), formed in the LDS
+// rows go (64 x (BLK + 1)) and written out by the workgroup of 64 lanes as
+// one contiguous run (coalesced); g_C goes straight to its place.
+template <int N, int MODE>
+__device__ __forceinline__ void grad_block(const double *__restrict__ red, int K, int k0,
+                                           const double *__restrict__ bra,
+                                           const double *__restrict__ scal,
+                                           const double *__restrict__ HC,
+                                           const double *__restrict__ SC, double gb,
+                                           double *__restrict__ g, double *go) {
+  using D = Dim<N>;
+  constexpr int RS = D::BLK + 1;  // odd row stride: the lanes' rows start in different banks
+  const int k = k0 + (int)threadIdx.x;
+  if (k < K) grad_one<N, MODE>(red, K, k, bra, scal, HC, SC, gb, g, go + threadIdx.x * RS);
+  __syncthreads();
+  const int n = min(64, K - k0) * D::BLK;
+  double *out = g + (size_t)k0 * D::BLK;
+  for (int i = threadIdx.x; i < n; i += 64) out[i] = go[(i / D::BLK) * RS + i % D::BLK];
 }
 
 }  // namespace necg
