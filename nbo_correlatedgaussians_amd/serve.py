@@ -175,12 +175,14 @@ class Server:
     @staticmethod
     def _eval_bytes(system, K):
         """Device bytes one evaluation of this system works through: H, S and
-        the element partials (4 K^2), the ket table (K nst KS), and the
-        gradient partials (about K/16 tiles of NACC x K)."""
+        the element partials (4 K^2), the ket table (K nst KS), the gradient
+        partials (about K/16 tiles of NACC x K) and the H C / S C partials of
+        the 64-column chunks."""
         n, nst = system.n, system.nst
         nacc = n * (n + 1) + 6 * n + 2
         ks = (n * (n + 1) + 6 * n + 4 + n * n + 1 + 15) // 16 * 16   # ket record, whole lines
-        return 8 * (4 * K * K + K * nst * ks + max(1, K // 16) * nacc * K)
+        return 8 * (4 * K * K + K * nst * ks + max(1, K // 16) * nacc * K
+                    + 2 * ((K + 63) // 64) * K)
 
     def close(self):
         if getattr(self, "_pool", None) and self._pool.value:
