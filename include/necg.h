@@ -175,6 +175,12 @@ int necg_status(necg_ctx *ctx, void *stream);
  * 2 for n = 6): `waves` must be 0 or that value.  Used by the autotuner. */
 int necg_set_tiling(necg_ctx *ctx, int32_t ket_tile, int32_t term_split, int32_t waves);
 
+/* Compute units that the pair passes size their launches for (0: the
+ * device's own count).  A smaller count suits a device shared with other
+ * work; it also changes which tiles of the lower-triangle passes are split
+ * into halves for the last round.  No counterpart in the reference. */
+int necg_set_launch_cus(necg_ctx *ctx, int32_t ncu);
+
 /* Launch shape that the last pair-kernel launch used (for reports). */
 int necg_get_tiling(necg_ctx *ctx, int32_t *ket_tile, int32_t *term_split, int32_t *waves);
 

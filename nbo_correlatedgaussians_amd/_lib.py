@@ -28,7 +28,7 @@ EXPORTS = ("necg_create", "necg_destroy", "necg_outprod", "necg_energy",
            "necg_set_pair_mode", "necg_pair_form", "necg_fp64_fma_rate",
            "necg_energy_fwd", "necg_energy_bwd",
            "necg_energy_vjp", "necg_matrices", "necg_matel", "necg_gradel",
-           "necg_dhds", "necg_status", "necg_set_tiling", "necg_get_tiling",
+           "necg_dhds", "necg_status", "necg_set_tiling", "necg_set_launch_cus", "necg_get_tiling",
            "necg_timing", "necg_strerror", "necg_batch_create", "necg_batch_destroy",
            "necg_batch_layout", "necg_batch_outprod", "necg_batch_matrices",
            "necg_batch_set_tiling", "necg_batch_status", "necg_batch_outprod_masked",
@@ -107,6 +107,7 @@ def load() -> ct.CDLL:
         "necg_dhds": [P, I, P, P, P, P],
         "necg_status": [P, P],
         "necg_set_tiling": [P, I, I, I],
+        "necg_set_launch_cus": [P, I],
         "necg_get_tiling": [P, ct.POINTER(I), ct.POINTER(I), ct.POINTER(I)],
         "necg_timing": [P, I, ct.POINTER(D), ct.POINTER(ct.c_int64)],
         "necg_batch_create": [P, I, P, ct.c_int, ct.POINTER(P)],

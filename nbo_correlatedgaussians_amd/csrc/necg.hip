@@ -94,32 +94,32 @@ __global__ void __launch_bounds__(64 * pair_waves(N), NECG_MIN_WAVES) pair_kerne
 }
 
 // Two-sided pair pass over the lower triangle.  blockIdx.x indexes the tile
-// list (tiles[2i], tiles[2i+1]) = (bra tile, ket tile) with ket tile start
-// <= bra tile end; blockIdx.y = term group z of gridDim.y.
-// bpart layout: [(ket tile * ts + z) * NACC + v] * K + k;
+// list (TriTile words: bra tile, ket tile, ket start <= bra tile end, whole
+// or half); blockIdx.y = term group z of gridDim.y.
+// bpart layout: [(ket tile * ts + z) * NACC + v] * K + k, the second half of
+// a split tile at xpart [(e * ts + z) * NACC + v] * K + k instead;
 // kpart layout: [((bra tile * ts + z) * K + l) * NACC + v].
 template <int N, bool R1>
 __global__ void __launch_bounds__(64 * pair_waves(N), NECG_MIN_WAVES) pair2_kernel(
     const double *__restrict__ bra, const double *__restrict__ ket, int K, int nst, int kt,
     int tps, SysConst sc, const int *__restrict__ tiles, const unsigned *__restrict__ tperm,
-    double *__restrict__ bpart, double *__restrict__ kpart, double *__restrict__ hpart,
-    double *__restrict__ spart, int *__restrict__ flag) {
+    double *__restrict__ bpart, double *__restrict__ xpart, double *__restrict__ kpart,
+    double *__restrict__ hpart, double *__restrict__ spart, int *__restrict__ flag) {
   using D = Dim<N>;
   static_assert(Two<N>::NCH <= 4, "two-sided form: at most 64 accumulator slots");
   __shared__ double acc_s[PairLds<N, MODE_SPLIT>::ACC];
   __shared__ double kacc_s[pair_waves(N) * 64];
   __shared__ double bra_s[BraLds<N>::NF * 64];
   __shared__ double kst_s[pair_waves(N) * 64];
-  const int bx = tiles[2 * blockIdx.x], by = tiles[2 * blockIdx.x + 1];
-  const int k0 = bx * 64, l0 = by * kt;
-  const int up = tri_up(K, kt), lup = k0 < up ? up : K;
-  const int l1 = l0 < lup ? min(min(K, l0 + kt), k0 + 64) : min(K, l0 + kt);
+  const TriTile t = tri_tile(tiles, blockIdx.x, K, kt);
   const int z = blockIdx.y, ts = gridDim.y;
   const int p0 = z * tps;
-  pair2_tile<N, R1>(bra, ket, K, nst, k0, l0, l1, lup, p0, min(nst, p0 + tps), sc, tperm,
-                bpart + (size_t)(by * ts + z) * D::NACC * K,
-                kpart + (size_t)(bx * ts + z) * K * D::NACC, hpart + (size_t)z * K * K,
-                spart + (size_t)z * K * K, flag, acc_s, kacc_s, bra_s, kst_s);
+  double *bp = t.e < 0 ? bpart + (size_t)(t.by * ts + z) * D::NACC * K
+                       : xpart + (size_t)(t.e * ts + z) * D::NACC * K;
+  pair2_tile<N, R1>(bra, ket, K, nst, t.k0, t.l0, t.l1, t.lup, p0, min(nst, p0 + tps), sc,
+                    tperm, bp, kpart + (size_t)(t.bx * ts + z) * K * D::NACC,
+                    hpart + (size_t)z * K * K, spart + (size_t)z * K * K, flag, acc_s, kacc_s,
+                    bra_s, kst_s);
 }
 
 // Element-only pass (MODE_ELEMENTS / MODE_TV) over the same lower-triangle
@@ -132,15 +132,12 @@ __global__ void __launch_bounds__(64 * pair_waves(N), NECG_MIN_WAVES) pair_tri_k
   static_assert(!mode_has_grad(MODE), "element modes only");
   __shared__ double acc_s[PairLds<N, MODE>::ACC];
   __shared__ double kst_s[pair_waves(N) * Dim<N>::KS];
-  const int bx = tiles[2 * blockIdx.x], by = tiles[2 * blockIdx.x + 1];
-  const int k0 = bx * 64, l0 = by * kt;
-  const int up = tri_up(K, kt), lup = k0 < up ? up : K;
-  const int l1 = l0 < lup ? min(min(K, l0 + kt), k0 + 64) : min(K, l0 + kt);
+  const TriTile t = tri_tile(tiles, blockIdx.x, K, kt);
   const int z = blockIdx.y;
   const int p0 = z * tps;
-  pair_tile<N, MODE, R1, true>(bra, ket, K, nst, k0, l0, l1, p0, min(nst, p0 + tps), sc, 0.0,
-                               nullptr, hpart + (size_t)z * K * K, spart + (size_t)z * K * K,
-                               flag, acc_s, lup, kst_s);
+  pair_tile<N, MODE, R1, true>(bra, ket, K, nst, t.k0, t.l0, t.l1, p0, min(nst, p0 + tps), sc,
+                               0.0, nullptr, hpart + (size_t)z * K * K,
+                               spart + (size_t)z * K * K, flag, acc_s, t.lup, kst_s);
 }
 
 // Two-sided partial sums per (function, slot).  Grid (ceil(K/64), nv),
@@ -148,9 +145,12 @@ __global__ void __launch_bounds__(64 * pair_waves(N), NECG_MIN_WAVES) pair_tri_k
 __global__ void __launch_bounds__(256) tile_reduce2(const double *__restrict__ bpart,
                                                     const double *__restrict__ kpart, int K,
                                                     int kt, int ts, int nacc,
-                                                    double *__restrict__ red) {
+                                                    double *__restrict__ red,
+                                                    const double *__restrict__ xpart,
+                                                    const int *__restrict__ xr) {
   __shared__ double sh[256];
-  tile_reduce2_block(bpart, kpart, K, kt, ts, nacc, blockIdx.y, blockIdx.x * 64, red, sh);
+  tile_reduce2_block(bpart, kpart, K, kt, ts, nacc, blockIdx.y, blockIdx.x * 64, red, sh, xpart,
+                     xr);
 }
 
 // Full symmetric H and S and the per-chunk (64 columns) partials of H C and
@@ -519,12 +519,13 @@ int finish_grad(necg_ctx *c, int K, int ntiles, const double *gbar, double *g, h
 
 // Ket tile and term groups of the lower-triangle passes.  One workgroup per
 // CU at a time, its waves taking alternate kets, so a launch takes about
-// (rounds of ncu workgroups) x (kets per wave + a per-tile overhead of 0.2
-// ket) ket-term sweeps; the ket tile minimising that is chosen (measured on
-// LiH K = 2048: the model orders nine tile sizes within 2 %; the overhead
-// term, from four repeated 4-vs-8 comparisons, also prices the per-tile
-// partials the reduction reads), then term groups while there are fewer
-// than two workgroups per CU.
+// (rounds of ncu workgroups) x (kets per wave + a per-tile overhead of 0.07
+// ket) ket-term sweeps, a last round of split tiles (tri_split_count) at
+// half the kets; the ket tile minimising that is chosen, then term groups
+// while there are fewer than two workgroups per CU.  The overhead is fitted
+// on LiH K = 2048: ket tile 8 beats 4 (33 rounds of 1 ket against 17 of 2:
+// more than 0.06), and 8 beats 16 by 5 % unsplit (17 rounds of 2 kets against
+// 9 of 4) and by 2.3 % split (16 + a half round against 8 + a half).
 void choose_tiling2(necg_ctx *c, int K, int &kt, int &ts) {
   const int ncu = std::max(1, c->ncu);
   const int w = pair_waves(c->n);
@@ -535,8 +536,12 @@ void choose_tiling2(necg_ctx *c, int K, int &kt, int &ts) {
     double best = -1.0;
     kt = 8;
     for (int q : cand) {
-      const long long n = tri_count(K, q);
-      const double cost = (double)((n + ncu - 1) / ncu) * ((q + w - 1) / w + 0.2);
+      long long n, nfull;
+      tri_counts(K, q, n, nfull);
+      const double sweep = (q + w - 1) / w + 0.07;
+      double cost = (double)(n / ncu) * sweep;
+      if (n % ncu)
+        cost += tri_split_count(n, nfull, q, ncu) ? (q / 2 + w - 1) / w + 0.07 : sweep;
       if (best < 0.0 || cost < best) {
         best = cost;
         kt = q;
@@ -553,14 +558,16 @@ void choose_tiling2(necg_ctx *c, int K, int &kt, int &ts) {
   }
 }
 
-// The tile list for (K, kt) on the device (tri_tiles order).
+// The tile list for (K, kt) on the device (tri_tile_list: tri_tiles order,
+// the last round's tiles split when that leaves CUs idle), then the
+// per-bra-tile extra-slot ranges.
 int ensure_tiles(necg_ctx *c, int K, int kt) {
   if (c->tiles_K == K && c->tiles_kt == kt) return NECG_OK;
-  std::vector<int> full;
-  tri_tiles(K, kt, [&](int bx, int by) {
-    full.push_back(bx);
-    full.push_back(by);
-  });
+  std::vector<int> full, xr;
+  const int nsplit = tri_tile_list(K, kt, std::max(1, c->ncu), full, xr);
+  const size_t ntile_words = full.size();
+  full.insert(full.end(), xr.begin(), xr.end());
+  c->tiles_K = 0;   // no valid list until the copy below completes
   if (full.size() > c->tiles_cap) {
     if (c->d_tiles) (void)hipFree(c->d_tiles);
     c->d_tiles = nullptr;
@@ -571,7 +578,9 @@ int ensure_tiles(necg_ctx *c, int K, int kt) {
   HIP_TRY(hipMemcpy(c->d_tiles, full.data(), full.size() * sizeof(int), hipMemcpyHostToDevice));
   c->tiles_K = K;
   c->tiles_kt = kt;
-  c->ntri = (int)(full.size() / 2);
+  c->ntri = (int)(ntile_words / 2);
+  c->nsplit = nsplit;
+  c->d_xr = c->d_tiles + ntile_words;
   return NECG_OK;
 }
 
@@ -611,7 +620,8 @@ int run_pairs2(necg_ctx *c, int K, hipStream_t st, int &kt, int &ts) {
   const int nkt = (K + kt - 1) / kt, nbt = (K + 63) / 64;
   int rc;
   if ((rc = ensure_tiles(c, K, kt))) return rc;
-  if ((rc = ensure_part(c, (size_t)nkt * ts * D::NACC * K))) return rc;
+  if ((rc = ensure_part(c, (size_t)(nkt + c->nsplit) * ts * D::NACC * K))) return rc;
+  double *xpart = c->part + (size_t)nkt * ts * D::NACC * K;
   if ((rc = ensure_kpart(c, (size_t)nbt * ts * K * D::NACC))) return rc;
   if ((rc = ensure_hpart(c, (size_t)ts * K * K))) return rc;
   hipEvent_t e1;
@@ -619,11 +629,11 @@ int run_pairs2(necg_ctx *c, int K, hipStream_t st, int &kt, int &ts) {
   if (c->sc.mrank1)
     hipLaunchKernelGGL((pair2_kernel<N, true>), dim3(c->ntri, ts), dim3(64 * pair_waves(N)), 0,
                        st, c->bra, c->ket, K, c->nst, kt, tps, c->sc, c->d_tiles, c->d_tperm,
-                       c->part, c->kpart, c->hpart, c->spart, c->d_flag);
+                       c->part, xpart, c->kpart, c->hpart, c->spart, c->d_flag);
   else
     hipLaunchKernelGGL((pair2_kernel<N, false>), dim3(c->ntri, ts), dim3(64 * pair_waves(N)), 0,
                        st, c->bra, c->ket, K, c->nst, kt, tps, c->sc, c->d_tiles, c->d_tperm,
-                       c->part, c->kpart, c->hpart, c->spart, c->d_flag);
+                       c->part, xpart, c->kpart, c->hpart, c->spart, c->d_flag);
   if (c->timing) HIP_TRY(hipEventRecord(e1, st));
   c->last_kt = kt;
   c->last_ts = ts;
@@ -662,8 +672,10 @@ template <int N>
 int finish_grad2(necg_ctx *c, int K, int kt, int ts, const double *gbar, double *g,
                  hipStream_t st) {
   using D = Dim<N>;
+  const int nkt = (K + kt - 1) / kt;
   hipLaunchKernelGGL(tile_reduce2, dim3((K + 63) / 64, D::NACC), dim3(256), 0, st, c->part,
-                     c->kpart, K, kt, ts, D::NACC, c->red);
+                     c->kpart, K, kt, ts, D::NACC, c->red,
+                     c->part + (size_t)nkt * ts * D::NACC * K, c->nsplit ? c->d_xr : nullptr);
   hipLaunchKernelGGL((grad_final<N, MODE_SPLIT>), dim3((K + 63) / 64), dim3(64), 0, st, c->red,
                      K, c->bra, c->scal, c->HC, c->SC, gbar, g);
   return hip_status(hipGetLastError());
@@ -880,7 +892,7 @@ int necg_create(const necg_system *sys, int device, necg_ctx **out) {
   int ncu = 0;
   if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess &&
       ncu > 0)
-    c->ncu = ncu;
+    c->ncu = c->dev_ncu = ncu;
   std::vector<unsigned> tab;
   if (two_sided_table(sys, tab)) {
     if ((rc = hip_status(hipMalloc((void **)&c->d_tperm, tab.size() * sizeof(unsigned)))) ||
@@ -1103,6 +1115,14 @@ int necg_pair_form(necg_ctx *c, int32_t *available, int32_t *last) {
   std::lock_guard<std::mutex> lk(c->mu);
   *available = c->two_ok;
   *last = c->last_two;
+  return NECG_OK;
+}
+
+int necg_set_launch_cus(necg_ctx *c, int32_t ncu) {
+  if (!c || ncu < 0) return NECG_ERR_ARG;
+  std::lock_guard<std::mutex> lk(c->mu);
+  c->ncu = ncu > 0 ? ncu : c->dev_ncu;
+  c->tiles_K = 0;   // the tile list depends on it
   return NECG_OK;
 }
 

@@ -37,16 +37,19 @@ struct necg_ctx {
   int req_kt = 0, req_ts = 0, req_w = 0;
   int last_kt = 0, last_ts = 0, last_w = 0;
   int last_K = 0;
-  int ncu = 256;  // compute units of the device (launch sizing)
+  int ncu = 256;      // compute units that launch sizing assumes
+  int dev_ncu = 256;  // compute units of the device
   // two-sided pair form: requested mode (0 auto, 1 two-sided, 2 one-sided),
   // whether the terms allow it, the per-(term, lane) target-slot words, and
   // whether the last gradient pair pass used it
   int pair_mode = 0, two_ok = 0, last_two = 0;
   unsigned *d_tperm = nullptr;
-  // lower-triangle tile list (bra tile, ket tile) for K = tiles_K, kt = tiles_kt
-  int *d_tiles = nullptr;
+  // lower-triangle tile list (TriTile words) for K = tiles_K, kt = tiles_kt:
+  // ntri entries, nsplit of its tiles split in halves; d_xr (inside the same
+  // allocation) = the extra-slot range of each bra tile
+  int *d_tiles = nullptr, *d_xr = nullptr;
   size_t tiles_cap = 0;
-  int tiles_K = 0, tiles_kt = 0, ntri = 0;
+  int tiles_K = 0, tiles_kt = 0, ntri = 0, nsplit = 0;
   // per-(bra tile, term group) ket-side accumulator records, [t][l][slot]
   double *kpart = nullptr;
   size_t kpart_cap = 0;

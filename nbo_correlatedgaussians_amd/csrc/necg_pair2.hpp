@@ -81,6 +81,42 @@ __host__ __device__ inline int tri_up(int K, int kt) {
   return ((K & 63) != 0 && kt > 0 && (64 % kt) == 0) ? 64 * ((K - 1) / 64) : K;
 }
 
+// Ket range [l0, l1) of lower-triangle tile (bra tile bx, ket tile by) and
+// the first ket of its upper pairs, lup.
+__host__ __device__ inline void tri_extent(int K, int kt, int bx, int by, int &l0, int &l1,
+                                           int &lup) {
+  const int k0 = 64 * bx, up = tri_up(K, kt);
+  lup = k0 < up ? up : K;
+  l0 = by * kt;
+  l1 = l0 + kt < K ? l0 + kt : K;
+  if (l0 < lup && k0 + 64 < l1) l1 = k0 + 64;
+}
+
+// One entry of a lower-triangle tile list: word 0 = bx | e << 16 | h << 30,
+// word 1 = by.  h = 0: the whole tile; h = 1, 2: its first and second half of
+// kets (kt even), the second half writing its bra partials to extra slot e.
+struct TriTile {
+  int bx, by, k0, l0, l1, lup, e;
+};
+
+__host__ __device__ inline TriTile tri_tile(const int *tiles, int i, int K, int kt) {
+  const unsigned w = (unsigned)tiles[2 * i];
+  TriTile t;
+  t.bx = (int)(w & 0xffffu);
+  t.by = tiles[2 * i + 1];
+  t.k0 = 64 * t.bx;
+  t.e = -1;
+  tri_extent(K, kt, t.bx, t.by, t.l0, t.l1, t.lup);
+  const unsigned h = w >> 30;
+  if (h == 1u) {
+    t.l1 = t.l0 + kt / 2;
+  } else if (h == 2u) {
+    t.l0 += kt / 2;
+    t.e = (int)((w >> 16) & 0x3fffu);
+  }
+  return t;
+}
+
 // Ket-side values per evaluation: X_H, X_S interleaved per packed (i, j),
 // then Y_H, Y_S interleaved per (i, x), then sigma_H, sigma_S; padded to
 // whole chunks of 16.
@@ -616,16 +652,26 @@ __host__ __device__ inline TriRanges tri_ranges(int K, int kt, int k) {
 // kpart[((x * ts + z) * K + k) * nacc + v], over tri_ranges.  Part q of np
 // (np lanes share one function) sums every np-th partial from the q-th on;
 // the caller adds the np part sums in order q = 0, 1, ...: a fixed order.
+// With a split tile list (xr not null), bra tile bx's split tiles have their
+// second halves' bra partials at xpart[((e * ts + z) * nacc + v) * K + k],
+// e in [xr[2 bx], xr[2 bx + 1]).
 __device__ __forceinline__ double tile_reduce2_part(const double *__restrict__ bpart,
                                                     const double *__restrict__ kpart, int K,
                                                     int kt, int ts, int nacc, int v, int k,
-                                                    int q, int np) {
+                                                    int q, int np,
+                                                    const double *__restrict__ xpart,
+                                                    const int *__restrict__ xr) {
   const TriRanges r = tri_ranges(K, kt, k);
   double s = 0.0;
   for (int t = r.yb0 * ts + q; t < r.yb1 * ts; t += np)
     s += bpart[((size_t)t * nacc + v) * K + k];
   for (int t = r.yu0 * ts + q; t < r.yu1 * ts; t += np)
     s += bpart[((size_t)t * nacc + v) * K + k];
+  if (xr) {                                 // second halves of split tiles
+    const int bx = k >> 6;
+    for (int t = xr[2 * bx] * ts + q; t < xr[2 * bx + 1] * ts; t += np)
+      s += xpart[((size_t)t * nacc + v) * K + k];
+  }
   for (int t = r.x0 * ts + q; t < r.x1 * ts; t += np)
     s += kpart[((size_t)t * K + k) * nacc + v];
   return s;
@@ -636,10 +682,13 @@ __device__ __forceinline__ double tile_reduce2_part(const double *__restrict__ b
 __device__ __forceinline__ void tile_reduce2_block(const double *__restrict__ bpart,
                                                    const double *__restrict__ kpart, int K,
                                                    int kt, int ts, int nacc, int v, int k0,
-                                                   double *__restrict__ red, double *sh) {
+                                                   double *__restrict__ red, double *sh,
+                                                   const double *__restrict__ xpart = nullptr,
+                                                   const int *__restrict__ xr = nullptr) {
   const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
   const int k = k0 + lane;
-  sh[threadIdx.x] = k < K ? tile_reduce2_part(bpart, kpart, K, kt, ts, nacc, v, k, q, 4) : 0.0;
+  sh[threadIdx.x] =
+      k < K ? tile_reduce2_part(bpart, kpart, K, kt, ts, nacc, v, k, q, 4, xpart, xr) : 0.0;
   __syncthreads();
   if (q == 0 && k < K)
     red[(size_t)v * K + k] = ((sh[lane] + sh[64 + lane]) + sh[128 + lane]) + sh[192 + lane];

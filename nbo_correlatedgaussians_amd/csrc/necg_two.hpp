@@ -140,4 +140,66 @@ inline long long tri_count(int K, int kt) {
   return n;
 }
 
+// Tiles of the tri_tiles list of (K, kt), and how many of them lead the list
+// as whole tiles (kt kets, 64 bras).
+inline void tri_counts(int K, int kt, long long &nt, long long &nfull) {
+  nt = nfull = 0;
+  tri_tiles(K, kt, [&](int bx, int by) {
+    int l0, l1, lup;
+    tri_extent(K, kt, bx, by, l0, l1, lup);
+    if (l1 - l0 == kt && K - 64 * bx >= 64 && nt == nfull) ++nfull;
+    ++nt;
+  });
+}
+
+// Whether a tri_tiles list of nt tiles (nfull whole-kt tiles first) on ncu
+// CUs, one workgroup per CU, has its whole-tile last round split: a last
+// round of r = nt mod ncu <= ncu / 2 workgroups becomes 2 r half tiles of
+// kt / 2 kets, one round of half length.  Returns r, or 0 for no split.
+inline int tri_split_count(long long nt, long long nfull, int kt, int ncu) {
+  if (kt < 2 || (kt & 1) || ncu < 2 || nt < 2LL * ncu) return 0;
+  const long long r = nt % ncu;
+  if (r == 0 || 2 * r > ncu || r > nfull) return 0;
+  return (int)r;
+}
+
+// The tile list of a lower-triangle pass as the kernels take it (TriTile
+// words): the tri_tiles list with the last r whole tiles split into halves
+// (tri_split_count), the halves launched last, first and second half of each
+// tile side by side.  xr[2 bx], xr[2 bx + 1] = the extra-slot range of bra
+// tile bx's split tiles (contiguous: the split tiles are a suffix of the
+// whole tiles, in bra-tile order).  Returns r.
+inline int tri_tile_list(int K, int kt, int ncu, std::vector<int> &list, std::vector<int> &xr) {
+  std::vector<int> t;
+  long long nt, nfull;
+  tri_counts(K, kt, nt, nfull);
+  t.reserve((size_t)2 * nt);
+  tri_tiles(K, kt, [&](int bx, int by) {
+    t.push_back(bx);
+    t.push_back(by);
+  });
+  const int nbt = (K + 63) / 64;
+  xr.assign((size_t)2 * nbt, 0);
+  const int r = tri_split_count(nt, nfull, kt, ncu);
+  if (r == 0) {
+    list.swap(t);
+    return 0;
+  }
+  list.clear();
+  list.reserve((size_t)2 * (nt + r));
+  const long long s0 = nfull - r;
+  list.insert(list.end(), t.begin(), t.begin() + 2 * s0);
+  list.insert(list.end(), t.begin() + 2 * nfull, t.end());
+  for (int e = 0; e < r; ++e) {
+    const int bx = t[(size_t)2 * (s0 + e)], by = t[(size_t)2 * (s0 + e) + 1];
+    if (xr[(size_t)2 * bx + 1] == 0) xr[(size_t)2 * bx] = e;
+    xr[(size_t)2 * bx + 1] = e + 1;
+    for (unsigned h = 1; h <= 2; ++h) {
+      list.push_back((int)((unsigned)bx | ((unsigned)e << 16) | (h << 30)));
+      list.push_back(by);
+    }
+  }
+  return r;
+}
+
 }  // namespace necg

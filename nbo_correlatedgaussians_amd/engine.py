@@ -91,6 +91,10 @@ class Engine:
         _lib.raise_for(self.lib.necg_set_tiling(self._ctx, ket_tile, term_split, waves),
                        "necg_set_tiling")
 
+    def set_launch_cus(self, ncu: int = 0) -> None:
+        """Compute units the pair passes size their launches for (0: the device's)."""
+        _lib.raise_for(self.lib.necg_set_launch_cus(self._ctx, ncu), "necg_set_launch_cus")
+
     def tiling(self):
         a, b, c = ct.c_int32(), ct.c_int32(), ct.c_int32()
         self.lib.necg_get_tiling(self._ctx, ct.byref(a), ct.byref(b), ct.byref(c))

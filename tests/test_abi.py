@@ -37,6 +37,7 @@ def test_create_rejects_bad_arguments_without_gpu_work():
     assert lib.necg_create(ct.byref(desc), 0, ct.byref(out)) == _lib.NECG_ERR_UNSUPPORTED
     assert lib.necg_destroy(None) == _lib.NECG_ERR_ARG
     assert lib.necg_set_tiling(None, 1, 1, 1) == _lib.NECG_ERR_ARG
+    assert lib.necg_set_launch_cus(None, 8) == _lib.NECG_ERR_ARG
     r = ct.c_double(-1.0)
     assert lib.necg_fp64_fma_rate(0, 0, ct.byref(r)) == _lib.NECG_ERR_ARG
     assert lib.necg_fp64_fma_rate(0, 10, None) == _lib.NECG_ERR_ARG

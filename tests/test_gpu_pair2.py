@@ -67,6 +67,43 @@ def test_two_sided_tilings_agree_and_repeat(oracle_lib):
         assert eng.pair_form()[1]
 
 
+def test_two_sided_split_last_round(oracle_lib):
+    """Lists whose last round is split into half tiles (a smaller launch CU
+    count makes small K reach it) give the oracle's result, in the gradient
+    pass, its tape's backward and the element-only forward."""
+    import torch
+    s = necg.lih_system()
+    eng = _engine(s)
+    eng.set_pair_mode(1)
+    # (K, kt, CUs): 192 = 3 whole bra tiles, 6 (1 + 2 + 3) x 64 / kt tiles:
+    # kt 8 -> 48 tiles, 8 left over on 20 CUs; kt 4 -> 96, 16 on 40;
+    # kt 2 -> 192, 32 on 80.  K 150 adds the partial bra tile's tiles.
+    cases = ((192, 8, 20), (192, 4, 40), (192, 2, 80), (150, 8, 9), (150, 4, 21), (150, 2, 50))
+    for K in (192, 150):
+        iv, C = necg.generate_parameters(s, K, 5)
+        x = necg.pack_x(iv, C)
+        Eo, go = oracle_lib.Oracle(s).outprod(x)
+        xd = torch.as_tensor(x, device="cuda:0")
+        for KK, kt, ncu in cases:
+            if KK != K:
+                continue
+            print(f"split tiles K={K} kt={kt} cus={ncu}", flush=True)
+            eng.set_launch_cus(ncu)
+            eng.set_tiling(kt, 1, 0)
+            runs = [eng.outprod(x) for _ in range(2)]
+            for E, g in runs:
+                assert _close(E, g, Eo, go, 1e-12, 1e-9), (K, kt, ncu)
+            assert runs[0][0] == runs[1][0] and np.array_equal(runs[0][1], runs[1][1])
+            assert eng.pair_form()[1]
+            Ef, tape = eng.energy_fwd_t(xd)
+            gt = eng.energy_bwd_t(xd, tape).cpu().numpy()
+            assert float(Ef) == runs[0][0] and np.array_equal(gt, runs[0][1])
+            assert abs(eng.energy(x) - Eo) <= 1e-12 * abs(Eo)
+    eng.set_launch_cus(0)
+    eng.set_tiling(0, 0, 0)
+    eng.check()
+
+
 def test_two_sided_tape_and_vjp():
     """energy_fwd takes the two-sided pass and its tape gives the outprod
     gradient; a change of form drops the tape (the backward recomputes)."""
