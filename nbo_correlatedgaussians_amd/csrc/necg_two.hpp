@@ -164,11 +164,13 @@ inline int tri_split_count(long long nt, long long nfull, int kt, int ncu) {
 }
 
 // The tile list of a lower-triangle pass as the kernels take it (TriTile
-// words): the tri_tiles list with the last r whole tiles split into halves
-// (tri_split_count), the halves launched last, first and second half of each
-// tile side by side.  xr[2 bx], xr[2 bx + 1] = the extra-slot range of bra
-// tile bx's split tiles (contiguous: the split tiles are a suffix of the
-// whole tiles, in bra-tile order).  Returns r.
+// words): the tri_tiles list with its first r whole tiles split into halves
+// (tri_split_count) and moved to the end, first and second half of each tile
+// side by side.  The first tiles belong to the bra tiles with the fewest ket
+// tiles, whose reduction (tile_reduce2) has the least to sum: their extra
+// slots leave the longest reduction, the last bra tile's, as it was.
+// xr[2 bx], xr[2 bx + 1] = the extra-slot range of bra tile bx's split tiles
+// (contiguous: the split tiles are in bra-tile order).  Returns r.
 inline int tri_tile_list(int K, int kt, int ncu, std::vector<int> &list, std::vector<int> &xr) {
   std::vector<int> t;
   long long nt, nfull;
@@ -187,11 +189,9 @@ inline int tri_tile_list(int K, int kt, int ncu, std::vector<int> &list, std::ve
   }
   list.clear();
   list.reserve((size_t)2 * (nt + r));
-  const long long s0 = nfull - r;
-  list.insert(list.end(), t.begin(), t.begin() + 2 * s0);
-  list.insert(list.end(), t.begin() + 2 * nfull, t.end());
+  list.insert(list.end(), t.begin() + 2 * r, t.end());
   for (int e = 0; e < r; ++e) {
-    const int bx = t[(size_t)2 * (s0 + e)], by = t[(size_t)2 * (s0 + e) + 1];
+    const int bx = t[(size_t)2 * e], by = t[(size_t)2 * e + 1];
     if (xr[(size_t)2 * bx + 1] == 0) xr[(size_t)2 * bx] = e;
     xr[(size_t)2 * bx + 1] = e + 1;
     for (unsigned h = 1; h <= 2; ++h) {
