@@ -2,7 +2,9 @@
 tests/native/tri_tiles_check.cpp, built with hipcc against the library's own
 headers, checks for K = 1..300 and larger sizes and 12 ket-tile sizes that
 every unordered pair of functions is evaluated exactly once and that each
-function reduces exactly the tiles that wrote its partials."""
+function reduces exactly the tiles that wrote its partials; the same for the
+kernels' lists whose last round is split into half tiles (tri_tile_list), over
+a range of CU counts."""
 
 import os
 import shutil
