@@ -316,7 +316,9 @@ class EngineGraph:
     write new parameters into ``x`` in place, call ``replay()``, read ``f``
     and ``g``.  The capture follows one warm-up call on a side stream, which
     sizes the engine's workspace, so the captured sequence holds kernel
-    launches only."""
+    launches only.  The graph keeps the addresses of that workspace and of the
+    tile list: while it is replayed, the engine must not evaluate a larger K
+    or change its tiling or launch CU count, which can reallocate them."""
 
     def __init__(self, engine: Engine, x0):
         torch = _torch()

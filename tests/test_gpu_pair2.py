@@ -76,9 +76,10 @@ def test_two_sided_split_last_round(oracle_lib):
     eng = _engine(s)
     eng.set_pair_mode(1)
     # (K, kt, CUs): 192 = 3 whole bra tiles, 6 (1 + 2 + 3) x 64 / kt tiles:
-    # kt 8 -> 48 tiles, 8 left over on 20 CUs; kt 4 -> 96, 16 on 40;
-    # kt 2 -> 192, 32 on 80.  K 150 adds the partial bra tile's tiles.
-    cases = ((192, 8, 20), (192, 4, 40), (192, 2, 80), (150, 8, 9), (150, 4, 21), (150, 2, 50))
+    # kt 8 -> 48 tiles, 8 left over on 20 CUs and split; kt 4 -> 96, 16 on
+    # 40; kt 2 -> 192, 32 on 80.  K 150 adds the partial bra tile's tiles and
+    # the upper pairs: 33, 66 and 129 tiles, of which 3, 3 and 9 are split.
+    cases = ((192, 8, 20), (192, 4, 40), (192, 2, 80), (150, 8, 10), (150, 4, 21), (150, 2, 60))
     for K in (192, 150):
         iv, C = necg.generate_parameters(s, K, 5)
         x = necg.pack_x(iv, C)

@@ -276,15 +276,20 @@ def test_k2048_rotation_invariance():
     assert np.max(np.abs(g2[K * s.blk:] - g[K * s.blk:])) <= 1e-9 * scale
 
 
-def test_engine_graph_replays():
+@pytest.mark.parametrize("cus,kt", [(0, 0), (14, 4)])
+def test_engine_graph_replays(cus, kt):
     """The single-system evaluation captured in a HIP graph: every replay,
     with new parameters written into x in place and eager work between
-    replays, is bitwise the eager outprod."""
+    replays, is bitwise the eager outprod.  The second case launches for 14
+    CUs with ket tile 4: K = 96 then has 32 tiles, 4 of them split in halves
+    for the last round."""
     import torch
     from nbo_correlatedgaussians_amd.engine import Engine, EngineGraph
     s = necg.lih_system()
     K = 96
     eng = Engine(s, "cuda:0")
+    eng.set_launch_cus(cus)
+    eng.set_tiling(kt, 1 if kt else 0, 0)
     xs = [necg.pack_x(*necg.generate_parameters(s, K, seed)) for seed in (31, 32, 33)]
     G = EngineGraph(eng, xs[0])
     for rep in range(4):
