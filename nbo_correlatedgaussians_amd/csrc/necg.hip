@@ -563,6 +563,7 @@ void choose_tiling2(necg_ctx *c, int K, int &kt, int &ts) {
 // per-bra-tile extra-slot ranges.
 int ensure_tiles(necg_ctx *c, int K, int kt) {
   if (c->tiles_K == K && c->tiles_kt == kt) return NECG_OK;
+  if ((K + 63) / 64 > 0xffff) return NECG_ERR_UNSUPPORTED;   // bra tile index has 16 bits
   std::vector<int> full, xr;
   const int nsplit = tri_tile_list(K, kt, std::max(1, c->ncu), full, xr);
   const size_t ntile_words = full.size();

@@ -159,7 +159,7 @@ inline void tri_counts(int K, int kt, long long &nt, long long &nfull) {
 inline int tri_split_count(long long nt, long long nfull, int kt, int ncu) {
   if (kt < 2 || (kt & 1) || ncu < 2 || nt < 2LL * ncu) return 0;
   const long long r = nt % ncu;
-  if (r == 0 || 2 * r > ncu || r > nfull) return 0;
+  if (r == 0 || 2 * r > ncu || r > nfull || r > 0x3fff) return 0;   // e has 14 bits
   return (int)r;
 }
 
