@@ -434,6 +434,17 @@ __device__ __forceinline__ bool pair_term2(const BraLds<N> &b, const double *__r
       Phi[t] = __builtin_fma(8.0, Ma1[t], Phk[t]);
       d1[t] = st[t] - b.S(t);
     }
+    const double wo = wb * ov;
+    const double woa = wo * alpha;
+    // the shift and weight slots first: their LDS adds then drain under the
+    // matrix products below instead of in front of the ket side's LDS reads
+#pragma unroll
+    for (int t = 0; t < N3; ++t) {
+      sink.add(D::A_YH + t, __builtin_fma(2.0 * woa, d1[t], wo * Phi[t]));
+      sink.add(D::A_YS + t, 2.0 * wo * d1[t]);
+    }
+    sink.add(D::A_SH, woa);
+    sink.add(D::A_SS, wo);
     double Tm[N * N];
 #pragma unroll
     for (int i = 0; i < N; ++i)
@@ -445,8 +456,6 @@ __device__ __forceinline__ bool pair_term2(const BraLds<N> &b, const double *__r
           t = __builtin_fma(__builtin_fma(12.0, Np[sx(i, m)], La[sx(i, m)]), Bi[sx(m, j)], t);
         Tm[i * N + j] = t;
       }
-    const double wo = wb * ov;
-    const double woa = wo * alpha;
 #pragma unroll
     for (int i = 0; i < N; ++i)
 #pragma unroll
@@ -466,13 +475,6 @@ __device__ __forceinline__ bool pair_term2(const BraLds<N> &b, const double *__r
         sink.add(D::A_XH + t, __builtin_fma(woa, x0, wo * (g - dp)));
         sink.add(D::A_XS + t, wo * x0);
       }
-#pragma unroll
-    for (int t = 0; t < N3; ++t) {
-      sink.add(D::A_YH + t, __builtin_fma(2.0 * woa, d1[t], wo * Phi[t]));
-      sink.add(D::A_YS + t, 2.0 * wo * d1[t]);
-    }
-    sink.add(D::A_SH, woa);
-    sink.add(D::A_SS, wo);
   }
 
   // Ket side: Phi' = Phk, G' = B^-1 (12 A M A + La) B^-1, dif2 = st - S'.
