@@ -55,22 +55,19 @@ struct BraLds {
   __device__ __forceinline__ double AMA(int t) const { return f(F_AMA + t); }
 };
 
-// Fill the tile's bra table: field f of bra k0 + lane from the bra record.
+// Where field f of the tile's bra table comes from in the bra record.
 template <int N>
-__device__ __forceinline__ void fill_bra_lds(double *dst, const double *__restrict__ rec, int f,
-                                             int lane) {
+__device__ __forceinline__ int bra_lds_src(int f) {
   using D = Dim<N>;
   using L = BraLds<N>;
-  int src;
-  if (f < L::F_S) src = D::B_A + f;
-  else if (f < L::F_AS) src = D::B_S + (f - L::F_S);
-  else if (f < L::F_AMA) src = D::B_AS + (f - L::F_AS);
-  else if (f < L::F_I1) src = D::B_AMA + (f - L::F_AMA);
-  else if (f == L::F_I1) src = D::B_I1;
-  else if (f == L::F_PK) src = D::B_PK;
-  else if (f == L::F_C) src = D::B_C;
-  else src = D::B_TRMA;
-  dst[64 * f + lane] = rec[src];
+  if (f < L::F_S) return D::B_A + f;
+  if (f < L::F_AS) return D::B_S + (f - L::F_S);
+  if (f < L::F_AMA) return D::B_AS + (f - L::F_AS);
+  if (f < L::F_I1) return D::B_AMA + (f - L::F_AMA);
+  if (f == L::F_I1) return D::B_I1;
+  if (f == L::F_PK) return D::B_PK;
+  if (f == L::F_C) return D::B_C;
+  return D::B_TRMA;
 }
 
 // First ket of the upper pairs of a lower-triangle pass: when K is not a
@@ -568,15 +565,31 @@ __device__ __forceinline__ void pair2_tile(const double *__restrict__ bra,
   const int k = k0 + lane;
   const bool active = k < K;
   const int kk = active ? k : K - 1;
+  // The wave's share of the bra table (fields wv, wv + W, ...) and its first
+  // ket record: every load issued before the first store, so the tile start
+  // waits for one memory round trip, not one per field.
+  constexpr int NF = BraLds<N>::NF, FW = (NF + W - 1) / W;
+  double bv[FW];
+#pragma unroll
+  for (int i = 0; i < FW; ++i) {
+    const int f = wv + i * W;
+    bv[i] = f < NF ? bra[(size_t)kk * D::BS + bra_lds_src<N>(f)] : 0.0;
+  }
+  double first = 0.0;
+  if (l0 + wv < l1 && lane < KF) first = ket[((size_t)(l0 + wv) * nst + p0) * D::KS + lane];
   for (int v = 0; v < D::NACC; ++v) acc_s[wv * NA + v * 64 + lane] = 0.0;
   double *kacc = kacc_s + wv * 64;
   kacc[lane] = 0.0;
-  for (int f = wv; f < BraLds<N>::NF; f += W) fill_bra_lds<N>(bra_s, bra + (size_t)kk * D::BS, f, lane);
+#pragma unroll
+  for (int i = 0; i < FW; ++i) {
+    const int f = wv + i * W;
+    if (f < NF) bra_s[64 * f + lane] = bv[i];
+  }
+  double *kst = kst_s + wv * 64;
+  if (lane < KF) kst[lane] = first;
   __syncthreads();
   const BraLds<N> b{bra_s + lane};
   LdsSink sink{acc_s + wv * NA + lane};
-  double *kst = kst_s + wv * 64;
-  if (l0 + wv < l1 && lane < KF) kst[lane] = ket[((size_t)(l0 + wv) * nst + p0) * D::KS + lane];
   bool ok = true;
   for (int l = l0 + wv; l < l1; l += W) {
     const bool upper = l >= lup;
