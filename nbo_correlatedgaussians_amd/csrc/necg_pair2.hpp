@@ -625,12 +625,26 @@ __device__ __forceinline__ void pair2_tile(const double *__restrict__ bra,
   if (active && !ok) atomicOr(flag, (int)F_NOT_PD);
   __syncthreads();
   if (!active) return;
-  double *o = bpt + k;
-  for (int v = wv; v < D::NACC; v += W) {
-    double t = 0.0;
+  // The wave's slots (wv, wv + W, ...) summed over the W waves' records: all
+  // LDS reads first, then all stores, so the tile ends after one LDS and one
+  // memory round trip rather than one per slot.
+  // (Only the last round of slots can pass NACC - 1: its reads are clamped
+  // into range, with no branch, and its store is skipped.)
+  constexpr int VW = (D::NACC + W - 1) / W;
+  double t[VW];
 #pragma unroll
-    for (int u = 0; u < W; ++u) t += acc_s[u * NA + v * 64 + lane];
-    o[(size_t)v * K] = t;
+  for (int i = 0; i < VW; ++i) {
+    const int v = min(wv + i * W, D::NACC - 1);
+    double a = 0.0;
+#pragma unroll
+    for (int u = 0; u < W; ++u) a += acc_s[u * NA + v * 64 + lane];
+    t[i] = a;
+  }
+  double *o = bpt + k;
+#pragma unroll
+  for (int i = 0; i < VW; ++i) {
+    const int v = wv + i * W;
+    if ((i + 1) * W <= D::NACC || v < D::NACC) o[(size_t)v * K] = t[i];
   }
 }
 
