@@ -9,6 +9,7 @@ synchronise, and raise on any latched device error.
 from __future__ import annotations
 
 import ctypes as ct
+import threading
 
 import numpy as np
 
@@ -48,6 +49,10 @@ class Engine:
         _lib.raise_for(rc, "necg_create")
         self._ctx = ctx
         self.blk = system.blk
+        # pinned host and device buffers of the host-array outprod, reused
+        # while x keeps its length
+        self._staging = None
+        self._stage_lock = threading.Lock()
 
     # -- plumbing ----------------------------------------------------------
     def close(self):
@@ -232,13 +237,32 @@ class Engine:
             raise FloatingPointError("x contains NaN or Inf")
         return x
 
+    def _stage(self, n: int):
+        """(pinned x, pinned g, pinned f, device x, device g, device f) for x
+        of n doubles; made again only when n changes."""
+        st = self._staging
+        if st is None or st[0].numel() != n:
+            torch = _torch()
+            pin = [torch.empty(m, dtype=torch.float64, pin_memory=True) for m in (n, n, 1)]
+            st = (*pin, self._empty(n), self._empty(n), self._empty(1))
+            self._staging = st
+        return st
+
     def outprod(self, x):
         """Drop-in for the reference ``outprod(x) -> (f, g)``
-        (python/outprod.py:20-76): E = C^T H C / C^T S C and its gradient."""
+        (python/outprod.py:20-76): E = C^T H C / C^T S C and its gradient,
+        g a new array.  x and g cross through pinned buffers with copies on
+        the engine's stream and one synchronisation per call."""
         x = self._host_x(x)
-        f, g = self.outprod_t(x)
-        self.check("outprod")
-        return float(f.item()), g.cpu().numpy()
+        with self._stage_lock:
+            px, pg, pf, dx, dg, df = self._stage(x.size)
+            px.numpy()[:] = x
+            dx.copy_(px, non_blocking=True)
+            self.outprod_t(dx, df, dg)
+            pg.copy_(dg, non_blocking=True)
+            pf.copy_(df, non_blocking=True)
+            self.check("outprod")            # synchronises the stream
+            return float(pf[0]), pg.numpy().copy()
 
     def energy(self, x, with_hs: bool = False):
         x = self._host_x(x)
