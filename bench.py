@@ -439,8 +439,10 @@ def main():
                                + ("E = ecg_energy(x) (torch.autograd.Function) + E.backward()"
                                   if args.path == "autograd" else "outprod(x)"),
                    "K": K, "pairs": pairs, "parallelism": "single GPU"},
-        "roofline": {"bound": "mfma", "engine": "FP64 VALU (MI355X FP64 vector and matrix "
-                                                 "peaks are both 78.6 TFLOP/s)",
+        "roofline": {"bound": "fp64_valu",
+                     "engine": "FP64 vector ALU, not MFMA and not HBM (the n <= 6 pair "
+                               "algebra is too small and irregular for matrix cores; "
+                               "MI355X FP64 vector and matrix peaks are both 78.6 TFLOP/s)",
                      "kernel": "pair2_kernel<5>" if two_sided else "pair_kernel<5,MODE_SPLIT>",
                      "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64_TFLOPS,

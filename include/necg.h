@@ -181,6 +181,17 @@ int necg_set_tiling(necg_ctx *ctx, int32_t ket_tile, int32_t term_split, int32_t
  * into halves for the last round.  No counterpart in the reference. */
 int necg_set_launch_cus(necg_ctx *ctx, int32_t ncu);
 
+/* Workspace pin for HIP-graph capture: pin = 1 adds a pin, pin = 0 removes
+ * one.  A captured evaluation holds the addresses of the context's workspace
+ * and reads its lower-triangle tile list, which belong to the K of the
+ * capture.  While any pin is held, a call that would reallocate a workspace
+ * buffer or rebuild the tile list (an evaluation at another K, or one whose
+ * pass needs more workspace than the captured one) returns NECG_ERR_ARG
+ * instead, and so do necg_set_tiling, necg_set_pair_mode and
+ * necg_set_launch_cus.  Evaluations that fit the pinned workspace run as
+ * usual.  No counterpart in the reference, which has no graph capture. */
+int necg_pin_workspace(necg_ctx *ctx, int32_t pin);
+
 /* Launch shape that the last pair-kernel launch used (for reports). */
 int necg_get_tiling(necg_ctx *ctx, int32_t *ket_tile, int32_t *term_split, int32_t *waves);
 
@@ -225,6 +236,13 @@ int necg_batch_matrices(necg_batch *batch, int32_t s, double *H, double *S, void
  * the autotuner.  Blocking. */
 int necg_batch_set_tiling(necg_batch *batch, double item_flops, int32_t grid_per_cu);
 
+/* Work-list pin for HIP-graph capture (pin = 1 adds, 0 removes one): a
+ * captured batch evaluation holds the addresses of the batch's work lists
+ * and partial buffers, which necg_batch_set_tiling replaces.  While any pin
+ * is held, necg_batch_set_tiling returns NECG_ERR_ARG.  No counterpart in the
+ * reference. */
+int necg_batch_pin(necg_batch *batch, int32_t pin);
+
 /* As necg_status, for a batch.  Blocking. */
 int necg_batch_status(necg_batch *batch, void *stream);
 
@@ -232,10 +250,12 @@ int necg_batch_status(necg_batch *batch, void *stream);
 
 typedef struct necg_pool necg_pool;
 
-/* A pool over contexts that have each completed one evaluation (necg_outprod
- * or necg_energy) at their working K; the pool reads their resident H, S,
- * H C, S C, C^T H C and C^T S C.  Trials are refused (NECG_ERR_ARG) if a
- * context was re-dimensioned since.  No counterpart in the reference: it has
+/* A pool over contexts that have each completed one evaluation (necg_outprod,
+ * necg_energy or necg_energy_fwd) at their working K; the pool reads their
+ * resident H, S, H C, S C, C^T H C and C^T S C.  Trials are refused
+ * (NECG_ERR_ARG) if a context was re-dimensioned since, or if a later call
+ * (necg_matrices, necg_matel, necg_energy_vjp, ...) rebuilt its tables or
+ * matrices without a full evaluation after it.  No counterpart in the reference: it has
  * no serving mode; a trial's row is the matel loop (python/outprod.py:49-54)
  * restricted to one row and its energy the quotient of python/outprod.py:65-68. */
 int necg_pool_create(necg_ctx *const *ctxs, int32_t nctx, necg_pool **out);

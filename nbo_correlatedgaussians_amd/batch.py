@@ -168,9 +168,13 @@ class Batch:
 
 class BatchGraph:
     """The batch evaluation captured in a HIP graph: write new parameters into
-    ``x`` (in place), call ``replay()``, read ``f`` and ``g``."""
+    ``x`` (in place), call ``replay()``, read ``f`` and ``g``.  The graph holds
+    the addresses of the batch's work lists and partial buffers, so it pins
+    them (``necg_batch_pin``): until ``release()`` (or garbage collection),
+    ``batch.set_tiling`` and ``autotune`` raise instead of replacing them."""
 
     def __init__(self, batch: Batch, x0):
+        self._pinned = False
         torch = _torch()
         self.batch = batch
         self.x = torch.as_tensor(np.asarray(x0, dtype=np.float64), device=batch.device).clone()
@@ -185,10 +189,27 @@ class BatchGraph:
         self.graph = torch.cuda.CUDAGraph()
         with torch.cuda.graph(self.graph):
             batch.outprod_t(self.x, self.f, self.g)
+        _lib.raise_for(batch.lib.necg_batch_pin(batch._h, 1), "necg_batch_pin")
+        self._pinned = True
 
     def replay(self):
+        if self.graph is None:
+            raise RuntimeError("the graph was released")
         self.graph.replay()
         return self.f, self.g
+
+    def release(self):
+        """Drop the graph and its pin on the batch's work lists."""
+        self.graph = None
+        if self._pinned:
+            self._pinned = False
+            self.batch.lib.necg_batch_pin(self.batch._h, 0)
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
 
 
 def autotune(batch: Batch, x, item_flops=(1.5e7, 3e7, 6e7, 1.2e8, 2.4e8), grid_per_cu=(0,),

@@ -364,10 +364,15 @@ int bra_c_off(int n) {
   }
 }
 
+// The capacity fields drop to 0 before any buffer they cover is freed and
+// rise only after every allocation succeeded, so a failed allocation leaves a
+// context whose next call reallocates instead of launching on null buffers.
 int ensure_K(necg_ctx *c, int K) {
   if (K <= c->Kcap) return NECG_OK;
+  if (c->pins) return NECG_ERR_ARG;   // a captured graph holds these addresses
   int rc;
   const size_t k = (size_t)K;
+  c->Kcap = 0;
   if ((rc = dev_alloc(&c->bra, k * bra_stride(c->n)))) return rc;
   if ((rc = dev_alloc(&c->ket, k * c->nst * ket_stride(c->n)))) return rc;
   if ((rc = dev_alloc(&c->H, k * k))) return rc;
@@ -385,6 +390,8 @@ int ensure_K(necg_ctx *c, int K) {
 
 int ensure_part(necg_ctx *c, size_t count) {
   if (count <= c->part_cap) return NECG_OK;
+  if (c->pins) return NECG_ERR_ARG;
+  c->part_cap = 0;
   int rc = dev_alloc(&c->part, count);
   if (rc) return rc;
   c->part_cap = count;
@@ -392,6 +399,8 @@ int ensure_part(necg_ctx *c, size_t count) {
 }
 int ensure_hpart(necg_ctx *c, size_t count) {
   if (count <= c->hpart_cap) return NECG_OK;
+  if (c->pins) return NECG_ERR_ARG;
+  c->hpart_cap = 0;
   int rc = dev_alloc(&c->hpart, count);
   if (rc) return rc;
   rc = dev_alloc(&c->spart, count);
@@ -426,6 +435,7 @@ void choose_tiling(necg_ctx *c, int K, int &kt, int &ts, int &w) {
 
 template <int N>
 int run_prep(necg_ctx *c, int K, const double *x, int with_c, hipStream_t st) {
+  c->gen++;
   hipLaunchKernelGGL((prep_bra<N>), dim3((K + 255) / 256), dim3(256), 0, st, x, K, with_c,
                      c->sc, c->bra, c->d_flag);
   const int nk = K * c->nst;
@@ -577,6 +587,7 @@ void choose_tiling2(necg_ctx *c, int K, int &kt, int &ts) {
 // per-bra-tile extra-slot ranges.
 int ensure_tiles(necg_ctx *c, int K, int kt) {
   if (c->tiles_K == K && c->tiles_kt == kt) return NECG_OK;
+  if (c->pins) return NECG_ERR_ARG;   // a captured graph reads the current list
   if ((K + 63) / 64 > 0xffff) return NECG_ERR_UNSUPPORTED;   // bra tile index has 16 bits
   std::vector<int> full, xr;
   const int nsplit = tri_tile_list(K, kt, std::max(1, c->ncu), full, xr);
@@ -601,10 +612,19 @@ int ensure_tiles(necg_ctx *c, int K, int kt) {
 
 int ensure_kpart(necg_ctx *c, size_t count) {
   if (count <= c->kpart_cap) return NECG_OK;
+  if (c->pins) return NECG_ERR_ARG;
+  c->kpart_cap = 0;
   int rc = dev_alloc(&c->kpart, count);
   if (rc) return rc;
   c->kpart_cap = count;
   return NECG_OK;
+}
+
+// A pinned context (necg_pin_workspace) runs the lower-triangle passes only
+// at the K whose tile list is on the device: checked before any launch, so a
+// refused call leaves the captured state untouched.
+bool pinned_elsewhere(const necg_ctx *c, int K) {
+  return c->pins && (K > c->Kcap || K != c->tiles_K);
 }
 
 // Timing events around one pair launch (necg_timing).
@@ -945,12 +965,15 @@ int necg_outprod(necg_ctx *c, int32_t K, const double *x, double *f, double *g, 
   if (!c || K < 1 || !x || !g) return NECG_ERR_ARG;
   NECG_REQUIRE_DEVICE((const void *)x, (const void *)f, (const void *)g);
   std::lock_guard<std::mutex> lk(c->mu);
+  if (pinned_elsewhere(c, K)) return NECG_ERR_ARG;
   c->tape_K = 0;
   DeviceGuard dg(c->device);
   int rc = ensure_K(c, K);
   if (rc) return rc;
   c->last_K = K;
-  DISPATCH_N(c->n, (do_outprod<NN>(c, K, x, f, g, (hipStream_t)stream)));
+  auto run = [&]() -> int { DISPATCH_N(c->n, (do_outprod<NN>(c, K, x, f, g, (hipStream_t)stream))); };
+  if ((rc = run()) == NECG_OK) c->full_gen = c->gen;
+  return rc;
 }
 
 int necg_energy(necg_ctx *c, int32_t K, const double *x, double *f, double *H, double *S,
@@ -958,12 +981,15 @@ int necg_energy(necg_ctx *c, int32_t K, const double *x, double *f, double *H, d
   if (!c || K < 1 || !x) return NECG_ERR_ARG;
   NECG_REQUIRE_DEVICE((const void *)x, (const void *)f, (const void *)H, (const void *)S);
   std::lock_guard<std::mutex> lk(c->mu);
+  if (pinned_elsewhere(c, K)) return NECG_ERR_ARG;
   c->tape_K = 0;
   DeviceGuard dg(c->device);
   int rc = ensure_K(c, K);
   if (rc) return rc;
   c->last_K = K;
-  DISPATCH_N(c->n, (do_energy<NN>(c, K, x, f, H, S, (hipStream_t)stream)));
+  auto run = [&]() -> int { DISPATCH_N(c->n, (do_energy<NN>(c, K, x, f, H, S, (hipStream_t)stream))); };
+  if ((rc = run()) == NECG_OK) c->full_gen = c->gen;
+  return rc;
 }
 
 int necg_energy_fwd(necg_ctx *c, int32_t K, const double *x, double *f, int64_t *tape,
@@ -971,6 +997,7 @@ int necg_energy_fwd(necg_ctx *c, int32_t K, const double *x, double *f, int64_t 
   if (!c || K < 1 || !x || !tape) return NECG_ERR_ARG;
   NECG_REQUIRE_DEVICE((const void *)x, (const void *)f);
   std::lock_guard<std::mutex> lk(c->mu);
+  if (pinned_elsewhere(c, K)) return NECG_ERR_ARG;
   c->tape_K = 0;
   *tape = 0;
   DeviceGuard dg(c->device);
@@ -978,7 +1005,10 @@ int necg_energy_fwd(necg_ctx *c, int32_t K, const double *x, double *f, int64_t 
   if (rc) return rc;
   c->last_K = K;
   auto run = [&]() -> int { DISPATCH_N(c->n, (do_energy_fwd<NN>(c, K, x, f, (hipStream_t)stream))); };
-  if ((rc = run()) == NECG_OK) *tape = c->tape_id = ++c->tape_seq;
+  if ((rc = run()) == NECG_OK) {
+    *tape = c->tape_id = ++c->tape_seq;
+    c->full_gen = c->gen;
+  }
   return rc;
 }
 
@@ -998,6 +1028,7 @@ int necg_energy_vjp(necg_ctx *c, int32_t K, const double *x, const double *gbar,
   if (!c || K < 1 || !x || !gx) return NECG_ERR_ARG;
   NECG_REQUIRE_DEVICE((const void *)x, (const void *)gbar, (const void *)gx);
   std::lock_guard<std::mutex> lk(c->mu);
+  if (pinned_elsewhere(c, K)) return NECG_ERR_ARG;
   c->tape_K = 0;
   if (c->last_K != K || K > c->Kcap) return NECG_ERR_ARG;  // needs necg_energy first
   DeviceGuard dg(c->device);
@@ -1008,6 +1039,7 @@ int necg_matrices(necg_ctx *c, int32_t K, const double *iv, double *H, double *S
   if (!c || K < 1 || !iv) return NECG_ERR_ARG;
   NECG_REQUIRE_DEVICE((const void *)iv, (const void *)H, (const void *)S);
   std::lock_guard<std::mutex> lk(c->mu);
+  if (pinned_elsewhere(c, K)) return NECG_ERR_ARG;
   c->tape_K = 0;
   DeviceGuard dg(c->device);
   int rc = ensure_K(c, K);
@@ -1020,6 +1052,7 @@ int necg_kinetic_potential(necg_ctx *c, int32_t K, const double *x, double *T, d
   if (!c || K < 1 || !x || !T || !V) return NECG_ERR_ARG;
   NECG_REQUIRE_DEVICE((const void *)x, (const void *)T, (const void *)V);
   std::lock_guard<std::mutex> lk(c->mu);
+  if (pinned_elsewhere(c, K)) return NECG_ERR_ARG;
   c->tape_K = 0;
   DeviceGuard dg(c->device);
   int rc = ensure_K(c, K);
@@ -1085,6 +1118,7 @@ int necg_set_tiling(necg_ctx *c, int32_t ket_tile, int32_t term_split, int32_t w
   if (!c || ket_tile < 0 || term_split < 0) return NECG_ERR_ARG;
   if (waves != 0 && waves != pair_waves(c->n)) return NECG_ERR_ARG;  // fixed per n
   std::lock_guard<std::mutex> lk(c->mu);
+  if (c->pins) return NECG_ERR_ARG;
   c->req_kt = ket_tile;
   c->req_ts = term_split;
   c->req_w = waves;
@@ -1119,6 +1153,7 @@ int necg_timing(necg_ctx *c, int32_t mode, double *ms, int64_t *launches) {
 int necg_set_pair_mode(necg_ctx *c, int32_t mode) {
   if (!c || mode < 0 || mode > 2) return NECG_ERR_ARG;
   std::lock_guard<std::mutex> lk(c->mu);
+  if (c->pins) return NECG_ERR_ARG;
   if (mode == 1 && !c->two_ok) return NECG_ERR_UNSUPPORTED;
   c->pair_mode = mode;
   c->tape_K = 0;
@@ -1136,8 +1171,21 @@ int necg_pair_form(necg_ctx *c, int32_t *available, int32_t *last) {
 int necg_set_launch_cus(necg_ctx *c, int32_t ncu) {
   if (!c || ncu < 0) return NECG_ERR_ARG;
   std::lock_guard<std::mutex> lk(c->mu);
+  if (c->pins) return NECG_ERR_ARG;
   c->ncu = ncu > 0 ? ncu : c->dev_ncu;
   c->tiles_K = 0;   // the tile list depends on it
+  return NECG_OK;
+}
+
+int necg_pin_workspace(necg_ctx *c, int32_t pin) {
+  if (!c) return NECG_ERR_ARG;
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (pin) {
+    c->pins++;
+  } else {
+    if (c->pins == 0) return NECG_ERR_ARG;
+    c->pins--;
+  }
   return NECG_OK;
 }
 

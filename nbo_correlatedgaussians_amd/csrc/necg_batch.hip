@@ -376,6 +376,7 @@ struct necg_batch {
   double item_flops = 0.0;  // target cost of one pair tile (model FLOP)
   int grid_per_cu = 0;      // persistent workgroups per CU (0: occupancy)
   long long nx = 0;
+  int pins = 0;  // necg_batch_pin: captured graphs hold the work lists
   std::mutex mu;
   std::vector<int> in_n, in_nst, in_K;
   std::vector<std::vector<double>> in_sml, in_sms, in_scvv;
@@ -675,10 +676,23 @@ int necg_batch_layout(necg_batch *B, int64_t *x_offsets) {
 int necg_batch_set_tiling(necg_batch *B, double item_flops, int32_t grid_per_cu) {
   if (!B || item_flops < 0 || grid_per_cu < 0) return NECG_ERR_ARG;
   std::lock_guard<std::mutex> lk(B->mu);
+  if (B->pins) return NECG_ERR_ARG;  // a captured evaluation uses the current lists
   DevGuard g(B->device);
   B->item_flops = item_flops;
   B->grid_per_cu = grid_per_cu;
   return plan(B);
+}
+
+int necg_batch_pin(necg_batch *B, int32_t pin) {
+  if (!B) return NECG_ERR_ARG;
+  std::lock_guard<std::mutex> lk(B->mu);
+  if (pin) {
+    B->pins++;
+  } else {
+    if (B->pins == 0) return NECG_ERR_ARG;
+    B->pins--;
+  }
+  return NECG_OK;
 }
 
 int necg_batch_outprod_masked(necg_batch *B, const int32_t *active, const double *x, double *f,

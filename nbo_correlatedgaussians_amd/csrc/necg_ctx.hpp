@@ -62,6 +62,15 @@ struct necg_ctx {
   int tape_K = 0, tape_ntiles = 0;
   int tape_two = 0, tape_kt = 0, tape_ts = 0;
   long long tape_id = 0, tape_seq = 0;
+  // state generations: `gen` counts the rebuilds of the bra/ket tables (every
+  // evaluation entry point); `full_gen` is the generation whose H, S, H C,
+  // S C and quotient were all formed from one x (a full evaluation at
+  // last_K).  A pool reads the resident state only while they are equal.
+  long long gen = 0, full_gen = -1;
+  // workspace pins (necg_pin_workspace): while > 0, no call may move a
+  // workspace buffer or rewrite the tile list, since a captured HIP graph
+  // holds their addresses and contents
+  int pins = 0;
   // pair-kernel timing (necg_timing)
   bool timing = false;
   std::vector<hipEvent_t> ev;  // start/stop pairs, reused

@@ -396,10 +396,15 @@ struct necg_pool {
 
 namespace {
 
+// The resident state of every context, taken under the context's lock; false
+// if a context holds no consistent state (no full evaluation yet, or a later
+// call rebuilt its tables or matrices without forming the quotient again).
 bool snapshot(necg_pool *P, std::vector<PoolSys> &out) {
   out.clear();
   for (necg_ctx *c : P->ctx) {
+    std::lock_guard<std::mutex> lk(c->mu);
     if (c->last_K < 1 || !c->bra || !c->H || !c->HC || !c->scal) return false;
+    if (c->full_gen != c->gen) return false;
     PoolSys b{};
     b.sc = c->sc;
     b.n = c->n;

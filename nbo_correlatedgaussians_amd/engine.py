@@ -340,11 +340,14 @@ class EngineGraph:
     write new parameters into ``x`` in place, call ``replay()``, read ``f``
     and ``g``.  The capture follows one warm-up call on a side stream, which
     sizes the engine's workspace, so the captured sequence holds kernel
-    launches only.  The graph keeps the addresses of that workspace and of the
-    tile list: while it is replayed, the engine must not evaluate a larger K
-    or change its tiling or launch CU count, which can reallocate them."""
+    launches only.  The graph keeps the addresses of that workspace and reads
+    the tile list of its K, so it pins them (``necg_pin_workspace``): until
+    ``release()`` (or garbage collection), an engine call at another K, one
+    that needs more workspace, or a change of tiling, pair form or launch CU
+    count raises instead of moving or rewriting them."""
 
     def __init__(self, engine: Engine, x0):
+        self._pinned = False
         torch = _torch()
         self.engine = engine
         self.x = engine._dev(x0).clone()
@@ -360,7 +363,24 @@ class EngineGraph:
         self.graph = torch.cuda.CUDAGraph()
         with torch.cuda.graph(self.graph):
             engine.outprod_t(self.x, self.f, self.g)
+        _lib.raise_for(engine.lib.necg_pin_workspace(engine._ctx, 1), "necg_pin_workspace")
+        self._pinned = True
 
     def replay(self):
+        if self.graph is None:
+            raise RuntimeError("the graph was released")
         self.graph.replay()
         return self.f, self.g
+
+    def release(self):
+        """Drop the graph and its pin on the engine's workspace."""
+        self.graph = None
+        if self._pinned:
+            self._pinned = False
+            self.engine.lib.necg_pin_workspace(self.engine._ctx, 0)
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass

@@ -577,6 +577,136 @@ done:
   return rc;
 }
 
+/* outprod as above, every sum (the term sums of each element and gradient
+ * block, C^T H C, C^T S C, the GRADA row sums and GRADC) carried in long
+ * double (x87 80-bit, 64-bit significand).  The per-term values are the same
+ * double-precision matel/gradel results.  Returns the double-accumulated
+ * (E, g) of necg_oracle_outprod in f[0], g and the long-double-accumulated
+ * ones in f[1], gl, so that one pass gives both the reference's summation
+ * and a value whose only rounding is that of the terms themselves: the
+ * full-size fixture of the K = 2048 evaluation is written from it. */
+int necg_oracle_outprod_ld(const osys *s, int K, const double *x, double *f, double *g,
+                           double *gl, int nthreads) {
+  const int n = s->n, blk = n * (n + 1) / 2 + 3 * n;
+  const double *iv = x, *C = x + (size_t)K * blk;
+  double *H = (double *)malloc(sizeof(double) * K * K);
+  double *S = (double *)malloc(sizeof(double) * K * K);
+  long double *HL = (long double *)malloc(sizeof(long double) * K * K);
+  long double *SL = (long double *)malloc(sizeof(long double) * K * K);
+  int bad = 0, rc = 0;
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : bad)
+#endif
+  for (int i = 0; i < K; ++i) {
+    for (int j = 0; j <= i; ++j) {
+      double h = 0.0, o = 0.0;
+      long double hl = 0.0L, ol = 0.0L;
+      for (int stf = 0; stf < s->nst; ++stf) {
+        double e[4];
+        if (!necg_oracle_matel(s, iv, stf, i, j, e)) { bad++; continue; }
+        h += s->scvv[stf] * (e[1] + e[2] - e[3]);
+        o += s->scvv[stf] * e[0];
+        hl += (long double)s->scvv[stf] * ((long double)e[1] + e[2] - e[3]);
+        ol += (long double)s->scvv[stf] * e[0];
+      }
+      H[(size_t)i * K + j] = H[(size_t)j * K + i] = h;
+      S[(size_t)i * K + j] = S[(size_t)j * K + i] = o;
+      HL[(size_t)i * K + j] = HL[(size_t)j * K + i] = hl;
+      SL[(size_t)i * K + j] = SL[(size_t)j * K + i] = ol;
+    }
+  }
+  if (bad) { rc = 1; goto done; }
+  double cHc = 0.0, cSc = 0.0;
+  long double cHcl = 0.0L, cScl = 0.0L;
+  for (int j = 0; j < K; ++j) {
+    double ch = 0.0, cs = 0.0;
+    long double chl = 0.0L, csl = 0.0L;
+    for (int i = 0; i < K; ++i) {
+      ch += C[i] * H[(size_t)i * K + j];
+      cs += C[i] * S[(size_t)i * K + j];
+      chl += (long double)C[i] * HL[(size_t)i * K + j];
+      csl += (long double)C[i] * SL[(size_t)i * K + j];
+    }
+    cHc += ch * C[j];
+    cSc += cs * C[j];
+    cHcl += chl * C[j];
+    cScl += csl * C[j];
+  }
+  if (cSc == 0.0 || cScl == 0.0L) { rc = 2; goto done; }
+  const double eng = cHc / cSc, norm = 1.0 / cSc;
+  const long double engl = cHcl / cScl, norml = 1.0L / cScl;
+  f[0] = eng;
+  f[1] = (double)engl;
+#ifdef _OPENMP
+#pragma omp parallel reduction(+ : bad)
+#endif
+  {
+    double tg[64], tgs[64];
+    long double *dl = (long double *)malloc(sizeof(long double) * 2 * blk * K);
+    double *dd = (double *)malloc(sizeof(double) * 2 * blk * K);
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 1)
+#endif
+    for (int l = 0; l < K; ++l) {
+      int ok = 1;
+      for (int k = 0; k < K && ok; ++k) {
+        double a[64], b[64];
+        long double al[64], bl[64];
+        for (int r = 0; r < blk; ++r) { a[r] = b[r] = 0.0; al[r] = bl[r] = 0.0L; }
+        for (int stf = 0; stf < s->nst; ++stf) {
+          if (!necg_oracle_gradel(s, iv, stf, l, k, tg, tgs)) { ok = 0; break; }
+          for (int r = 0; r < blk; ++r) {
+            a[r] += s->scvv[stf] * tg[r];
+            b[r] += s->scvv[stf] * tgs[r];
+            al[r] += (long double)s->scvv[stf] * tg[r];
+            bl[r] += (long double)s->scvv[stf] * tgs[r];
+          }
+        }
+        for (int r = 0; r < blk; ++r) {
+          dd[(size_t)r * K + k] = a[r];
+          dd[(size_t)(blk + r) * K + k] = b[r];
+          dl[(size_t)r * K + k] = al[r];
+          dl[(size_t)(blk + r) * K + k] = bl[r];
+        }
+      }
+      if (!ok) { bad++; continue; }
+      for (int r = 0; r < blk; ++r) {
+        double val = 0.0;
+        long double vall = 0.0L;
+        for (int i = 0; i < K; ++i) {
+          const double w = (i != l) ? 2.0 : 1.0;
+          double gm = norm * (dd[(size_t)r * K + i] - eng * dd[(size_t)(blk + r) * K + i]);
+          long double gml = norml * (dl[(size_t)r * K + i] - engl * dl[(size_t)(blk + r) * K + i]);
+          val += w * C[i] * C[l] * gm;
+          vall += (long double)w * C[i] * C[l] * gml;
+        }
+        g[(size_t)l * blk + r] = val;
+        gl[(size_t)l * blk + r] = (double)vall;
+      }
+    }
+    free(dl);
+    free(dd);
+  }
+  if (bad) { rc = 1; goto done; }
+  for (int i = 0; i < K; ++i) {  /* GRADC (python/make_grad.py:85) */
+    double a = 0.0;
+    long double al = 0.0L;
+    for (int j = 0; j < K; ++j) {
+      a += (H[(size_t)i * K + j] - eng * S[(size_t)i * K + j]) * C[j];
+      al += (HL[(size_t)i * K + j] - engl * SL[(size_t)i * K + j]) * C[j];
+    }
+    g[(size_t)K * blk + i] = 2.0 * norm * a;
+    gl[(size_t)K * blk + i] = (double)(2.0L * norml * al);
+  }
+done:
+  free(H);
+  free(S);
+  free(HL);
+  free(SL);
+  return rc;
+}
+
 /* Per-pair sums over all terms: H_ij, S_ij of the lower-triangle element
  * (bra max(i,j), ket min(i,j)) and the DH/DS blocks of ordered pair (i, j). */
 int necg_oracle_pair_sums(const osys *s, const double *iv, int npairs, const int32_t *pairs,

@@ -48,6 +48,7 @@ def _load():
         _lib.necg_oracle_gradel.argtypes = [P, P, ct.c_int, ct.c_int, ct.c_int, P, P]
         _lib.necg_oracle_matrices.argtypes = [P, ct.c_int, P, P, P, ct.c_int]
         _lib.necg_oracle_outprod.argtypes = [P, ct.c_int, P, P, P, P, P, ct.c_int]
+        _lib.necg_oracle_outprod_ld.argtypes = [P, ct.c_int, P, P, P, P, ct.c_int]
         _lib.necg_oracle_grada_rows.argtypes = [P, ct.c_int, P, ct.c_double, ct.c_double,
                                                 ct.c_int, ct.c_int, P, ct.c_int]
         _lib.necg_oracle_pair_sums.argtypes = [P, P, ct.c_int, P, P, P, P, P]
@@ -126,6 +127,21 @@ class Oracle:
         if return_hs:
             return float(f[0]), g, H, S
         return float(f[0]), g
+
+    def outprod_ld(self, x, nthreads=0):
+        """(E, g) with double sums as ``outprod`` and (E, g) with every sum in
+        long double, from one pass over the pair terms."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        K = x.size // (self.sys.blk + 1)
+        f = np.zeros(2)
+        g = np.zeros(x.size)
+        gl = np.zeros(x.size)
+        rc = self.lib.necg_oracle_outprod_ld(self._po, K, _p(x), _p(f), _p(g), _p(gl), nthreads)
+        if rc == 1:
+            raise np.linalg.LinAlgError("singular pair matrix")
+        if rc == 2:
+            raise ZeroDivisionError("C^T S C == 0")
+        return (float(f[0]), g), (float(f[1]), gl)
 
     def grada_rows(self, x, E, norm, row0, nrows, nthreads=0):
         x = np.ascontiguousarray(x, dtype=np.float64)

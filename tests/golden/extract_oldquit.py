@@ -1,18 +1,20 @@
 """Extract the LiH non-BO system of the reference input ``oldquit.dat`` into
-``lih_oldquit.json`` (run once, in a checkout that has the reference).
+the package's ``data/lih_oldquit.json`` (run once, in a checkout that has the
+reference).
 
 The reference input is MIT-licensed (c) 2026 Mauricio Cafiero; the numbers are
 data (mass tensor, charges, symmetry terms, and the 12-function basis).  The
 token layout is the one ``python/read_input.py:16-157`` reads.
 
-    python extract_oldquit.py /path/to/reference/oldquit.dat
+    python tests/golden/extract_oldquit.py /path/to/reference/oldquit.dat
 """
 
 import json
 import os
 import sys
 
-sys.path.insert(0, os.path.join(os.path.dirname(__file__), "..", ".."))
+ROOT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "..", "..")
+sys.path.insert(0, ROOT)
 
 from nbo_correlatedgaussians_amd.system import parse_tokens  # noqa: E402
 
@@ -37,11 +39,13 @@ def main(path):
         "iv": iv.tolist(),
         "C": C.tolist(),
     }
-    dst = os.path.join(os.path.dirname(__file__), "lih_oldquit.json")
+    dst = os.path.join(ROOT, "nbo_correlatedgaussians_amd", "data", "lih_oldquit.json")
     with open(dst, "w", encoding="utf-8") as fh:
         json.dump(out, fh, separators=(",", ":"))
     print("wrote", dst)
 
 
 if __name__ == "__main__":
-    main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference/oldquit.dat")
+    if len(sys.argv) != 2:
+        sys.exit("usage: extract_oldquit.py /path/to/reference/oldquit.dat")
+    main(sys.argv[1])

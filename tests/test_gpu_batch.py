@@ -91,6 +91,10 @@ def test_batch_graph_replays_and_tilings():
     f, g = G.replay()
     E1, _ = _single(systems[1], x2[b.offsets[1]:b.offsets[2]])
     assert abs(f[1].item() - E1) <= 1e-13 * max(1.0, abs(E1))
+    from nbo_correlatedgaussians_amd import _lib
+    with pytest.raises(_lib.NecgError):     # the graph pins the work lists
+        b.set_tiling(1e6, 0)
+    G.release()
     best, res = autotune(b, xa, item_flops=(1e6, 3e7, 3e8), reps=1)
     for (E, gi), (E2, g2) in zip(ref, b.outprod([c[1] for c in cases])):
         assert abs(E - E2) <= 1e-13 * max(1.0, abs(E))

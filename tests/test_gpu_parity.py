@@ -301,6 +301,24 @@ def test_engine_graph_replays(cus, kt):
         Eg, gg = float(f.item()), g.cpu().numpy()
         Ee, ge = eng.outprod(x)          # eager work between replays
         assert Eg == Ee and np.array_equal(gg, ge), rep
+    # while the graph lives, nothing may move or rewrite what it captured:
+    # another K (smaller: a new tile list; larger: new buffers) and the
+    # launch-shape setters are refused, and the replay still matches
+    from nbo_correlatedgaussians_amd import _lib
+    for Kx in (K - 32, K + 32):
+        with pytest.raises(_lib.NecgError):
+            eng.outprod(necg.pack_x(*necg.generate_parameters(s, Kx, 34)))
+    for call in (lambda: eng.set_tiling(8, 1, 0), lambda: eng.set_launch_cus(0),
+                 lambda: eng.set_pair_mode(2)):
+        with pytest.raises(_lib.NecgError):
+            call()
+    G.x.copy_(torch.as_tensor(xs[1], device="cuda:0"))
+    f, g = G.replay()
+    Ee, ge = eng.outprod(xs[1])
+    assert float(f.item()) == Ee and np.array_equal(g.cpu().numpy(), ge)
+    G.release()
+    E64, _ = eng.outprod(necg.pack_x(*necg.generate_parameters(s, K - 32, 34)))
+    assert np.isfinite(E64)
 
 
 def test_host_pointers_are_refused():
