@@ -35,7 +35,8 @@ enum necg_status_code {
   NECG_ERR_NOT_PD = 5,      /* A_k + A_l' not positive definite / L_ii == 0    */
   NECG_ERR_CSC = 6,         /* C^T S C <= 0 (reference: ZeroDivisionError)     */
   NECG_ERR_NONFINITE = 7,   /* NaN or Inf in x or in a result                  */
-  NECG_ERR_STALE = 8        /* necg_energy_bwd without its necg_energy_fwd     */
+  NECG_ERR_STALE = 8,       /* necg_energy_bwd without its necg_energy_fwd     */
+  NECG_ERR_SCRATCH = 9      /* workspace above necg_outprod_noalloc's scratch  */
 };
 
 /* The physics block of an input file: the module globals filled by
@@ -64,6 +65,22 @@ int necg_destroy(necg_ctx *ctx);
  * python/make_grad.py:18-90) and the Fortran SFUN outprod(n, x, f, g)
  * (source/outprod.f90:2).  x, f, g: device. Async. */
 int necg_outprod(necg_ctx *ctx, int32_t K, const double *x, double *f, double *g, void *stream);
+
+/* necg_outprod without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP-graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same kernels, same results (bitwise) as necg_outprod at the
+ * same K and launch shape; every workspace buffer (bra/ket tables, H, S,
+ * partials, tile list) is carved out of one static 64 MiB device array per
+ * GPU, NECG_ERR_SCRATCH if the evaluation needs more (LiH, n = 5 with 24
+ * terms: K = 500 fits, K = 600 does not).  The context's own workspace, resident
+ * state and pins are untouched (its tape is dropped, as by any evaluation).
+ * All calls on one GPU share that array, which keeps the last call's
+ * contents: calls on one stream are ordered, but calls that overlap on one
+ * GPU (other streams, other threads, other contexts) race.  x, f, g: device.
+ * Async. */
+int necg_outprod_noalloc(necg_ctx *ctx, int32_t K, const double *x, double *f, double *g,
+                         void *stream);
 
 /* Forward half of outprod: H, S (K*K row-major, symmetric) and *f = E.
  * Replaces the matel loop + quotient (python/outprod.py:49-68).  H and S may be

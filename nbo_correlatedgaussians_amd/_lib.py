@@ -22,8 +22,9 @@ NECG_ERR_NOT_PD = 5
 NECG_ERR_CSC = 6
 NECG_ERR_NONFINITE = 7
 NECG_ERR_STALE = 8
+NECG_ERR_SCRATCH = 9
 
-EXPORTS = ("necg_create", "necg_destroy", "necg_outprod", "necg_energy",
+EXPORTS = ("necg_create", "necg_destroy", "necg_outprod", "necg_outprod_noalloc", "necg_energy",
            "necg_kinetic_potential", "necg_dipole", "necg_set_dipole_mode", "necg_device_visible",
            "necg_set_pair_mode", "necg_pair_form", "necg_fp64_fma_rate",
            "necg_energy_fwd", "necg_energy_bwd",
@@ -91,6 +92,7 @@ def load() -> ct.CDLL:
         "necg_create": [ct.POINTER(NecgSystem), ct.c_int, ct.POINTER(P)],
         "necg_destroy": [P],
         "necg_outprod": [P, I, P, P, P, P],
+        "necg_outprod_noalloc": [P, I, P, P, P, P],
         "necg_energy": [P, I, P, P, P, P, P],
         "necg_energy_vjp": [P, I, P, P, P, P],
         "necg_matrices": [P, I, P, P, P, P],

@@ -333,6 +333,22 @@ int dev_alloc(double **p, size_t count) {
   return hip_status(hipMalloc((void **)p, count * sizeof(double)));
 }
 
+// Workspace buffers of the context: device allocations, or in
+// allocation-free mode slices of the arena (256-byte aligned; nothing is
+// freed, the previous pointers belong to the context's own workspace).
+int ws_alloc(necg_ctx *c, void **p, size_t bytes) {
+  if (!c->arena) return dev_alloc((double **)p, (bytes + sizeof(double) - 1) / sizeof(double));
+  *p = nullptr;
+  const size_t take = (bytes + 255) & ~(size_t)255;
+  if (c->arena_used + take > c->arena_cap) return NECG_ERR_SCRATCH;
+  *p = c->arena + c->arena_used;
+  c->arena_used += take;
+  return NECG_OK;
+}
+int ws_alloc(necg_ctx *c, double **p, size_t count) {
+  return ws_alloc(c, (void **)p, count * sizeof(double));
+}
+
 int bra_stride(int n) {
   switch (n) {
     case 1: return Dim<1>::BS;
@@ -369,41 +385,41 @@ int bra_c_off(int n) {
 // context whose next call reallocates instead of launching on null buffers.
 int ensure_K(necg_ctx *c, int K) {
   if (K <= c->Kcap) return NECG_OK;
-  if (c->pins) return NECG_ERR_ARG;   // a captured graph holds these addresses
+  if (c->pins && !c->arena) return NECG_ERR_ARG;   // a captured graph holds these addresses
   int rc;
   const size_t k = (size_t)K;
   c->Kcap = 0;
-  if ((rc = dev_alloc(&c->bra, k * bra_stride(c->n)))) return rc;
-  if ((rc = dev_alloc(&c->ket, k * c->nst * ket_stride(c->n)))) return rc;
-  if ((rc = dev_alloc(&c->H, k * k))) return rc;
-  if ((rc = dev_alloc(&c->S, k * k))) return rc;
-  if ((rc = dev_alloc(&c->HC, k))) return rc;
-  if ((rc = dev_alloc(&c->SC, k))) return rc;
-  if ((rc = dev_alloc(&c->scal, 8))) return rc;
+  if ((rc = ws_alloc(c, &c->bra, k * bra_stride(c->n)))) return rc;
+  if ((rc = ws_alloc(c, &c->ket, k * c->nst * ket_stride(c->n)))) return rc;
+  if ((rc = ws_alloc(c, &c->H, k * k))) return rc;
+  if ((rc = ws_alloc(c, &c->S, k * k))) return rc;
+  if ((rc = ws_alloc(c, &c->HC, k))) return rc;
+  if ((rc = ws_alloc(c, &c->SC, k))) return rc;
+  if ((rc = ws_alloc(c, &c->scal, 8))) return rc;
   const size_t nchunk = (k + 63) / 64;  // H C / S C partials per 64 columns
-  if ((rc = dev_alloc(&c->hcp, nchunk * k))) return rc;
-  if ((rc = dev_alloc(&c->scp, nchunk * k))) return rc;
-  if ((rc = dev_alloc(&c->red, (size_t)Dim<kMaxN>::NACC * k))) return rc;
+  if ((rc = ws_alloc(c, &c->hcp, nchunk * k))) return rc;
+  if ((rc = ws_alloc(c, &c->scp, nchunk * k))) return rc;
+  if ((rc = ws_alloc(c, &c->red, (size_t)Dim<kMaxN>::NACC * k))) return rc;
   c->Kcap = K;
   return NECG_OK;
 }
 
 int ensure_part(necg_ctx *c, size_t count) {
   if (count <= c->part_cap) return NECG_OK;
-  if (c->pins) return NECG_ERR_ARG;
+  if (c->pins && !c->arena) return NECG_ERR_ARG;
   c->part_cap = 0;
-  int rc = dev_alloc(&c->part, count);
+  int rc = ws_alloc(c, &c->part, count);
   if (rc) return rc;
   c->part_cap = count;
   return NECG_OK;
 }
 int ensure_hpart(necg_ctx *c, size_t count) {
   if (count <= c->hpart_cap) return NECG_OK;
-  if (c->pins) return NECG_ERR_ARG;
+  if (c->pins && !c->arena) return NECG_ERR_ARG;
   c->hpart_cap = 0;
-  int rc = dev_alloc(&c->hpart, count);
+  int rc = ws_alloc(c, &c->hpart, count);
   if (rc) return rc;
-  rc = dev_alloc(&c->spart, count);
+  rc = ws_alloc(c, &c->spart, count);
   if (rc) return rc;
   c->hpart_cap = count;
   return NECG_OK;
@@ -446,17 +462,13 @@ int run_prep(necg_ctx *c, int K, const double *x, int with_c, hipStream_t st) {
 
 // Runs the pair kernel in MODE and the finishing kernels.  f, H, S, g may be
 // null where the mode does not produce them.
+// Launch shape and workspace of the per-ordering pair pass in MODE.
 template <int N, int MODE>
-int run_pairs(necg_ctx *c, int K, hipStream_t st, int &ntiles, int &ts_out) {
-  int kt, ts, w;
+int reserve_pairs(necg_ctx *c, int K, int &kt, int &ts, int &w, int &tps) {
   choose_tiling(c, K, kt, ts, w);
-  const int tps = (c->nst + ts - 1) / ts;
+  tps = (c->nst + ts - 1) / ts;
   ts = (c->nst + tps - 1) / tps;
-  const int kts = (K + kt - 1) / kt;
-  const int threads = 64 * w;
-  dim3 grid((K + 63) / 64, kts, ts);
-  ntiles = kts * ts;
-  ts_out = ts;
+  const int ntiles = ((K + kt - 1) / kt) * ts;
   int rc;
   if (mode_has_grad(MODE)) {
     if ((rc = ensure_part(c, (size_t)ntiles * Dim<N>::NACC * K))) return rc;
@@ -464,6 +476,18 @@ int run_pairs(necg_ctx *c, int K, hipStream_t st, int &ntiles, int &ts_out) {
   if (MODE != MODE_COMBINED) {
     if ((rc = ensure_hpart(c, (size_t)ts * K * K))) return rc;
   }
+  return NECG_OK;
+}
+
+template <int N, int MODE>
+int run_pairs(necg_ctx *c, int K, hipStream_t st, int &ntiles, int &ts_out) {
+  int kt, ts, w, tps, rc;
+  if ((rc = reserve_pairs<N, MODE>(c, K, kt, ts, w, tps))) return rc;
+  const int kts = (K + kt - 1) / kt;
+  const int threads = 64 * w;
+  dim3 grid((K + 63) / 64, kts, ts);
+  ntiles = kts * ts;
+  ts_out = ts;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (c->timing) {
     if ((size_t)(2 * c->nev + 2) > c->ev.size()) {
@@ -585,15 +609,35 @@ void choose_tiling2(necg_ctx *c, int K, int &kt, int &ts) {
 // The tile list for (K, kt) on the device (tri_tile_list: tri_tiles order,
 // the last round's tiles split when that leaves CUs idle), then the
 // per-bra-tile extra-slot ranges.
-int ensure_tiles(necg_ctx *c, int K, int kt) {
+int ensure_tiles(necg_ctx *c, int K, int kt, hipStream_t st) {
   if (c->tiles_K == K && c->tiles_kt == kt) return NECG_OK;
-  if (c->pins) return NECG_ERR_ARG;   // a captured graph reads the current list
+  if (c->pins && !c->arena) return NECG_ERR_ARG;   // a captured graph reads the current list
   if ((K + 63) / 64 > 0xffff) return NECG_ERR_UNSUPPORTED;   // bra tile index has 16 bits
   std::vector<int> full, xr;
   const int nsplit = tri_tile_list(K, kt, std::max(1, c->ncu), full, xr);
   const size_t ntile_words = full.size();
   full.insert(full.end(), xr.begin(), xr.end());
   c->tiles_K = 0;   // no valid list until the copy below completes
+  if (c->arena) {
+    // the list goes into the arena on the call's stream, after the kernels of
+    // earlier calls on that stream that read the arena; its host copy is kept
+    // on the context so that the pending copy's source stays valid
+    int *d = nullptr;
+    int rc = ws_alloc(c, (void **)&d, full.size() * sizeof(int));
+    if (rc) return rc;
+    const long long key = ((long long)K << 32) ^ ((long long)kt << 16) ^ (long long)c->ncu;
+    std::vector<int> &keep = c->arena_lists[key];   // the same list for the same key
+    if (keep.empty()) keep = full;
+    HIP_TRY(hipMemcpyAsync(d, keep.data(), keep.size() * sizeof(int), hipMemcpyHostToDevice, st));
+    c->d_tiles = d;
+    c->tiles_cap = keep.size();
+    c->tiles_K = K;
+    c->tiles_kt = kt;
+    c->ntri = (int)(ntile_words / 2);
+    c->nsplit = nsplit;
+    c->d_xr = c->d_tiles + ntile_words;
+    return NECG_OK;
+  }
   if (full.size() > c->tiles_cap) {
     if (c->d_tiles) (void)hipFree(c->d_tiles);
     c->d_tiles = nullptr;
@@ -612,9 +656,9 @@ int ensure_tiles(necg_ctx *c, int K, int kt) {
 
 int ensure_kpart(necg_ctx *c, size_t count) {
   if (count <= c->kpart_cap) return NECG_OK;
-  if (c->pins) return NECG_ERR_ARG;
+  if (c->pins && !c->arena) return NECG_ERR_ARG;
   c->kpart_cap = 0;
-  int rc = dev_alloc(&c->kpart, count);
+  int rc = ws_alloc(c, &c->kpart, count);
   if (rc) return rc;
   c->kpart_cap = count;
   return NECG_OK;
@@ -646,19 +690,28 @@ int timing_begin(necg_ctx *c, hipStream_t st, hipEvent_t &e1) {
 }
 
 // Two-sided gradient pair pass (MODE_SPLIT partials) over the lower triangle.
+// Launch shape, tile list and workspace of the two-sided pair pass.
 template <int N>
-int run_pairs2(necg_ctx *c, int K, hipStream_t st, int &kt, int &ts) {
+int reserve_pairs2(necg_ctx *c, int K, hipStream_t st, int &kt, int &ts, int &tps) {
   using D = Dim<N>;
   choose_tiling2(c, K, kt, ts);
-  const int tps = (c->nst + ts - 1) / ts;
+  tps = (c->nst + ts - 1) / ts;
   ts = (c->nst + tps - 1) / tps;
   const int nkt = (K + kt - 1) / kt, nbt = (K + 63) / 64;
   int rc;
-  if ((rc = ensure_tiles(c, K, kt))) return rc;
+  if ((rc = ensure_tiles(c, K, kt, st))) return rc;
   if ((rc = ensure_part(c, (size_t)(nkt + c->nsplit) * ts * D::NACC * K))) return rc;
-  double *xpart = c->part + (size_t)nkt * ts * D::NACC * K;
   if ((rc = ensure_kpart(c, (size_t)nbt * ts * K * D::NACC))) return rc;
-  if ((rc = ensure_hpart(c, (size_t)ts * K * K))) return rc;
+  return ensure_hpart(c, (size_t)ts * K * K);
+}
+
+template <int N>
+int run_pairs2(necg_ctx *c, int K, hipStream_t st, int &kt, int &ts) {
+  using D = Dim<N>;
+  int tps, rc;
+  if ((rc = reserve_pairs2<N>(c, K, st, kt, ts, tps))) return rc;
+  const int nkt = (K + kt - 1) / kt;
+  double *xpart = c->part + (size_t)nkt * ts * D::NACC * K;
   hipEvent_t e1;
   if ((rc = timing_begin(c, st, e1))) return rc;
   if (c->sc.mrank1)
@@ -684,7 +737,7 @@ int run_tri(necg_ctx *c, int K, hipStream_t st, int &ts) {
   const int tps = (c->nst + ts - 1) / ts;
   ts = (c->nst + tps - 1) / tps;
   int rc;
-  if ((rc = ensure_tiles(c, K, kt))) return rc;
+  if ((rc = ensure_tiles(c, K, kt, st))) return rc;
   if ((rc = ensure_hpart(c, (size_t)ts * K * K))) return rc;
   hipEvent_t e1;
   if ((rc = timing_begin(c, st, e1))) return rc;
@@ -738,6 +791,17 @@ int grad_pass(necg_ctx *c, int K, hipStream_t st, int &ntiles, int &ts, int &kt)
   kt = c->last_kt;
   c->last_two = 0;
   return NECG_OK;
+}
+
+// Everything the gradient pass of K needs, reserved before any launch (the
+// allocation-free entry point checks the whole workspace up front).
+template <int N>
+int reserve_grad_pass(necg_ctx *c, int K, hipStream_t st) {
+  int kt, ts, w, tps;
+  if constexpr (N <= 5) {
+    if (use_two(c)) return reserve_pairs2<N>(c, K, st, kt, ts, tps);
+  }
+  return reserve_pairs<N, MODE_SPLIT>(c, K, kt, ts, w, tps);
 }
 
 template <int N>
@@ -862,6 +926,87 @@ struct DeviceGuard {
 
 }  // namespace
 
+// ================================================ allocation-free scratch ===
+
+// Device memory of the allocation-free entry point (necg_outprod_noalloc):
+// every workspace buffer of an evaluation is carved out of this array, so the
+// call makes no device allocation.  Each GPU the library runs on holds its
+// own copy until the process ends.  In host code `scratch` is a host-side
+// stand-in, not a device address: only the address that hipGetSymbolAddress
+// returns for the current device (scratch_base) is handed to kernels and
+// copies.
+static constexpr size_t kScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+char *g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on `device` (current), looked up on the
+// first call per device and kept.
+int scratch_base(int device, char **out) {
+  if (device < 0 || device >= kMaxDevices) return NECG_ERR_DEVICE;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) return NECG_ERR_DEVICE;
+    g_scratch_ptr[device] = (char *)p;
+  }
+  *out = g_scratch_ptr[device];
+  return NECG_OK;
+}
+
+// Points the context's workspace at the arena for one call and puts its own
+// workspace, tile list and resident-state bookkeeping back afterwards: the
+// context's buffers and the state a pool or a tape reads are untouched.
+struct ArenaScope {
+  necg_ctx *c;
+  int Kcap, tiles_K, tiles_kt, ntri, nsplit, last_K, last_kt, last_ts, last_w, last_two;
+  double *bra, *ket, *H, *S, *HC, *SC, *scal, *part, *hpart, *spart, *hcp, *scp, *red, *kpart;
+  size_t part_cap, hpart_cap, kpart_cap, tiles_cap;
+  int *d_tiles, *d_xr;
+  long long gen, full_gen;
+  bool timing;
+  ArenaScope(necg_ctx *cc, char *base, size_t cap) : c(cc) {
+    Kcap = c->Kcap; tiles_K = c->tiles_K; tiles_kt = c->tiles_kt; ntri = c->ntri;
+    nsplit = c->nsplit; last_K = c->last_K; last_kt = c->last_kt; last_ts = c->last_ts;
+    last_w = c->last_w; last_two = c->last_two;
+    bra = c->bra; ket = c->ket; H = c->H; S = c->S; HC = c->HC; SC = c->SC; scal = c->scal;
+    part = c->part; hpart = c->hpart; spart = c->spart; hcp = c->hcp; scp = c->scp;
+    red = c->red; kpart = c->kpart;
+    part_cap = c->part_cap; hpart_cap = c->hpart_cap; kpart_cap = c->kpart_cap;
+    tiles_cap = c->tiles_cap; d_tiles = c->d_tiles; d_xr = c->d_xr;
+    gen = c->gen; full_gen = c->full_gen; timing = c->timing;
+    c->Kcap = 0; c->tiles_K = 0; c->tiles_kt = 0;
+    c->bra = c->ket = c->H = c->S = c->HC = c->SC = c->scal = nullptr;
+    c->part = c->hpart = c->spart = c->hcp = c->scp = c->red = c->kpart = nullptr;
+    c->part_cap = c->hpart_cap = c->kpart_cap = c->tiles_cap = 0;
+    c->d_tiles = c->d_xr = nullptr;
+    c->timing = false;
+    c->arena = base;
+    c->arena_cap = cap;
+    c->arena_used = 0;
+  }
+  ~ArenaScope() {
+    c->Kcap = Kcap; c->tiles_K = tiles_K; c->tiles_kt = tiles_kt; c->ntri = ntri;
+    c->nsplit = nsplit; c->last_K = last_K; c->last_kt = last_kt; c->last_ts = last_ts;
+    c->last_w = last_w; c->last_two = last_two;
+    c->bra = bra; c->ket = ket; c->H = H; c->S = S; c->HC = HC; c->SC = SC; c->scal = scal;
+    c->part = part; c->hpart = hpart; c->spart = spart; c->hcp = hcp; c->scp = scp;
+    c->red = red; c->kpart = kpart;
+    c->part_cap = part_cap; c->hpart_cap = hpart_cap; c->kpart_cap = kpart_cap;
+    c->tiles_cap = tiles_cap; c->d_tiles = d_tiles; c->d_xr = d_xr;
+    c->gen = gen; c->full_gen = full_gen; c->timing = timing;
+    c->arena = nullptr;
+    c->arena_cap = c->arena_used = 0;
+  }
+};
+
+}  // namespace
+
 // ================================================================= C ABI ===
 
 extern "C" {
@@ -877,6 +1022,8 @@ const char *necg_strerror(int code) {
     case NECG_ERR_CSC: return "C^T S C <= 0";
     case NECG_ERR_NONFINITE: return "NaN or Inf in input or result";
     case NECG_ERR_STALE: return "no forward tape for this backward (another evaluation ran since)";
+    case NECG_ERR_SCRATCH:
+      return "workspace larger than the allocation-free entry point's 64 MiB scratch";
     default: return "unknown status";
   }
 }
@@ -974,6 +1121,26 @@ int necg_outprod(necg_ctx *c, int32_t K, const double *x, double *f, double *g, 
   auto run = [&]() -> int { DISPATCH_N(c->n, (do_outprod<NN>(c, K, x, f, g, (hipStream_t)stream))); };
   if ((rc = run()) == NECG_OK) c->full_gen = c->gen;
   return rc;
+}
+
+int necg_outprod_noalloc(necg_ctx *c, int32_t K, const double *x, double *f, double *g,
+                         void *stream) {
+  if (!c || K < 1 || !x || !g) return NECG_ERR_ARG;
+  NECG_REQUIRE_DEVICE((const void *)x, (const void *)f, (const void *)g);
+  std::lock_guard<std::mutex> lk(c->mu);
+  DeviceGuard dg(c->device);
+  char *base = nullptr;
+  int rc = scratch_base(c->device, &base);
+  if (rc) return rc;
+  c->tape_K = 0;
+  ArenaScope arena(c, base, kScratchBytes);
+  hipStream_t st = (hipStream_t)stream;
+  // the whole workspace is carved before the first launch: a call that does
+  // not fit is refused with nothing enqueued except, possibly, the tile list
+  if ((rc = ensure_K(c, K))) return rc;
+  auto reserve = [&]() -> int { DISPATCH_N(c->n, (reserve_grad_pass<NN>(c, K, st))); };
+  if ((rc = reserve())) return rc;
+  DISPATCH_N(c->n, (do_outprod<NN>(c, K, x, f, g, st)));
 }
 
 int necg_energy(necg_ctx *c, int32_t K, const double *x, double *f, double *H, double *S,

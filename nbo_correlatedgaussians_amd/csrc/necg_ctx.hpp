@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <map>
 #include <mutex>
 #include <vector>
 
@@ -71,6 +72,13 @@ struct necg_ctx {
   // workspace buffer or rewrite the tile list, since a captured HIP graph
   // holds their addresses and contents
   int pins = 0;
+  // allocation-free mode (necg_outprod_noalloc): while arena is set, the
+  // workspace is carved from [arena, arena + arena_cap) instead of allocated,
+  // and tile lists are uploaded from host copies kept here per (K, ket tile,
+  // CU count) for the life of the context
+  char *arena = nullptr;
+  size_t arena_cap = 0, arena_used = 0;
+  std::map<long long, std::vector<int>> arena_lists;
   // pair-kernel timing (necg_timing)
   bool timing = false;
   std::vector<hipEvent_t> ev;  // start/stop pairs, reused
