@@ -23,8 +23,8 @@
  * Gaussian elimination with partial pivoting, the same LU route as
  * numpy.linalg.inv / det (python/matel.py:93, :99).
  *
- * Pinning (what checks this file besides the reference's own numbers): the
- * golden vectors written from the reference, central finite differences of
+ * Pinning: the golden vectors written from the reference (outprod, matel,
+ * gradel, rrms and the dipole loop), central finite differences of
  * matel against gradel, the hydrogen-atom closed form, the Gaussian product
  * theorem for overlaps, and F0(0) = 1 / erf form of the Boys function; rrms
  * and the dipole by the Gaussian product theorem (one particle: overlap and

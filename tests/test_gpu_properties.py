@@ -42,6 +42,21 @@ def test_dipole_oldquit_vs_oracle(oracle_lib):
     _close(Engine(s, "cuda:0").dipole(d["x"]), oracle_lib.Oracle(s).dipole(d["x"], nthreads=16))
 
 
+@pytest.mark.parametrize("case", ["lih_k12_s0", "h2_k8_s4"])
+def test_dipole_vs_reference_golden(case):
+    """The driver's dipole loop as the reference computes it
+    (python/necg.py:200-232 over python/rrms.py), both forms of the pass."""
+    from nbo_correlatedgaussians_amd.engine import Engine
+    d = golden(f"dipole_{case}")
+    s = system_by_name(d["system"])
+    eng = Engine(s, "cuda:0")
+    ref = (float(d["cSc"]), d["centres"], d["dipole"])
+    for mode in (0, 2):                # symmetry-reduced (where it applies) and full double sum
+        eng.set_dipole_mode(mode)
+        print(f"dipole {case} mode {mode} vs the reference", flush=True)
+        _close(eng.dipole(d["x"]), ref)
+
+
 def test_dipole_hydrogen_closed_form():
     """One particle, two Gaussians: the C-weighted mean of the centres A, B
     and P = (aA + bB)/(a + b) with the product-theorem overlap."""

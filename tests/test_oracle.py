@@ -193,6 +193,31 @@ def _identity_only(s):
                        sml=np.eye(n)[None], sms=np.eye(3 * n)[None], name=s.name + "-id")
 
 
+@pytest.mark.parametrize("case", ["lih_k12_s0", "h2_k8_s4"])
+def test_rrms_and_dipole_vs_reference_golden(O, case):
+    """The reference's own rrms (python/rrms.py:21-120) on sampled
+    (bra term, ket term, bra, ket, component) queries, most with a
+    non-identity bra term, and the driver's dipole loop over it
+    (python/necg.py:200-232): overlap and component to 1e-12 relative (the
+    component against max(|value|, 1e-3 |overlap|)), centres, C^T s C and
+    dipole to 1e-12."""
+    d = golden(f"dipole_{case}")
+    s = system_by_name(d["system"])
+    o = O(s)
+    x = d["x"]
+    K = x.size // (s.blk + 1)
+    iv = x[:K * s.blk]
+    assert np.sum(d["queries"][:, 0] != 1) >= len(d["queries"]) // 3   # non-identity bra terms
+    for (stfe, stf, i, j, n1), (ov, tk) in zip(d["queries"], d["rrms"]):
+        ov2, tk2 = o.rrms(iv, stfe - 1, stf - 1, i - 1, j - 1)
+        assert abs(ov2 - ov) <= 1e-12 * abs(ov), (stfe, stf, i, j)
+        assert abs(tk2[n1 - 1] - tk) <= 1e-12 * max(abs(tk), 1e-3 * abs(ov)), (stfe, stf, i, j, n1)
+    css, rbar, dip = o.dipole(x)
+    assert abs(css - float(d["cSc"])) <= 1e-12 * abs(float(d["cSc"]))
+    assert np.max(np.abs(rbar - d["centres"])) <= 1e-12 * max(1.0, np.max(np.abs(d["centres"])))
+    assert np.max(np.abs(dip - d["dipole"])) <= 1e-12 * max(1.0, np.max(np.abs(d["dipole"])))
+
+
 def test_rrms_gaussian_product_theorem(O):
     """One particle, two Gaussians: overlap (4ab/(a+b)^2)^(3/4) exp(-mu d^2)
     and centre P = (aA + bB)/(a+b); the dipole pass of two functions is the
