@@ -58,6 +58,47 @@ def cpu_baseline(sysd, K, x, rows, threads):
     }
 
 
+def host_cpu():
+    """The host's core count and model, and the CPU share a run may use."""
+    import subprocess
+    model = None
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    nproc = os.cpu_count() or 1
+    share = int(os.environ.get("OMP_NUM_THREADS") or nproc)
+    return {"nproc": nproc, "model": model, "share": max(1, min(share, nproc))}
+
+
+def reference_structure_baseline(sysd, K, x, E, norm, share):
+    """The NumPy restatement of the reference's per-pair loops
+    (oracle/kron_ref.py: one matel/gradel call per (term, bra, ket) in the
+    reference's Kronecker operator form, the make_grad GRADA loop) on a
+    bounded sample of the same evaluation: one process (1 core,
+    single-threaded BLAS), then `share` processes with rows dealt
+    rank-cyclically as the Fortran code deals them; both extrapolated."""
+    from oracle.kron_ref import time_sample
+    out = {}
+    for procs in (1, share):
+        r = time_sample(sysd, x, K, E, norm, procs=procs, rows_per_proc=2, kets=48)
+        out[f"cores_{procs}"] = {
+            "value": (K * (K + 1) / 2) / r["est_seconds_per_eval"], "unit": "pairs/s",
+            "cores": procs, "kind": "port",
+            "sample": (f"oracle/kron_ref.py, {r['gradient_blocks']} ordered pairs x "
+                       f"{sysd.nst} terms of gradel + {r['element_pairs']} pairs of matel + "
+                       f"one GRADA row, {r['wall_s']:.1f} s wall on {procs} process(es), "
+                       f"extrapolated to one full K={K} evaluation "
+                       f"({r['est_seconds_per_eval']:.0f} s)"),
+            "est_seconds_per_eval": r["est_seconds_per_eval"],
+            "s_per_gradient_block": r["s_per_gradient_block"],
+            "s_per_element_pair": r["s_per_element_pair"]}
+    return out
+
+
 def batched_leg(reps=5):
     """64 independent systems (He/H2/LiH, K log-uniform in [32, 1024], seeds
     100..163) evaluated together (autotuned tiles, HIP-graph replay) against
@@ -192,8 +233,11 @@ def cpu_optimizer(opt):
     """The optimizer leg's runs with the oracle (one thread) as the objective.
     L-BFGS-B trajectories are compared iteration by iteration.  Truncated
     Newton forms Hessian-vector products from gradient differences over a
-    ~1e-8 step; on this input a 1e-13 relative change of the gradient moves
-    its first iteration's energy by O(1), so only E0 is comparable there."""
+    ~1e-8 step: on this input, seeded gradient noise of 1e-15 relative (a few
+    ulp) moves the oracle-driven run's energies by O(1) from its first or
+    second iteration on (tests/test_driver_cpu.py), so two runs whose
+    gradients differ at the 1e-13 level are not comparable beyond E0 and,
+    at best, the first iteration."""
     from nbo_correlatedgaussians_amd.driver import optimize
     from oracle.oracle import Oracle
     s, K, x0 = small_lih_input()
@@ -484,8 +528,15 @@ def main():
         out["served"] = served_leg(args.serve_requests, args.serve_rate)
         print(f"bench: served leg done", file=sys.stderr, flush=True)
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
+        host = host_cpu()
+        threads = host["share"]          # the box's CPU share for this GPU
         out["cpu_baseline"] = cpu_baseline(sysd, K, x, args.cpu_rows, threads)
+        out["cpu_baseline"]["host"] = host
+        print(f"bench: oracle baseline done", file=sys.stderr, flush=True)
+        Eh = float(E.item())          # norm does not change the work timed
+        out["cpu_baseline"]["numpy_reference_structure"] = reference_structure_baseline(
+            sysd, K, x, Eh, 1.0, threads)
+        print(f"bench: reference-structure baseline done", file=sys.stderr, flush=True)
         if "optimizer" in out:
             out["cpu_baseline"]["optimizer"] = cpu_optimizer(out["optimizer"])
     print(json.dumps(out), flush=True)

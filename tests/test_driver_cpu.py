@@ -33,3 +33,27 @@ def test_optimize_with_oracle_writes_punch(oracle_lib, tmp_path, method):
     xp = read_punch(p)
     assert xp.size == x0.size
     assert abs(o.outprod(xp)[0] - r.energies[-1]) <= 1e-12 * abs(r.energies[-1])
+
+
+def test_truncated_newton_amplifies_ulp_noise(oracle_lib):
+    """Truncated Newton on the generator's LiH K = 12 input (seed 0), the
+    optimiser leg's workload: the oracle-driven run, and the same run with
+    its gradient multiplied by (1 + 1e-15 n), n seeded N(0, 1).  The first
+    iteration agrees; from the second on the runs part by O(1e-2) and more
+    (Hessian-vector products from gradient differences, source/tn.f:1301-1328,
+    amplify the noise).  A GPU-driven run, whose gradient differs from the
+    oracle's at the 1e-13 level, is therefore compared on E0 only."""
+    _, s, iv, C = necg.generated_lih_input(12, 0)
+    x0 = necg.pack_x(iv, C)
+    o = oracle_lib.Oracle(s)
+    clean = optimize(lambda x: o.outprod(x, 1), x0, method="TN", maxiter=4, siv=12 * s.blk, M=12)
+    rng = np.random.default_rng(2)
+
+    def noisy(x):
+        E, g = o.outprod(x, 1)
+        return E, g * (1.0 + 1e-15 * rng.standard_normal(g.size))
+
+    pert = optimize(noisy, x0, method="TN", maxiter=4, siv=12 * s.blk, M=12)
+    a, b = clean.energies, pert.energies
+    assert abs(a[0] - b[0]) <= 1e-10 * abs(a[0]), (a[0], b[0])
+    assert max(abs(u - v) for u, v in zip(a[1:], b[1:])) > 1e-2, (a, b)

@@ -1,0 +1,106 @@
+"""Host code under AddressSanitizer and UndefinedBehaviorSanitizer (CPU).
+
+* The oracle's entry points (outprod with both summations, matel, gradel,
+  rrms, the dipole pass) built as one program with -fsanitize=address,undefined
+  (oracle/san_check.c) on LiH, He and the n = 6 test system: every call runs
+  clean and gives the numbers of the ordinary build.
+* The host tiling checker (tests/native/tri_tiles_check.cpp), whose tile
+  lists the kernels take, built with the sanitizers on its host code."""
+
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+import nbo_correlatedgaussians_amd as necg
+from conftest import system_by_name
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE = os.path.join(HERE, "..", "oracle")
+
+
+@pytest.fixture(scope="module")
+def san_exe():
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc not available")
+    subprocess.run(["make", "-s", "-C", ORACLE, "san_check"], check=True,
+                   stdout=subprocess.DEVNULL)
+    return os.path.join(ORACLE, "san_check")
+
+
+@pytest.mark.parametrize("name,K", [("LiH", 3), ("He", 5), ("C", 2)])
+def test_oracle_under_sanitizers(oracle_lib, san_exe, tmp_path, name, K):
+    s = system_by_name(name)
+    iv, C = necg.generate_parameters(s, K, 11)
+    x = necg.pack_x(iv, C)
+    n, nst, blk = s.n, s.nst, s.blk
+    inp, out = tmp_path / "in.bin", tmp_path / "out.bin"
+    with open(inp, "wb") as fh:
+        np.array([n, nst, K], dtype="<i4").tofile(fh)
+        for a in (np.array([s.efs]), s.mass, s.ncv, s.scvv, s.sml, s.sms, x):
+            np.ascontiguousarray(a, dtype="<f8").tofile(fh)
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=1:abort_on_error=0",
+               UBSAN_OPTIONS="print_stacktrace=1", OMP_NUM_THREADS="2")
+    r = subprocess.run([san_exe, str(inp), str(out)], capture_output=True, text=True,
+                       timeout=600, env=env)
+    assert r.returncode == 0, r.stderr[-3000:]
+    assert "runtime error" not in r.stderr and "ERROR: AddressSanitizer" not in r.stderr
+    v = np.fromfile(out, dtype="<f8")
+    o = oracle_lib.Oracle(s)
+    E, g, H, S = o.outprod(x, return_hs=True)
+    (_, _), (El, gl) = o.outprod_ld(x)
+    nx = x.size
+    pos = 0
+
+    def take(m):
+        nonlocal pos
+        a = v[pos:pos + m]
+        pos += m
+        return a
+
+    np.testing.assert_allclose(take(1)[0], E, rtol=1e-14)
+    np.testing.assert_allclose(take(nx), g, rtol=1e-12, atol=1e-14 * np.max(np.abs(g)))
+    np.testing.assert_allclose(take(1)[0], El, rtol=1e-14)
+    np.testing.assert_allclose(take(nx), gl, rtol=1e-12, atol=1e-14 * np.max(np.abs(gl)))
+    np.testing.assert_allclose(take(K * K), H.ravel(), rtol=1e-13, atol=1e-14 * np.abs(H).max())
+    np.testing.assert_allclose(take(K * K), S.ravel(), rtol=1e-13, atol=1e-14 * np.abs(S).max())
+    for p in range(nst):
+        for i in range(K):
+            for j in range(K):
+                np.testing.assert_allclose(take(4), o.matel(iv, p, i, j), rtol=1e-13, atol=1e-300)
+                a, b = o.gradel(iv, p, i, j)
+                np.testing.assert_allclose(take(blk), a, rtol=1e-12,
+                                           atol=1e-14 * max(np.abs(a).max(), 1e-300))
+                np.testing.assert_allclose(take(blk), b, rtol=1e-12,
+                                           atol=1e-14 * max(np.abs(b).max(), 1e-300))
+    for p in range(nst):
+        for q in range(nst):
+            for i in range(K):
+                for j in range(K):
+                    ov, tk = o.rrms(iv, p, q, i, j)
+                    np.testing.assert_allclose(take(1)[0], ov, rtol=1e-13)
+                    np.testing.assert_allclose(take(3 * n), tk, rtol=1e-12,
+                                               atol=1e-14 * max(abs(ov), 1e-300))
+    css, rbar, dip = o.dipole(x)
+    np.testing.assert_allclose(take(1)[0], css, rtol=1e-13)
+    np.testing.assert_allclose(take(3 * n), rbar.ravel(), rtol=1e-12, atol=1e-14)
+    np.testing.assert_allclose(take(3), dip, rtol=1e-12, atol=1e-14)
+    assert pos == v.size
+
+
+def test_tiling_checker_under_sanitizers(tmp_path):
+    hipcc = next((c for c in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc",
+                              shutil.which("hipcc")) if c and os.path.exists(c)), None)
+    if hipcc is None:
+        pytest.skip("hipcc not available")
+    exe = str(tmp_path / "tri_tiles_check_san")
+    src = os.path.join(HERE, "native", "tri_tiles_check.cpp")
+    flags = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
+             "-Xarch_host", "-fno-sanitize-recover=all"]
+    subprocess.run([hipcc, "-O1", "-g", "-std=c++17", *flags, "-o", exe, src], check=True,
+                   stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0 and r.stdout.strip().endswith("ok"), (r.stdout[-1500:],
+                                                                   r.stderr[-1500:])
