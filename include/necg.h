@@ -198,6 +198,14 @@ int necg_set_tiling(necg_ctx *ctx, int32_t ket_tile, int32_t term_split, int32_t
  * into halves for the last round.  No counterpart in the reference. */
 int necg_set_launch_cus(necg_ctx *ctx, int32_t ncu);
 
+/* Folded diagonal blocks in the lower-triangle passes (fold = 1, the
+ * default, or 0): when the ket tile divides 32, each full 64-function bra
+ * tile's diagonal block is evaluated in 32 sweeps of two kets each (every
+ * lane busy) instead of 64 half-idle ones, and a small diagonal pass adds
+ * the diagonals of the block's upper half.  Results agree to rounding.  No
+ * counterpart in the reference. */
+int necg_set_fold(necg_ctx *ctx, int32_t fold);
+
 /* Workspace pin for HIP-graph capture: pin = 1 adds a pin, pin = 0 removes
  * one.  A captured evaluation holds the addresses of the context's workspace
  * and reads its lower-triangle tile list, which belong to the K of the

@@ -30,7 +30,7 @@ EXPORTS = ("necg_create", "necg_destroy", "necg_outprod", "necg_outprod_noalloc"
            "necg_energy_fwd", "necg_energy_bwd",
            "necg_energy_vjp", "necg_matrices", "necg_matel", "necg_gradel",
            "necg_dhds", "necg_status", "necg_set_tiling", "necg_set_launch_cus", "necg_get_tiling",
-           "necg_pin_workspace", "necg_batch_pin",
+           "necg_pin_workspace", "necg_batch_pin", "necg_set_fold",
            "necg_timing", "necg_strerror", "necg_batch_create", "necg_batch_destroy",
            "necg_batch_layout", "necg_batch_outprod", "necg_batch_matrices",
            "necg_batch_set_tiling", "necg_batch_status", "necg_batch_outprod_masked",
@@ -113,6 +113,7 @@ def load() -> ct.CDLL:
         "necg_set_launch_cus": [P, I],
         "necg_get_tiling": [P, ct.POINTER(I), ct.POINTER(I), ct.POINTER(I)],
         "necg_pin_workspace": [P, I],
+        "necg_set_fold": [P, I],
         "necg_batch_pin": [P, I],
         "necg_timing": [P, I, ct.POINTER(D), ct.POINTER(ct.c_int64)],
         "necg_batch_create": [P, I, P, ct.c_int, ct.POINTER(P)],

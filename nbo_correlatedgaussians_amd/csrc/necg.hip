@@ -116,10 +116,17 @@ __global__ void __launch_bounds__(64 * pair_waves(N), NECG_MIN_WAVES) pair2_kern
   const int p0 = z * tps;
   double *bp = t.e < 0 ? bpart + (size_t)(t.by * ts + z) * D::NACC * K
                        : xpart + (size_t)(t.e * ts + z) * D::NACC * K;
-  pair2_tile<N, R1>(bra, ket, K, nst, t.k0, t.l0, t.l1, t.lup, p0, min(nst, p0 + tps), sc,
-                    tperm, bp, kpart + (size_t)(t.bx * ts + z) * K * D::NACC,
-                    hpart + (size_t)z * K * K, spart + (size_t)z * K * K, flag, acc_s, kacc_s,
-                    bra_s, kst_s);
+  double *kp = kpart + (size_t)(t.bx * ts + z) * K * D::NACC;
+  double *hp = hpart + (size_t)z * K * K, *sp = spart + (size_t)z * K * K;
+  if (t.fold == 1)   // uniform over the workgroup
+    pair2_fold_tile<N, R1>(bra, ket, K, nst, t.k0, t.l0, t.l1, p0, min(nst, p0 + tps), sc,
+                           tperm, bp, kp, hp, sp, flag, acc_s, kacc_s, bra_s, kst_s);
+  else if (t.fold == 2)
+    pair2_diag_tile<N, R1>(bra, ket, K, nst, t.k0, p0, min(nst, p0 + tps), sc, tperm, bp, hp,
+                           sp, flag, acc_s, kacc_s, bra_s, kst_s);
+  else
+    pair2_tile<N, R1>(bra, ket, K, nst, t.k0, t.l0, t.l1, t.lup, p0, min(nst, p0 + tps), sc,
+                      tperm, bp, kp, hp, sp, flag, acc_s, kacc_s, bra_s, kst_s);
 }
 
 // Element-only pass (MODE_ELEMENTS / MODE_TV) over the same lower-triangle
@@ -135,9 +142,17 @@ __global__ void __launch_bounds__(64 * pair_waves(N), NECG_MIN_WAVES) pair_tri_k
   const TriTile t = tri_tile(tiles, blockIdx.x, K, kt);
   const int z = blockIdx.y;
   const int p0 = z * tps;
-  pair_tile<N, MODE, R1, true>(bra, ket, K, nst, t.k0, t.l0, t.l1, p0, min(nst, p0 + tps), sc,
-                               0.0, nullptr, hpart + (size_t)z * K * K,
-                               spart + (size_t)z * K * K, flag, acc_s, t.lup, kst_s);
+  // a fold tile of sweeps [l0, l1) covers kets k0 + [l0, l1) and k0 + 32 + [l0, l1) of the
+  // diagonal block: here, as two ordinary runs of kets (bra >= ket), which
+  // include the diagonals; a diagonal tile has nothing left to do
+  if (t.fold == 2) return;
+  const int runs = t.fold ? 2 : 1;
+  const int base = t.fold ? t.k0 : 0;
+  for (int r = 0; r < runs; ++r)
+    pair_tile<N, MODE, R1, true>(bra, ket, K, nst, t.k0, base + 32 * r + t.l0,
+                                 base + 32 * r + t.l1, p0, min(nst, p0 + tps), sc, 0.0, nullptr,
+                                 hpart + (size_t)z * K * K, spart + (size_t)z * K * K, flag,
+                                 acc_s, t.lup, kst_s);
 }
 
 // Two-sided partial sums per (function, slot).  Grid (ceil(K/64), nv),
@@ -147,10 +162,10 @@ __global__ void __launch_bounds__(256) tile_reduce2(const double *__restrict__ b
                                                     int kt, int ts, int nacc,
                                                     double *__restrict__ red,
                                                     const double *__restrict__ xpart,
-                                                    const int *__restrict__ xr) {
+                                                    const int *__restrict__ xr, int fold) {
   __shared__ double sh[256];
   tile_reduce2_block(bpart, kpart, K, kt, ts, nacc, blockIdx.y, blockIdx.x * 64, red, sh, xpart,
-                     xr);
+                     xr, fold);
 }
 
 // Full symmetric H and S and the per-chunk (64 columns) partials of H C and
@@ -552,56 +567,78 @@ int finish_grad(necg_ctx *c, int K, int ntiles, const double *gbar, double *g, h
 // ----------------------------------------------------- lower triangle ----
 
 // Ket tile and term groups of the lower-triangle passes.  One workgroup per
-// CU at a time, its waves taking alternate kets, so a launch takes about
-// (rounds of ncu workgroups) x (kets per wave + a per-tile overhead of 0.07
-// ket) ket-term sweeps, a last round of split tiles (tri_split_count) at
-// half the kets; the ket tile minimising that is chosen, then term groups
-// while there are fewer than two workgroups per CU.  The overhead is fitted
-// on LiH K = 2048: ket tile 8 beats 4 (33 rounds of 1 ket against 17 of 2:
-// more than 0.06), and 8 beats 16 by 5 % unsplit (17 rounds of 2 kets against
-// 9 of 4) and by 2.3 % split (16 + a half round against 8 + a half).
+// CU at a time, its waves taking alternate kets; for each candidate ket tile
+// the term groups ts are doubled while there are fewer than two workgroups
+// per CU, and the launch is priced in rounds of ncu workgroups, each
+// workgroup (kets per wave) x (terms per group / nst) ket sweeps plus a
+// per-workgroup overhead of 0.07 sweep; a last partial round costs a whole
+// workgroup, or half of one when it is split (tri_split_count, ts = 1).  The
+// overhead is fitted on LiH K = 2048: ket tile 8 beats 4 (33 rounds of 1 ket
+// against 17 of 2: more than 0.06), and 8 beats 16 by 5 % unsplit (17 rounds
+// of 2 kets against 9 of 4) and by 2.3 % split (16 + a half round against 8
+// + a half).  With folded diagonal blocks (tri_fold) the fold tiles, 1.12
+// ordinary ones, start first, so one of them lies on the critical path; the
+// diagonal tiles (about a third of a workgroup each) take no slot of a round.
+double tiling2_cost(int K, int q, int fold, int ncu, int nst, int w, int req_ts, int &ts_out) {
+  long long n, nfull;
+  tri_counts(K, q, fold, n, nfull);
+  const long long nd = tri_fold_tiles(K, q, fold), nf = nd * (32 / q), nr = n - nd;
+  int ts = 1;
+  if (req_ts > 0) {
+    ts = std::min(req_ts, nst);
+  } else {
+    while (n * ts < 2LL * ncu && ts < nst) ts *= 2;
+    ts = std::min(ts, nst);
+  }
+  ts_out = ts;
+  const int tps = (nst + ts - 1) / ts;
+  const double frac = (double)tps / nst;
+  const double kw = (double)((q + w - 1) / w) * frac;
+  const double tt = kw + 0.07;
+  const long long wg = nr * ts;
+  double cost = (double)(wg / ncu) * tt;
+  if (wg % ncu) {
+    const bool split = ts == 1 && tri_split_count(nr, nfull, q, ncu);
+    cost += split ? (double)((q / 2 + w - 1) / w) * frac + 0.07 : tt;
+  }
+  if (nf > 0) cost += 0.12 * kw;
+  return cost + (double)(nd * ts) * (tt / 3.0) / ncu;
+}
+
 void choose_tiling2(necg_ctx *c, int K, int &kt, int &ts) {
   const int ncu = std::max(1, c->ncu);
   const int w = pair_waves(c->n);
   // the choice walks every candidate's tile list (about 0.3 ms of host time
   // at K = 2048): kept for the next evaluation with the same inputs
   if (c->tc_K == K && c->tc_req_kt == c->req_kt && c->tc_req_ts == c->req_ts &&
-      c->tc_ncu == ncu) {
+      c->tc_ncu == ncu && c->tc_fold == c->fold_mode) {
     kt = c->tc_kt;
     ts = c->tc_ts;
     return;
   }
   if (c->req_kt > 0) {
     kt = c->req_kt;
+    (void)tiling2_cost(K, kt, c->fold_mode, ncu, c->nst, w, c->req_ts, ts);
   } else {
     static const int cand[] = {2, 4, 8, 16, 32, 64};   // divisors of 64 (upper pairs)
     double best = -1.0;
     kt = 8;
+    ts = 1;
     for (int q : cand) {
-      long long n, nfull;
-      tri_counts(K, q, n, nfull);
-      const double sweep = (q + w - 1) / w + 0.07;
-      double cost = (double)(n / ncu) * sweep;
-      if (n % ncu)
-        cost += tri_split_count(n, nfull, q, ncu) ? (q / 2 + w - 1) / w + 0.07 : sweep;
+      int tq;
+      const double cost = tiling2_cost(K, q, c->fold_mode, ncu, c->nst, w, c->req_ts, tq);
       if (best < 0.0 || cost < best) {
         best = cost;
         kt = q;
+        ts = tq;
       }
     }
-  }
-  const long long n = tri_count(K, kt);
-  if (c->req_ts > 0) {
-    ts = std::min(c->req_ts, c->nst);
-  } else {
-    ts = 1;
-    while (n * ts < 2LL * ncu && ts < c->nst) ts *= 2;
-    ts = std::min(ts, c->nst);
   }
   c->tc_K = K;
   c->tc_req_kt = c->req_kt;
   c->tc_req_ts = c->req_ts;
   c->tc_ncu = ncu;
+  c->tc_fold = c->fold_mode;
   c->tc_kt = kt;
   c->tc_ts = ts;
 }
@@ -610,11 +647,12 @@ void choose_tiling2(necg_ctx *c, int K, int &kt, int &ts) {
 // the last round's tiles split when that leaves CUs idle), then the
 // per-bra-tile extra-slot ranges.
 int ensure_tiles(necg_ctx *c, int K, int kt, hipStream_t st) {
-  if (c->tiles_K == K && c->tiles_kt == kt) return NECG_OK;
+  const int fold = tri_fold(kt, c->fold_mode) ? 1 : 0;
+  if (c->tiles_K == K && c->tiles_kt == kt && c->tiles_fold == fold) return NECG_OK;
   if (c->pins && !c->arena) return NECG_ERR_ARG;   // a captured graph reads the current list
   if ((K + 63) / 64 > 0xffff) return NECG_ERR_UNSUPPORTED;   // bra tile index has 16 bits
   std::vector<int> full, xr;
-  const int nsplit = tri_tile_list(K, kt, std::max(1, c->ncu), full, xr);
+  const int nsplit = tri_tile_list(K, kt, std::max(1, c->ncu), fold, full, xr);
   const size_t ntile_words = full.size();
   full.insert(full.end(), xr.begin(), xr.end());
   c->tiles_K = 0;   // no valid list until the copy below completes
@@ -625,7 +663,8 @@ int ensure_tiles(necg_ctx *c, int K, int kt, hipStream_t st) {
     int *d = nullptr;
     int rc = ws_alloc(c, (void **)&d, full.size() * sizeof(int));
     if (rc) return rc;
-    const long long key = ((long long)K << 32) ^ ((long long)kt << 16) ^ (long long)c->ncu;
+    const long long key = ((long long)K << 32) ^ ((long long)kt << 16) ^
+                          ((long long)fold << 15) ^ (long long)c->ncu;
     std::vector<int> &keep = c->arena_lists[key];   // the same list for the same key
     if (keep.empty()) keep = full;
     HIP_TRY(hipMemcpyAsync(d, keep.data(), keep.size() * sizeof(int), hipMemcpyHostToDevice, st));
@@ -633,6 +672,7 @@ int ensure_tiles(necg_ctx *c, int K, int kt, hipStream_t st) {
     c->tiles_cap = keep.size();
     c->tiles_K = K;
     c->tiles_kt = kt;
+    c->tiles_fold = fold;
     c->ntri = (int)(ntile_words / 2);
     c->nsplit = nsplit;
     c->d_xr = c->d_tiles + ntile_words;
@@ -648,6 +688,7 @@ int ensure_tiles(necg_ctx *c, int K, int kt, hipStream_t st) {
   HIP_TRY(hipMemcpy(c->d_tiles, full.data(), full.size() * sizeof(int), hipMemcpyHostToDevice));
   c->tiles_K = K;
   c->tiles_kt = kt;
+  c->tiles_fold = fold;
   c->ntri = (int)(ntile_words / 2);
   c->nsplit = nsplit;
   c->d_xr = c->d_tiles + ntile_words;
@@ -763,7 +804,8 @@ int finish_grad2(necg_ctx *c, int K, int kt, int ts, const double *gbar, double 
   const int nkt = (K + kt - 1) / kt;
   hipLaunchKernelGGL(tile_reduce2, dim3((K + 63) / 64, D::NACC), dim3(256), 0, st, c->part,
                      c->kpart, K, kt, ts, D::NACC, c->red,
-                     c->part + (size_t)nkt * ts * D::NACC * K, c->nsplit ? c->d_xr : nullptr);
+                     c->part + (size_t)nkt * ts * D::NACC * K, c->nsplit ? c->d_xr : nullptr,
+                     c->tiles_fold);
   hipLaunchKernelGGL((grad_final<N, MODE_SPLIT>), dim3((K + 63) / 64), dim3(64), 0, st, c->red,
                      K, c->bra, c->scal, c->HC, c->SC, gbar, g);
   return hip_status(hipGetLastError());
@@ -964,14 +1006,16 @@ int scratch_base(int device, char **out) {
 // context's buffers and the state a pool or a tape reads are untouched.
 struct ArenaScope {
   necg_ctx *c;
-  int Kcap, tiles_K, tiles_kt, ntri, nsplit, last_K, last_kt, last_ts, last_w, last_two;
+  int Kcap, tiles_K, tiles_kt, tiles_fold, ntri, nsplit, last_K, last_kt, last_ts, last_w,
+      last_two;
   double *bra, *ket, *H, *S, *HC, *SC, *scal, *part, *hpart, *spart, *hcp, *scp, *red, *kpart;
   size_t part_cap, hpart_cap, kpart_cap, tiles_cap;
   int *d_tiles, *d_xr;
   long long gen, full_gen;
   bool timing;
   ArenaScope(necg_ctx *cc, char *base, size_t cap) : c(cc) {
-    Kcap = c->Kcap; tiles_K = c->tiles_K; tiles_kt = c->tiles_kt; ntri = c->ntri;
+    Kcap = c->Kcap; tiles_K = c->tiles_K; tiles_kt = c->tiles_kt; tiles_fold = c->tiles_fold;
+    ntri = c->ntri;
     nsplit = c->nsplit; last_K = c->last_K; last_kt = c->last_kt; last_ts = c->last_ts;
     last_w = c->last_w; last_two = c->last_two;
     bra = c->bra; ket = c->ket; H = c->H; S = c->S; HC = c->HC; SC = c->SC; scal = c->scal;
@@ -991,7 +1035,8 @@ struct ArenaScope {
     c->arena_used = 0;
   }
   ~ArenaScope() {
-    c->Kcap = Kcap; c->tiles_K = tiles_K; c->tiles_kt = tiles_kt; c->ntri = ntri;
+    c->Kcap = Kcap; c->tiles_K = tiles_K; c->tiles_kt = tiles_kt; c->tiles_fold = tiles_fold;
+    c->ntri = ntri;
     c->nsplit = nsplit; c->last_K = last_K; c->last_kt = last_kt; c->last_ts = last_ts;
     c->last_w = last_w; c->last_two = last_two;
     c->bra = bra; c->ket = ket; c->H = H; c->S = S; c->HC = HC; c->SC = SC; c->scal = scal;
@@ -1341,6 +1386,15 @@ int necg_set_launch_cus(necg_ctx *c, int32_t ncu) {
   if (c->pins) return NECG_ERR_ARG;
   c->ncu = ncu > 0 ? ncu : c->dev_ncu;
   c->tiles_K = 0;   // the tile list depends on it
+  return NECG_OK;
+}
+
+int necg_set_fold(necg_ctx *c, int32_t fold) {
+  if (!c || fold < 0 || fold > 1) return NECG_ERR_ARG;
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (c->pins) return NECG_ERR_ARG;
+  c->fold_mode = fold;
+  c->tape_K = 0;
   return NECG_OK;
 }
 

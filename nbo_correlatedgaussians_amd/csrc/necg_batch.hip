@@ -39,14 +39,15 @@ struct BatchSys {
   long long vec_off;                 // HC, SC: K
   long long hcp_off;                 // chunk partials: nchunk*K
   long long sml_off, sms_off, scvv_off;
-  // two-sided pair form (lower-triangle items): enabled, ket tile, the
-  // system's target-slot table and its ket partials ((K+63)/64 * ts * K * nacc)
-  int two, kt;
+  // two-sided pair form (lower-triangle items): enabled, ket tile, folded
+  // diagonal blocks (tri_fold), the system's target-slot table and its ket
+  // partials ((K+63)/64 * ts * K * nacc)
+  int two, kt, fold;
   long long tperm_off, kpart_off;
 };
 
-struct Item {  // one pair tile
-  int sys, k0, l0, l1, p0, p1, tile;
+struct Item {  // one pair tile (fold: sweeps [l0, l1) of a folded diagonal block)
+  int sys, k0, l0, l1, p0, p1, tile, fold;
 };
 
 // Segment s with prefix[s] <= idx < prefix[s+1] (prefix has nseg+1 entries).
@@ -164,12 +165,27 @@ __global__ void __launch_bounds__(256, 1) b_pair2_all(
     switch (b.n) {
 #define NECG_B2(NN)                                                                          \
   case NN:                                                                                   \
-    if (b.sc.mrank1)                                                                         \
+    if (w.fold == 2) {                                                                       \
+      if (b.sc.mrank1)                                                                       \
+        pair2_diag_tile<NN, true>(br, kt, K, b.nst, w.k0, w.p0, w.p1, b.sc, tp, bp, hp, sp,  \
+                                  flag, acc_s, kacc_s, bra_s, kst_s);                        \
+      else                                                                                   \
+        pair2_diag_tile<NN, false>(br, kt, K, b.nst, w.k0, w.p0, w.p1, b.sc, tp, bp, hp, sp, \
+                                   flag, acc_s, kacc_s, bra_s, kst_s);                       \
+    } else if (w.fold) {                                                                     \
+      if (b.sc.mrank1)                                                                       \
+        pair2_fold_tile<NN, true>(br, kt, K, b.nst, w.k0, w.l0, w.l1, w.p0, w.p1, b.sc, tp,  \
+                                  bp, kp, hp, sp, flag, acc_s, kacc_s, bra_s, kst_s);        \
+      else                                                                                   \
+        pair2_fold_tile<NN, false>(br, kt, K, b.nst, w.k0, w.l0, w.l1, w.p0, w.p1, b.sc, tp, \
+                                   bp, kp, hp, sp, flag, acc_s, kacc_s, bra_s, kst_s);       \
+    } else if (b.sc.mrank1) {                                                                \
       pair2_tile<NN, true>(br, kt, K, b.nst, w.k0, w.l0, w.l1, lup, w.p0, w.p1, b.sc, tp, bp, \
                            kp, hp, sp, flag, acc_s, kacc_s, bra_s, kst_s);                   \
-    else                                                                                     \
+    } else {                                                                                 \
       pair2_tile<NN, false>(br, kt, K, b.nst, w.k0, w.l0, w.l1, lup, w.p0, w.p1, b.sc, tp,   \
                             bp, kp, hp, sp, flag, acc_s, kacc_s, bra_s, kst_s);              \
+    }                                                                                        \
     break;
       NECG_B2(1)
       NECG_B2(2)
@@ -263,7 +279,7 @@ __global__ void __launch_bounds__(256) b_tile_reduce(const BatchSys *__restrict_
   const int v = r / nch, k0 = (r - v * nch) * 64;
   if (b.two) {
     tile_reduce2_block(part + b.part_off, kpart + b.kpart_off, b.K, b.kt, b.ts, b.nacc, v, k0,
-                       red + b.red_off, sh);
+                       red + b.red_off, sh, nullptr, nullptr, b.fold);
   } else if (threadIdx.x < 64 && k0 + (int)threadIdx.x < b.K) {
     tile_reduce_one(part + b.part_off, b.ntiles, b.K, b.nacc, v, k0 + threadIdx.x,
                     red + b.red_off);
@@ -416,6 +432,7 @@ int plan(necg_batch *B) {
     const int ts = (nst + tps - 1) / tps;
     const int kts = (K + kt - 1) / kt;
     b.kt = kt;
+    b.fold = (b.two && tri_fold(kt, 1)) ? 1 : 0;
     b.ts = ts;
     b.tps = tps;
     b.ntiles = kts * ts;
@@ -426,33 +443,44 @@ int plan(necg_batch *B) {
     b.kpart_off = kp_tot;
     if (b.two) kp_tot += (long long)((K + 63) / 64) * ts * K * b.nacc;
     const int li = b.two ? 0 : b.n;
-    auto add = [&](int bt, int y) {
+    auto add = [&](int bt, int y, int f) {
       for (int z = 0; z < ts; ++z) {
         Item it;
         it.sys = s;
         it.k0 = bt * 64;
-        it.l0 = y * kt;
-        it.l1 = std::min(K, it.l0 + kt);
-        if (b.two) {
-          const int up = tri_up(K, kt);
-          if (it.l0 < (it.k0 < up ? up : K)) it.l1 = std::min(it.l1, it.k0 + 64);
+        it.fold = f;
+        if (f) {   // sweeps of the folded diagonal block (two kets each), or its diagonal tile
+          it.l0 = (y - it.k0 / kt) * kt;
+          it.l1 = it.l0 + kt;
+          if (it.l0 == 32) it.fold = 2;
+        } else {
+          it.l0 = y * kt;
+          it.l1 = std::min(K, it.l0 + kt);
+          if (b.two) {
+            const int up = tri_up(K, kt);
+            if (it.l0 < (it.k0 < up ? up : K)) it.l1 = std::min(it.l1, it.k0 + 64);
+          }
         }
         it.p0 = z * tps;
         it.p1 = std::min(nst, it.p0 + tps);
         it.tile = y * ts + z;
-        const double c = (double)std::min(64, K - it.k0) * (it.l1 - it.l0) * (it.p1 - it.p0) *
-                         tc;
+        // fold sweeps at 1.12 ket sweeps; a diagonal tile first of all (it is
+        // short but long-latency)
+        const double c = it.fold == 2 ? 1e30
+                                      : (double)std::min(64, K - it.k0) * (it.l1 - it.l0) *
+                                            (f ? 1.12 : 1.0) * (it.p1 - it.p0) * tc;
         lists[li].push_back(it);
         costs[li].push_back(c);
       }
     };
     if (b.two) {
-      tri_tiles(K, kt, add);
+      tri_tiles(K, kt, b.fold, add);
     } else {
       for (int bt = 0; bt < (K + 63) / 64; ++bt)
-        for (int y = 0; y < kts; ++y) add(bt, y);
+        for (int y = 0; y < kts; ++y) add(bt, y, 0);
     }
   }
+
   if ((rc = dalloc(&B->part, (size_t)part_tot))) return rc;
   if ((rc = dalloc(&B->hpart, (size_t)hp_tot))) return rc;
   if ((rc = dalloc(&B->spart, (size_t)hp_tot))) return rc;

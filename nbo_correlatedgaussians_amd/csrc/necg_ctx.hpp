@@ -51,9 +51,12 @@ struct necg_ctx {
   int *d_tiles = nullptr, *d_xr = nullptr;
   size_t tiles_cap = 0;
   int tiles_K = 0, tiles_kt = 0, ntri = 0, nsplit = 0;
+  // folded diagonal blocks (tri_fold): requested (necg_set_fold, default on)
+  // and whether the current tile list has them
+  int fold_mode = 1, tiles_fold = 0;
   // the lower-triangle passes' last tiling choice (choose_tiling2): kt, ts
   // for K = tc_K, requested (tc_req_kt, tc_req_ts) and tc_ncu CUs
-  int tc_K = 0, tc_req_kt = -1, tc_req_ts = -1, tc_ncu = 0, tc_kt = 0, tc_ts = 0;
+  int tc_K = 0, tc_req_kt = -1, tc_req_ts = -1, tc_ncu = 0, tc_kt = 0, tc_ts = 0, tc_fold = -1;
   // per-(bra tile, term group) ket-side accumulator records, [t][l][slot]
   double *kpart = nullptr;
   size_t kpart_cap = 0;

@@ -89,11 +89,37 @@ __host__ __device__ inline void tri_extent(int K, int kt, int bx, int by, int &l
   if (l0 < lup && k0 + 64 < l1) l1 = k0 + 64;
 }
 
+// Folded diagonal blocks.  The diagonal block of a full bra tile (64
+// functions, bras = kets) holds 64 * 65 / 2 pairs; one wave per ket leaves
+// half of its lanes idle there.  With folding, sweep s (0 <= s < 32) of the
+// block serves two kets, s and s + 32, with every lane its own bra: lanes
+// s .. s + 32 (lane offset d = (lane - s) mod 64 <= 32) pair with ket s,
+// the diagonal at d = 0 included; the other 31 lanes pair with ket s + 32
+// (offset 1 .. 31 from it).  Every off-diagonal pair {a, b} of the block is
+// then evaluated exactly once, by ket b when (a - b) mod 64 is in its range,
+// else by ket a; a pair with bra < ket is as valid as its mirror image in
+// the two-sided form.  The 32 sweeps leave the diagonals of kets 32 .. 63,
+// which one diagonal tile per block evaluates (bra half only).  Folding
+// applies when kt divides 32: the block's 64 / kt ket tiles become 32 / kt
+// fold tiles of kt sweeps (bra partial slots 64 bx / kt + g) and a diagonal
+// tile (slot 64 bx / kt + 32 / kt).
+__host__ __device__ inline bool tri_fold(int kt, int fold) {
+  return fold && kt >= 1 && kt <= 32 && (32 % kt) == 0;
+}
+
+// The ket offset (s or s + 32) that lane `lane` pairs with in sweep s.
+__host__ __device__ inline int fold_ket(int s, int lane) {
+  return ((lane - s) & 63) <= 32 ? s : s + 32;
+}
+
 // One entry of a lower-triangle tile list: word 0 = bx | e << 16 | h << 30,
 // word 1 = by.  h = 0: the whole tile; h = 1, 2: its first and second half of
-// kets (kt even), the second half writing its bra partials to extra slot e.
+// kets (kt even), the second half writing its bra partials to extra slot e;
+// h = 3: fold tile g = by - 64 bx / kt of bra tile bx, sweeps [l0, l1) =
+// [g kt, g kt + kt) of its diagonal block (fold = 1), or for g = 32 / kt its
+// diagonal tile (fold = 2).
 struct TriTile {
-  int bx, by, k0, l0, l1, lup, e;
+  int bx, by, k0, l0, l1, lup, e, fold;
 };
 
 __host__ __device__ inline TriTile tri_tile(const int *tiles, int i, int K, int kt) {
@@ -103,8 +129,17 @@ __host__ __device__ inline TriTile tri_tile(const int *tiles, int i, int K, int 
   t.by = tiles[2 * i + 1];
   t.k0 = 64 * t.bx;
   t.e = -1;
-  tri_extent(K, kt, t.bx, t.by, t.l0, t.l1, t.lup);
+  t.fold = 0;
   const unsigned h = w >> 30;
+  if (h == 3u) {
+    const int g = t.by - t.k0 / kt;
+    t.fold = g * kt == 32 ? 2 : 1;
+    t.l0 = g * kt;
+    t.l1 = t.l0 + kt;
+    t.lup = K;
+    return t;
+  }
+  tri_extent(K, kt, t.bx, t.by, t.l0, t.l1, t.lup);
   if (h == 1u) {
     t.l1 = t.l0 + kt / 2;
   } else if (h == 2u) {
@@ -202,11 +237,17 @@ __device__ __forceinline__ double wave_sum16(const double *v, int lane) {
 // scvv_P C_k, summed over the wave) to the ket accumulator kacc.  R1: the
 // mass matrix is diag(mdiag) + mc 1 1^T (SysConst::mrank1), so M a1 takes
 // one column sum instead of a full product.
-template <int N, bool R1>
+// FOLD (a sweep of a folded diagonal block): kd is this lane's ket record
+// (one of the sweep's two), the lane's ket-side values go to kacc where
+// mA = 1 and to kaccB where mA = 0 (two wave sums), and the bra side's two
+// weight sums, sigma_H and sigma_S, are added to sig[0], sig[1] (registers)
+// instead of their LDS slots, which the fold tile gives to the second ket.
+template <int N, bool R1, bool FOLD = false>
 __device__ __forceinline__ bool pair_term2(const BraLds<N> &b, const double *__restrict__ kd,
                                            const SysConst &sc_, double mb, double mk,
                                            unsigned pw, double &hel, double &sel, LdsSink &sink,
-                                           double *kacc, int lane) {
+                                           double *kacc, int lane, double mA = 1.0,
+                                           double *kaccB = nullptr, double *sig = nullptr) {
   using D = Dim<N>;
   constexpr int NS = D::NS, N3 = D::N3;
   const double *__restrict__ Ap = kd + D::K_AP;
@@ -440,8 +481,13 @@ __device__ __forceinline__ bool pair_term2(const BraLds<N> &b, const double *__r
       sink.add(D::A_YH + t, __builtin_fma(2.0 * woa, d1[t], wo * Phi[t]));
       sink.add(D::A_YS + t, 2.0 * wo * d1[t]);
     }
-    sink.add(D::A_SH, woa);
-    sink.add(D::A_SS, wo);
+    if constexpr (FOLD) {
+      sig[0] += woa;
+      sig[1] += wo;
+    } else {
+      sink.add(D::A_SH, woa);
+      sink.add(D::A_SS, wo);
+    }
     double Tm[N * N];
 #pragma unroll
     for (int i = 0; i < N; ++i)
@@ -524,10 +570,28 @@ __device__ __forceinline__ bool pair_term2(const BraLds<N> &b, const double *__r
     for (int t = Two<N>::NV; t < Two<N>::KV; ++t) kv[t] = 0.0;
 #pragma unroll
     for (int ch = 0; ch < Two<N>::NCH; ++ch) {
-      const double r = wave_sum16(kv + 16 * ch, lane);
       const unsigned tgt = (pw >> (8 * ch)) & 255u;
-      if ((lane & 3) == 0 && tgt != 255u)
-        __hip_atomic_fetch_add(kacc + tgt, r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+      if constexpr (FOLD) {
+        // split by ket: a = this lane's values where it pairs with the first
+        // ket, c = the rest (exact: mA is 0 or 1)
+        double a[16], c[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+          a[i] = kv[16 * ch + i] * mA;
+          c[i] = kv[16 * ch + i] - a[i];
+        }
+        const double ra = wave_sum16(a, lane);
+        const double rc = wave_sum16(c, lane);
+        if ((lane & 3) == 0 && tgt != 255u) {
+          __hip_atomic_fetch_add(kacc + tgt, ra, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+          __hip_atomic_fetch_add(kaccB + tgt, rc, __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_WAVEFRONT);
+        }
+      } else {
+        const double r = wave_sum16(kv + 16 * ch, lane);
+        if ((lane & 3) == 0 && tgt != 255u)
+          __hip_atomic_fetch_add(kacc + tgt, r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+      }
     }
   }
   return ok;
@@ -648,6 +712,187 @@ __device__ __forceinline__ void pair2_tile(const double *__restrict__ bra,
   }
 }
 
+// One fold tile: sweeps [s0, s1) of the folded diagonal block of the full
+// bra tile at k0 (tri_fold), terms [p0, p1), the pair_waves(N) waves taking
+// alternate sweeps.  Sweep s pairs every lane (its own bra k0 + lane) with
+// ket k0 + fold_ket(s, lane): two kets per sweep, each with its own staged
+// record and ket accumulator.  LDS is the regular tile's: the accumulators
+// keep NACC - 2 slots per wave (sigma_H and sigma_S are summed in registers),
+// and the 4 KB this frees at the end of acc_s holds the second ket's
+// accumulator and staging buffer per wave.  Writes the element partials of
+// each lane's pair at [min(k, l)][max(k, l)], the tile's bra accumulators to
+// bpt ([slot][k]) and both kets' records to kpt ([l][slot]).
+template <int N, bool R1>
+__device__ __forceinline__ void pair2_fold_tile(const double *__restrict__ bra,
+                                                const double *__restrict__ ket, int K, int nst,
+                                                int k0, int s0, int s1, int p0, int p1,
+                                                const SysConst &sc,
+                                                const unsigned *__restrict__ tperm,
+                                                double *__restrict__ bpt, double *__restrict__ kpt,
+                                                double *__restrict__ hp, double *__restrict__ sp,
+                                                int *__restrict__ flag, double *acc_s,
+                                                double *kacc_s, double *bra_s, double *kst_s) {
+  using D = Dim<N>;
+  constexpr int W = pair_waves(N);
+  constexpr int KF = Two<N>::KF;
+  constexpr int NV = D::NACC - 2;   // LDS slots: every slot but sigma_H, sigma_S
+  constexpr int NA = NV * 64;
+  static_assert(D::A_SH == NV && D::A_SS == NV + 1, "the two weight slots are the last two");
+  static_assert((D::NACC * 64 - NA) * W == 2 * W * 64,
+                "the two freed slots per wave hold the second ket's buffers");
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int k = k0 + lane;          // a full bra tile: every lane is a bra
+  constexpr int NF = BraLds<N>::NF, FW = (NF + W - 1) / W;
+  double bv[FW];
+#pragma unroll
+  for (int i = 0; i < FW; ++i) {
+    const int f = wv + i * W;
+    bv[i] = f < NF ? bra[(size_t)k * D::BS + bra_lds_src<N>(f)] : 0.0;
+  }
+  double firstA = 0.0, firstB = 0.0;
+  if (s0 + wv < s1 && lane < KF) {
+    firstA = ket[((size_t)(k0 + s0 + wv) * nst + p0) * D::KS + lane];
+    firstB = ket[((size_t)(k0 + s0 + wv + 32) * nst + p0) * D::KS + lane];
+  }
+  for (int v = 0; v < NV; ++v) acc_s[wv * NA + v * 64 + lane] = 0.0;
+  double *tail = acc_s + W * NA;    // W * 128 doubles: kaccB, then kstB, per wave
+  double *kaccA = kacc_s + wv * 64, *kaccB = tail + wv * 64;
+  kaccA[lane] = 0.0;
+  kaccB[lane] = 0.0;
+#pragma unroll
+  for (int i = 0; i < FW; ++i) {
+    const int f = wv + i * W;
+    if (f < NF) bra_s[64 * f + lane] = bv[i];
+  }
+  double *kstA = kst_s + wv * 64, *kstB = tail + W * 64 + wv * 64;
+  if (lane < KF) {
+    kstA[lane] = firstA;
+    kstB[lane] = firstB;
+  }
+  __syncthreads();
+  const BraLds<N> b{bra_s + lane};
+  LdsSink sink{acc_s + wv * NA + lane};
+  double sig[2] = {0.0, 0.0};
+  bool ok = true;
+  for (int s = s0 + wv; s < s1; s += W) {
+    const int d = (lane - s) & 63;
+    const bool first = d <= 32;                 // ket s (its diagonal at d = 0), else s + 32
+    const int l = k0 + (first ? s : s + 32);
+    const double mk = d == 0 ? 0.0 : 1.0;       // the diagonal takes the bra half only
+    const double mA = first ? 1.0 : 0.0;
+    const double *kd = first ? kstA : kstB;
+    double hel = 0.0, sel = 0.0;
+    for (int p = p0; p < p1; ++p) {
+      const int pn = p + 1 < p1 ? p + 1 : p0;
+      const int sn = p + 1 < p1 ? s : s + W;
+      double nxA = 0.0, nxB = 0.0;
+      if (sn < s1 && lane < KF) {
+        nxA = ket[((size_t)(k0 + sn) * nst + pn) * D::KS + lane];
+        nxB = ket[((size_t)(k0 + sn + 32) * nst + pn) * D::KS + lane];
+      }
+      ok &= pair_term2<N, R1, true>(b, kd, sc, 1.0, mk, tperm[p * 64 + lane], hel, sel, sink,
+                                    kaccA, lane, mA, kaccB, sig);
+      if (lane < KF) {   // after every read of the term (in-order LDS operations)
+        kstA[lane] = nxA;
+        kstB[lane] = nxB;
+      }
+    }
+    const size_t o = k < l ? (size_t)k * K + l : (size_t)l * K + k;
+    hp[o] = hel;
+    sp[o] = sel;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    if (lane < D::NACC) {
+      const double va = __hip_atomic_exchange(kaccA + lane, 0.0, __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_WAVEFRONT);
+      const double vb = __hip_atomic_exchange(kaccB + lane, 0.0, __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_WAVEFRONT);
+      kpt[(size_t)(k0 + s) * D::NACC + lane] = va;
+      kpt[(size_t)(k0 + s + 32) * D::NACC + lane] = vb;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  }
+  if (!ok) atomicOr(flag, (int)F_NOT_PD);
+  __syncthreads();
+  // the weight sums: each wave's registers into the freed tail, then every
+  // slot summed over the W waves in a fixed order
+  tail[(2 * wv) * 64 + lane] = sig[0];
+  tail[(2 * wv + 1) * 64 + lane] = sig[1];
+  __syncthreads();
+  double *o = bpt + k;
+  for (int v = wv; v < D::NACC; v += W) {
+    double t = 0.0;
+    if (v < NV) {
+#pragma unroll
+      for (int u = 0; u < W; ++u) t += acc_s[u * NA + v * 64 + lane];
+    } else {
+#pragma unroll
+      for (int u = 0; u < W; ++u) t += tail[(2 * u + (v - NV)) * 64 + lane];
+    }
+    o[(size_t)v * K] = t;
+  }
+}
+
+// The diagonal tile of a folded bra tile: the pairs (k, k) of the upper 32
+// functions of the full bra tile at k0, terms [p0, p1), bra half only; the
+// pair_waves(N) waves take alternate terms and every lane's own ket record is
+// read from the ket table directly.  Writes the element partials at [k][k]
+// and the bra accumulators of all 64 functions (zero for the lower 32) to bpt
+// ([slot][k]), each the sum over the waves in a fixed order.  LDS is the
+// regular tile's.
+template <int N, bool R1>
+__device__ __forceinline__ void pair2_diag_tile(const double *__restrict__ bra,
+                                                const double *__restrict__ ket, int K, int nst,
+                                                int k0, int p0, int p1, const SysConst &sc,
+                                                const unsigned *__restrict__ tperm,
+                                                double *__restrict__ bpt, double *__restrict__ hp,
+                                                double *__restrict__ sp, int *__restrict__ flag,
+                                                double *acc_s, double *kacc_s, double *bra_s,
+                                                double *kst_s) {
+  using D = Dim<N>;
+  constexpr int W = pair_waves(N);
+  constexpr int NF = BraLds<N>::NF;
+  constexpr int NA = D::NACC * 64;
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int k = k0 + lane;
+  for (int f = wv; f < NF; f += W)
+    bra_s[64 * f + lane] = bra[(size_t)k * D::BS + bra_lds_src<N>(f)];
+  for (int v = 0; v < D::NACC; ++v) acc_s[wv * NA + v * 64 + lane] = 0.0;
+  double *kacc = kacc_s + wv * 64;   // the ket side is zero (mk = 0)
+  kacc[lane] = 0.0;
+  __syncthreads();
+  const BraLds<N> b{bra_s + lane};
+  LdsSink sink{acc_s + wv * NA + lane};
+  const double mb = lane >= 32 ? 1.0 : 0.0;
+  double hel = 0.0, sel = 0.0;
+  bool ok = true;
+  for (int p = p0 + wv; p < p1; p += W)
+    ok &= pair_term2<N, R1>(b, ket + ((size_t)k * nst + p) * D::KS, sc, mb, 0.0,
+                            tperm[p * 64 + lane], hel, sel, sink, kacc, lane);
+  if (lane >= 32 && !ok) atomicOr(flag, (int)F_NOT_PD);
+  kst_s[wv * 64 + lane] = hel;   // the staging buffers are free here
+  kacc[lane] = sel;
+  __syncthreads();
+  if (wv == 0 && lane >= 32) {
+    double h = 0.0, s2 = 0.0;
+#pragma unroll
+    for (int u = 0; u < W; ++u) {
+      h += kst_s[u * 64 + lane];
+      s2 += kacc_s[u * 64 + lane];
+    }
+    hp[(size_t)k * K + k] = h;
+    sp[(size_t)k * K + k] = s2;
+  }
+  double *o = bpt + k;
+  for (int v = wv; v < D::NACC; v += W) {
+    double t = 0.0;
+#pragma unroll
+    for (int u = 0; u < W; ++u) t += acc_s[u * NA + v * 64 + lane];
+    o[(size_t)v * K] = t;
+  }
+}
+
 // The partials that function k sums in a lower-triangle pass (tri_tiles):
 // as a bra, those of ket tiles [yb0, yb1) and [yu0, yu1) of its bra tile; as
 // a ket, those of bra tiles [x0, x1).
@@ -655,13 +900,15 @@ struct TriRanges {
   int yb0, yb1, yu0, yu1, x0, x1;
 };
 
-__host__ __device__ inline TriRanges tri_ranges(int K, int kt, int k) {
+__host__ __device__ inline TriRanges tri_ranges(int K, int kt, int k, int fold) {
   const int bx = k >> 6;
   const int nkt = (K + kt - 1) / kt, nbt = (K + 63) / 64;
   const int up = tri_up(K, kt);
   TriRanges r;
   r.yb0 = (64 * bx >= up) ? up / kt : 0;   // tile P alone: its own block
   r.yb1 = (64 * bx + 63) / kt + 1 < nkt ? (64 * bx + 63) / kt + 1 : nkt;
+  if (tri_fold(kt, fold) && 64 * bx + 64 <= K)   // fold tiles and the diagonal slot
+    r.yb1 = 64 * bx / kt + 32 / kt + 1;
   r.yu0 = r.yu1 = 0;
   if (64 * bx < up && up < K) {             // upper pairs with tile P's functions
     r.yu0 = up / kt;
@@ -689,8 +936,8 @@ __device__ __forceinline__ double tile_reduce2_part(const double *__restrict__ b
                                                     int kt, int ts, int nacc, int v, int k,
                                                     int q, int np,
                                                     const double *__restrict__ xpart,
-                                                    const int *__restrict__ xr) {
-  const TriRanges r = tri_ranges(K, kt, k);
+                                                    const int *__restrict__ xr, int fold) {
+  const TriRanges r = tri_ranges(K, kt, k, fold);
   double s = 0.0;
   for (int t = r.yb0 * ts + q; t < r.yb1 * ts; t += np)
     s += bpart[((size_t)t * nacc + v) * K + k];
@@ -712,12 +959,13 @@ __device__ __forceinline__ void tile_reduce2_block(const double *__restrict__ bp
                                                    const double *__restrict__ kpart, int K,
                                                    int kt, int ts, int nacc, int v, int k0,
                                                    double *__restrict__ red, double *sh,
-                                                   const double *__restrict__ xpart = nullptr,
-                                                   const int *__restrict__ xr = nullptr) {
+                                                   const double *__restrict__ xpart,
+                                                   const int *__restrict__ xr, int fold) {
   const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
   const int k = k0 + lane;
   sh[threadIdx.x] =
-      k < K ? tile_reduce2_part(bpart, kpart, K, kt, ts, nacc, v, k, q, 4, xpart, xr) : 0.0;
+      k < K ? tile_reduce2_part(bpart, kpart, K, kt, ts, nacc, v, k, q, 4, xpart, xr, fold)
+            : 0.0;
   __syncthreads();
   if (q == 0 && k < K)
     red[(size_t)v * K + k] = ((sh[lane] + sh[64 + lane]) + sh[128 + lane]) + sh[192 + lane];

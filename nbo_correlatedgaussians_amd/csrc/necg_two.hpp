@@ -104,52 +104,75 @@ inline bool two_sided_table(const necg_system *sys, std::vector<unsigned> &tab) 
 }
 
 // Lower-triangle tiles of the two-sided and element-only passes, in launch
-// order (full tiles first): emit(bx, by) for bra tile bx (64 functions) and
-// ket tile by (kt functions).  Bra tile bx covers its kets l <= its last
-// function.  When K is not a multiple of 64 and kt divides 64, the functions
-// of the last, partial bra tile P are paired with the full tiles' lanes
-// instead (upper pairs, kets >= tri_up(K, kt)), and tile P covers only its
-// own diagonal block: fewer idle lanes.
+// order (full tiles first): emit(bx, by, f) for bra tile bx (64 functions)
+// and ket tile by (kt functions), f = 1 for the tiles of a folded diagonal
+// block (tri_fold: by = 64 bx / kt + g, sweeps [g kt, g kt + kt) for
+// g < 32 / kt, the block's diagonal tile for g = 32 / kt).
+// Bra tile bx covers its kets l <= its last function.  When K is not a
+// multiple of 64 and kt divides 64, the functions of the last, partial bra
+// tile P are paired with the full tiles' lanes instead (upper pairs, kets >=
+// tri_up(K, kt)), and tile P covers only its own diagonal block: fewer idle
+// lanes.  With fold set (and kt dividing 32) every full bra tile's diagonal
+// block is folded.
 template <class Emit>
-void tri_tiles(int K, int kt, Emit emit) {
+void tri_tiles(int K, int kt, int fold, Emit emit) {
   const int nbt = (K + 63) / 64, nkt = (K + kt - 1) / kt;
   const int up = tri_up(K, kt);
+  const bool fd = tri_fold(kt, fold);
   std::vector<int> part;
   auto add = [&](int bx, int by) {
     const int l0 = by * kt, k0 = 64 * bx;
     const int lup = k0 < up ? up : K;
     int l1 = std::min(K, l0 + kt);
     if (l0 < lup) l1 = std::min(l1, k0 + 64);
-    if (l1 - l0 == kt && std::min(64, K - k0) == 64) emit(bx, by);
+    if (l1 - l0 == kt && std::min(64, K - k0) == 64) emit(bx, by, 0);
     else { part.push_back(bx); part.push_back(by); }
   };
+  // folded blocks first, their diagonal tiles (short) then their fold tiles
+  // (1.1 ordinary tiles): the longer tiles start before the ordinary ones
+  const int nfb = fd ? K / 64 : 0;
+  for (int bx = 0; bx < nfb; ++bx) emit(bx, 64 * bx / kt + 32 / kt, 1);
+  for (int bx = 0; bx < nfb; ++bx)
+    for (int g = 0; g < 32 / kt; ++g) emit(bx, 64 * bx / kt + g, 1);
   for (int bx = 0; bx < nbt; ++bx) {
     const int k0 = 64 * bx;
     const int lo = (k0 >= up) ? up / kt : 0;           // tile P alone: its own block
     const int nyl = std::min(nkt, (k0 + 63) / kt + 1);
-    for (int by = lo; by < nyl; ++by) add(bx, by);
+    if (bx < nfb) {
+      for (int by = lo; by < k0 / kt; ++by) add(bx, by);   // below the diagonal block
+    } else {
+      for (int by = lo; by < nyl; ++by) add(bx, by);
+    }
     if (k0 < up && up < K)                             // upper pairs with tile P's functions
       for (int by = up / kt; by < nkt; ++by) add(bx, by);
   }
-  for (size_t i = 0; i < part.size(); i += 2) emit(part[i], part[i + 1]);
+  for (size_t i = 0; i < part.size(); i += 2) emit(part[i], part[i + 1], 0);
 }
 
-inline long long tri_count(int K, int kt) {
+inline long long tri_count(int K, int kt, int fold) {
   long long n = 0;
-  tri_tiles(K, kt, [&](int, int) { ++n; });
+  tri_tiles(K, kt, fold, [&](int, int, int) { ++n; });
   return n;
 }
 
-// Tiles of the tri_tiles list of (K, kt), and how many of them lead the list
-// as whole tiles (kt kets, 64 bras).
-inline void tri_counts(int K, int kt, long long &nt, long long &nfull) {
+// Tiles of the tri_tiles list of (K, kt), and how many whole regular tiles
+// (kt kets, 64 bras, not fold tiles) come before its first partial tile.
+inline void tri_counts(int K, int kt, int fold, long long &nt, long long &nfull) {
   nt = nfull = 0;
-  tri_tiles(K, kt, [&](int bx, int by) {
+  bool partial = false;
+  tri_tiles(K, kt, fold, [&](int bx, int by, int f) {
+    ++nt;
+    if (f) return;
     int l0, l1, lup;
     tri_extent(K, kt, bx, by, l0, l1, lup);
-    if (l1 - l0 == kt && K - 64 * bx >= 64 && nt == nfull) ++nfull;
-    ++nt;
+    if (l1 - l0 == kt && K - 64 * bx >= 64 && !partial) ++nfull;
+    else partial = true;
   });
+}
+
+// Bra tiles whose diagonal block is folded (each has a diagonal tile).
+inline int tri_fold_tiles(int K, int kt, int fold) {
+  return tri_fold(kt, fold) ? K / 64 : 0;
 }
 
 // Whether a tri_tiles list of nt tiles (nfull whole-kt tiles first) on ncu
@@ -164,34 +187,46 @@ inline int tri_split_count(long long nt, long long nfull, int kt, int ncu) {
 }
 
 // The tile list of a lower-triangle pass as the kernels take it (TriTile
-// words): the tri_tiles list with its first r whole tiles split into halves
-// (tri_split_count) and moved to the end, first and second half of each tile
-// side by side.  The first tiles belong to the bra tiles with the fewest ket
-// tiles, whose reduction (tile_reduce2) has the least to sum: their extra
-// slots leave the longest reduction, the last bra tile's, as it was.
-// xr[2 bx], xr[2 bx + 1] = the extra-slot range of bra tile bx's split tiles
-// (contiguous: the split tiles are in bra-tile order).  Returns r.
-inline int tri_tile_list(int K, int kt, int ncu, std::vector<int> &list, std::vector<int> &xr) {
+// words): the tri_tiles list with its first r whole regular tiles split into
+// halves (tri_split_count) and moved to the end, first and second half of
+// each tile side by side.  The first tiles belong to the bra tiles with the
+// fewest ket tiles, whose reduction (tile_reduce2) has the least to sum:
+// their extra slots leave the longest reduction, the last bra tile's, as it
+// was.  xr[2 bx], xr[2 bx + 1] = the extra-slot range of bra tile bx's split
+// tiles (contiguous: the split tiles are in bra-tile order).  Returns r.
+inline int tri_tile_list(int K, int kt, int ncu, int fold, std::vector<int> &list,
+                         std::vector<int> &xr) {
   std::vector<int> t;
   long long nt, nfull;
-  tri_counts(K, kt, nt, nfull);
+  tri_counts(K, kt, fold, nt, nfull);
   t.reserve((size_t)2 * nt);
-  tri_tiles(K, kt, [&](int bx, int by) {
-    t.push_back(bx);
+  tri_tiles(K, kt, fold, [&](int bx, int by, int f) {
+    t.push_back((int)((unsigned)bx | (f ? 3u << 30 : 0u)));
     t.push_back(by);
   });
   const int nbt = (K + 63) / 64;
   xr.assign((size_t)2 * nbt, 0);
-  const int r = tri_split_count(nt, nfull, kt, ncu);
+  // the diagonal tiles (short, dispatched first) take no slot of a round
+  const int r = tri_split_count(nt - tri_fold_tiles(K, kt, fold), nfull, kt, ncu);
   if (r == 0) {
     list.swap(t);
     return 0;
   }
   list.clear();
   list.reserve((size_t)2 * (nt + r));
-  list.insert(list.end(), t.begin() + 2 * r, t.end());
+  std::vector<int> split;   // the first r whole regular tiles, in list order
+  for (size_t i = 0; i < t.size(); i += 2) {
+    const bool regular = ((unsigned)t[i] >> 30) == 0u;
+    if (regular && (int)split.size() < 2 * r) {
+      split.push_back(t[i]);
+      split.push_back(t[i + 1]);
+    } else {
+      list.push_back(t[i]);
+      list.push_back(t[i + 1]);
+    }
+  }
   for (int e = 0; e < r; ++e) {
-    const int bx = t[(size_t)2 * e], by = t[(size_t)2 * e + 1];
+    const int bx = split[(size_t)2 * e], by = split[(size_t)2 * e + 1];
     if (xr[(size_t)2 * bx + 1] == 0) xr[(size_t)2 * bx] = e;
     xr[(size_t)2 * bx + 1] = e + 1;
     for (unsigned h = 1; h <= 2; ++h) {

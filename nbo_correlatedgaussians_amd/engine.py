@@ -96,6 +96,10 @@ class Engine:
         _lib.raise_for(self.lib.necg_set_tiling(self._ctx, ket_tile, term_split, waves),
                        "necg_set_tiling")
 
+    def set_fold(self, fold: bool = True) -> None:
+        """Folded diagonal blocks in the lower-triangle passes (default on)."""
+        _lib.raise_for(self.lib.necg_set_fold(self._ctx, int(bool(fold))), "necg_set_fold")
+
     def set_launch_cus(self, ncu: int = 0) -> None:
         """Compute units the pair passes size their launches for (0: the device's)."""
         _lib.raise_for(self.lib.necg_set_launch_cus(self._ctx, ncu), "necg_set_launch_cus")

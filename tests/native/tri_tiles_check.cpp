@@ -1,12 +1,14 @@
 // Host check of the lower-triangle tiling of the two-sided pass (tri_tiles,
 // tri_up, tri_ranges in csrc/necg_two.hpp and csrc/necg_pair2.hpp), with the
-// per-tile ket ranges and pair rules of pair2_tile: every unordered pair of
-// functions {k, l} is evaluated exactly once, each function's bra and ket
-// partials are summed over exactly the tiles that wrote them, and no tile is
-// empty.  The same for the lists the kernels take (tri_tile_list, decoded by
-// tri_tile), whose last round may be split into half tiles: each function's
-// bra partials then also come from the extra slots xr names.  Prints "ok" and
-// exits 0, or names the first failure.
+// per-tile ket ranges and pair rules of pair2_tile and pair2_fold_tile: every
+// unordered pair of functions {k, l} is evaluated exactly once (the
+// diagonals that fold tiles leave counted in the diagonal tiles), each
+// function's bra and ket partials are summed over exactly the tiles that
+// wrote them, and no tile is empty.  The same for the lists the kernels take
+// (tri_tile_list, decoded by tri_tile), whose last round may be split into
+// half tiles: each function's bra partials then also come from the extra
+// slots xr names.  Folded and unfolded tilings.  Prints "ok" and exits 0, or
+// names the first failure.
 #include <cstdio>
 #include <set>
 #include <vector>
@@ -15,13 +17,74 @@
 
 using namespace necg;
 
-static int check(int K, int kt) {
-  std::vector<unsigned char> seen((size_t)K * K, 0);
-  std::vector<std::set<int>> bra_tiles(K), ket_tiles(K);
+// The pairs and partial slots of one tile: bra partial slot by for the bras
+// [k0, k0 + 64), a ket record for bra tile bx per ket; returns false on a
+// pair seen twice.
+struct Cover {
+  int K;
+  std::vector<unsigned char> seen;
+  std::vector<std::set<int>> bra_tiles, ket_tiles;
+  explicit Cover(int K_) : K(K_), seen((size_t)K_ * K_, 0), bra_tiles(K_), ket_tiles(K_) {}
+  bool pair(int k, int l) {
+    const int a = k > l ? k : l, b = k > l ? l : k;
+    return seen[(size_t)a * K + b]++ == 0;
+  }
+  // a regular tile: bras [k0, k0+64) x kets [l0, l1), pairs k >= l or l >= lup
+  bool regular(int bx, int y, int l0, int l1, int lup) {
+    const int k0 = 64 * bx;
+    bool ok = true;
+    for (int k = k0; k < std::min(K, k0 + 64); ++k) bra_tiles[k].insert(y);
+    for (int l = l0; l < l1; ++l) ket_tiles[l].insert(bx);
+    for (int k = k0; k < std::min(K, k0 + 64); ++k)
+      for (int l = l0; l < l1; ++l)
+        if (k >= l || l >= lup) ok &= pair(k, l);
+    return ok;
+  }
+  // a fold tile: sweeps [s0, s1) of bra tile bx's diagonal block
+  bool fold(int bx, int y, int s0, int s1) {
+    const int k0 = 64 * bx;
+    bool ok = true;
+    for (int k = k0; k < k0 + 64; ++k) bra_tiles[k].insert(y);
+    for (int s = s0; s < s1; ++s) {
+      ket_tiles[k0 + s].insert(bx);
+      ket_tiles[k0 + s + 32].insert(bx);
+      for (int j = 0; j < 64; ++j) ok &= pair(k0 + j, k0 + fold_ket(s, j));
+    }
+    return ok;
+  }
+  // the diagonal tile of a folded bra tile: diagonals of its upper 32
+  // functions, bra partial slot y for all 64
+  bool diag(int bx, int y) {
+    const int k0 = 64 * bx;
+    bool ok = true;
+    for (int k = k0; k < k0 + 64; ++k) bra_tiles[k].insert(y);
+    for (int j = 32; j < 64; ++j) ok &= pair(k0 + j, k0 + j);
+    return ok;
+  }
+  int missed() const {
+    for (int a = 0; a < K; ++a)
+      for (int b = 0; b <= a; ++b)
+        if (seen[(size_t)a * K + b] != 1) return a * K + b;
+    return -1;
+  }
+};
+
+static int check(int K, int kt, int fold) {
+  Cover c(K);
   const int up = tri_up(K, kt);
   int bad = 0;
-  tri_tiles(K, kt, [&](int bx, int by) {
-    const int k0 = 64 * bx, l0 = by * kt;
+  tri_tiles(K, kt, fold, [&](int bx, int by, int f) {
+    const int k0 = 64 * bx;
+    if (f) {
+      const int g = by - k0 / kt;
+      if (g * kt == 32) {
+        if (!c.diag(bx, by)) ++bad;
+      } else if (!c.fold(bx, by, g * kt, g * kt + kt)) {
+        ++bad;
+      }
+      return;
+    }
+    const int l0 = by * kt;
     const int lup = k0 < up ? up : K;
     const int l1 = l0 < lup ? std::min(std::min(K, l0 + kt), k0 + 64) : std::min(K, l0 + kt);
     if (l1 <= l0 || k0 >= K) {
@@ -29,129 +92,137 @@ static int check(int K, int kt) {
       ++bad;
       return;
     }
-    for (int k = k0; k < std::min(K, k0 + 64); ++k) bra_tiles[k].insert(by);
-    for (int l = l0; l < l1; ++l) ket_tiles[l].insert(bx);
-    for (int k = k0; k < std::min(K, k0 + 64); ++k)
-      for (int l = l0; l < l1; ++l)
-        if (k >= l || l >= lup) {
-          const int a = k > l ? k : l, b = k > l ? l : k;
-          if (seen[(size_t)a * K + b]++) {
-            std::printf("K=%d kt=%d: pair (%d, %d) twice\n", K, kt, a, b);
-            ++bad;
-          }
-        }
+    if (!c.regular(bx, by, l0, l1, lup)) ++bad;
   });
-  for (int a = 0; a < K; ++a)
-    for (int b = 0; b <= a; ++b)
-      if (seen[(size_t)a * K + b] != 1) {
-        std::printf("K=%d kt=%d: pair (%d, %d) seen %d times\n", K, kt, a, b,
-                    seen[(size_t)a * K + b]);
-        return bad + 1;
-      }
+  if (bad) {
+    std::printf("K=%d kt=%d fold=%d: a pair twice\n", K, kt, fold);
+    return bad;
+  }
+  const int m = c.missed();
+  if (m >= 0) {
+    std::printf("K=%d kt=%d fold=%d: pair (%d, %d) not seen once\n", K, kt, fold, m / K, m % K);
+    return 1;
+  }
   for (int k = 0; k < K; ++k) {
-    const TriRanges r = tri_ranges(K, kt, k);
+    const TriRanges r = tri_ranges(K, kt, k, fold);
     std::set<int> yb, xk;
     for (int y = r.yb0; y < r.yb1; ++y) yb.insert(y);
     for (int y = r.yu0; y < r.yu1; ++y)
       if (!yb.insert(y).second) ++bad;
     for (int x = r.x0; x < r.x1; ++x) xk.insert(x);
-    if (yb != bra_tiles[k] || xk != ket_tiles[k]) {
-      std::printf("K=%d kt=%d: function %d reduces the wrong tiles\n", K, kt, k);
+    if (yb != c.bra_tiles[k] || xk != c.ket_tiles[k]) {
+      std::printf("K=%d kt=%d fold=%d: function %d reduces the wrong tiles\n", K, kt, fold, k);
       return bad + 1;
     }
   }
-  if (tri_count(K, kt) <= 0) ++bad;
+  if (tri_count(K, kt, fold) <= 0) ++bad;
   return bad;
 }
 
 // The kernels' list for ncu CUs; counts the lists that have split tiles.
-static int check_list(int K, int kt, int ncu, int &nsplit_lists) {
+static int check_list(int K, int kt, int ncu, int fold, int &nsplit_lists) {
   std::vector<int> list, xr;
-  const int r = tri_tile_list(K, kt, ncu, list, xr);
+  const int r = tri_tile_list(K, kt, ncu, fold, list, xr);
   long long nt, nfull;
-  tri_counts(K, kt, nt, nfull);
-  if (nt != tri_count(K, kt) || (long long)list.size() != 2 * (nt + r) ||
-      r != tri_split_count(nt, nfull, kt, ncu)) {
-    std::printf("K=%d kt=%d ncu=%d: list size\n", K, kt, ncu);
+  tri_counts(K, kt, fold, nt, nfull);
+  if (nt != tri_count(K, kt, fold) || (long long)list.size() != 2 * (nt + r) ||
+      r != tri_split_count(nt - tri_fold_tiles(K, kt, fold), nfull, kt, ncu)) {
+    std::printf("K=%d kt=%d ncu=%d fold=%d: list size\n", K, kt, ncu, fold);
     return 1;
   }
   if (r) ++nsplit_lists;
-  std::vector<unsigned char> seen((size_t)K * K, 0);
-  std::vector<std::set<int>> bra_y(K), bra_e(K), ket_x(K);
+  Cover c(K);
+  std::vector<std::set<int>> bra_e(K);
   std::vector<int> e_use(r, 0);
   for (size_t i = 0; i < list.size() / 2; ++i) {
     const TriTile t = tri_tile(list.data(), (int)i, K, kt);
-    if (t.l1 <= t.l0 || t.k0 >= K || t.e >= r) {
-      std::printf("K=%d kt=%d ncu=%d: bad entry %zu\n", K, kt, ncu, i);
+    if (t.l1 <= t.l0 || t.k0 >= K || t.e >= r || (t.fold && (t.e >= 0 || t.k0 + 64 > K))) {
+      std::printf("K=%d kt=%d ncu=%d fold=%d: bad entry %zu\n", K, kt, ncu, fold, i);
       return 1;
     }
-    if (t.e >= 0) ++e_use[t.e];
-    for (int k = t.k0; k < std::min(K, t.k0 + 64); ++k) {
-      if (t.e >= 0) bra_e[k].insert(t.e);
-      else bra_y[k].insert(t.by);
+    bool ok;
+    if (t.fold == 2) {
+      ok = c.diag(t.bx, t.by);
+    } else if (t.fold) {
+      ok = c.fold(t.bx, t.by, t.l0, t.l1);
+    } else if (t.e >= 0) {
+      ++e_use[t.e];
+      for (int k = t.k0; k < std::min(K, t.k0 + 64); ++k) bra_e[k].insert(t.e);
+      // the second half's bra partials go to extra slot e, not to slot by
+      ok = c.regular(t.bx, -1, t.l0, t.l1, t.lup);
+      for (int k = t.k0; k < std::min(K, t.k0 + 64); ++k) c.bra_tiles[k].erase(-1);
+    } else {
+      ok = c.regular(t.bx, t.by, t.l0, t.l1, t.lup);
     }
-    for (int l = t.l0; l < t.l1; ++l) ket_x[l].insert(t.bx);
-    for (int k = t.k0; k < std::min(K, t.k0 + 64); ++k)
-      for (int l = t.l0; l < t.l1; ++l)
-        if (k >= l || l >= t.lup) {
-          const int a = k > l ? k : l, b = k > l ? l : k;
-          if (seen[(size_t)a * K + b]++) {
-            std::printf("K=%d kt=%d ncu=%d: pair (%d, %d) twice\n", K, kt, ncu, a, b);
-            return 1;
-          }
-        }
+    if (!ok) {
+      std::printf("K=%d kt=%d ncu=%d fold=%d: a pair twice\n", K, kt, ncu, fold);
+      return 1;
+    }
   }
   for (int e = 0; e < r; ++e)
     if (e_use[e] != 1) {
       std::printf("K=%d kt=%d ncu=%d: extra slot %d used %d times\n", K, kt, ncu, e, e_use[e]);
       return 1;
     }
-  for (int a = 0; a < K; ++a)
-    for (int b = 0; b <= a; ++b)
-      if (seen[(size_t)a * K + b] != 1) {
-        std::printf("K=%d kt=%d ncu=%d: pair (%d, %d) missed\n", K, kt, ncu, a, b);
-        return 1;
-      }
+  const int m = c.missed();
+  if (m >= 0) {
+    std::printf("K=%d kt=%d ncu=%d fold=%d: pair (%d, %d) missed\n", K, kt, ncu, fold, m / K,
+                m % K);
+    return 1;
+  }
   for (int k = 0; k < K; ++k) {
-    const TriRanges g = tri_ranges(K, kt, k);
+    const TriRanges g = tri_ranges(K, kt, k, fold);
     std::set<int> yb, ex, xk;
     for (int y = g.yb0; y < g.yb1; ++y) yb.insert(y);
     for (int y = g.yu0; y < g.yu1; ++y) yb.insert(y);
     for (int e = xr[2 * (k >> 6)]; e < xr[2 * (k >> 6) + 1]; ++e) ex.insert(e);
     for (int x = g.x0; x < g.x1; ++x) xk.insert(x);
-    if (yb != bra_y[k] || ex != bra_e[k] || xk != ket_x[k]) {
-      std::printf("K=%d kt=%d ncu=%d: function %d reduces the wrong partials\n", K, kt, ncu, k);
+    if (yb != c.bra_tiles[k] || ex != bra_e[k] || xk != c.ket_tiles[k]) {
+      std::printf("K=%d kt=%d ncu=%d fold=%d: function %d reduces the wrong partials\n", K, kt,
+                  ncu, fold, k);
       return 1;
     }
   }
   return 0;
 }
 
-int main() {
+int main(int argc, char **argv) {
+  // "quick": a smaller sweep (for builds with sanitizers, several times slower)
+  const bool quick = argc > 1 && argv[1][0] == 'q';
   const int kts[] = {1, 2, 3, 4, 5, 7, 8, 12, 16, 32, 64, 100};
   int bad = 0;
-  for (int K = 1; K <= 300 && !bad; ++K)
-    for (int kt : kts) bad += check(K, kt);
-  const int big[] = {511, 513, 1000, 1024, 1025, 2048, 2049};
-  for (int K : big)
-    for (int kt : kts) bad += check(K, kt);
+  for (int fold = 0; fold <= 1; ++fold) {
+    for (int K = 1; K <= (quick ? 140 : 300) && !bad; ++K)
+      for (int kt : kts) bad += check(K, kt, fold);
+    const int big[] = {511, 513, 1000, 1024, 1025, 2048, 2049};
+    for (int K : big)
+      if (!quick || K < 1100)
+        for (int kt : kts) bad += check(K, kt, fold);
+  }
   // the kernels' lists, with and without split last rounds
   int nsplit_lists = 0;
   const int cus[] = {1, 2, 3, 5, 8, 9, 20, 21, 40, 50, 80, 256};
-  for (int K = 1; K <= 200 && !bad; K += 7)
-    for (int kt : kts)
-      for (int ncu : cus) bad += check_list(K, kt, ncu, nsplit_lists);
-  for (int K : big)
-    for (int kt : {2, 4, 8, 16})
-      for (int ncu : {200, 256}) bad += check_list(K, kt, ncu, nsplit_lists);
+  for (int fold = 0; fold <= 1; ++fold) {
+    for (int K = 1; K <= 200 && !bad; K += (quick ? 13 : 7))
+      for (int kt : kts)
+        for (int ncu : cus) bad += check_list(K, kt, ncu, fold, nsplit_lists);
+    const int big[] = {511, 513, 1000, 1024, 1025, 2048, 2049};
+    for (int K : big)
+      for (int kt : {2, 4, 8, 16})
+        for (int ncu : {200, 256}) bad += check_list(K, kt, ncu, fold, nsplit_lists);
+  }
   {
     std::vector<int> l, x;
-    if (tri_tile_list(2048, 8, 256, l, x) != 128) {   // LiH K = 2048: 4,224 tiles on 256 CUs
+    if (tri_tile_list(2048, 8, 256, 0, l, x) != 128) {   // LiH K = 2048: 4,224 tiles
       std::printf("K=2048 kt=8: expected 128 split tiles\n");
       ++bad;
     }
+    if (tri_count(2048, 8, 1) != 4128 || tri_tile_list(2048, 8, 256, 1, l, x) != 0) {
+      std::printf("K=2048 kt=8 folded: expected 4,096 + 32 diagonal tiles, none split\n");
+      ++bad;
+    }
   }
-  if (nsplit_lists < 50) {
+  if (nsplit_lists < (quick ? 20 : 50)) {
     std::printf("only %d lists with split tiles\n", nsplit_lists);
     ++bad;
   }

@@ -80,6 +80,7 @@ def test_two_sided_split_last_round(oracle_lib):
     # 40; kt 2 -> 192, 32 on 80.  K 150 adds the partial bra tile's tiles and
     # the upper pairs: 33, 66 and 129 tiles, of which 3, 3 and 9 are split.
     cases = ((192, 8, 20), (192, 4, 40), (192, 2, 80), (150, 8, 10), (150, 4, 21), (150, 2, 60))
+    eng.set_fold(False)     # the tile counts above are those of the unfolded lists
     for K in (192, 150):
         iv, C = necg.generate_parameters(s, K, 5)
         x = necg.pack_x(iv, C)
@@ -100,6 +101,46 @@ def test_two_sided_split_last_round(oracle_lib):
             gt = eng.energy_bwd_t(xd, tape).cpu().numpy()
             assert float(Ef) == runs[0][0] and np.array_equal(gt, runs[0][1])
             assert abs(eng.energy(x) - Eo) <= 1e-12 * abs(Eo)
+    eng.set_launch_cus(0)
+    eng.set_tiling(0, 0, 0)
+    eng.set_fold(True)
+    eng.check()
+
+
+@pytest.mark.parametrize("name,K,seed", [("LiH", 64, 21), ("LiH", 150, 22), ("LiH", 192, 23),
+                                         ("He", 130, 24), ("H2", 100, 25), ("Be", 65, 26)])
+def test_folded_diagonal_blocks(oracle_lib, name, K, seed):
+    """Folded diagonal blocks (two kets per sweep, the diagonal pass for the
+    upper halves' diagonals) for every ket tile that divides 32, against the
+    unfolded pass and the oracle: the gradient pass, repeated bitwise, its
+    tape's backward, and the element-only forward that walks the same tile
+    list.  K = 150 adds a partial bra tile with upper pairs (unfolded), 192
+    a list whose last round is split on 16 CUs (fold tiles are not split)."""
+    import torch
+    s = system_by_name(name)
+    iv, C = necg.generate_parameters(s, K, seed)
+    x = necg.pack_x(iv, C)
+    Eo, go = oracle_lib.Oracle(s).outprod(x)
+    xd = torch.as_tensor(x, device="cuda:0")
+    eng = _engine(s)
+    eng.set_pair_mode(1)
+    for kt in (2, 4, 8, 16, 32):
+        for ncu in ((0, 16) if K == 192 else (0,)):
+            eng.set_launch_cus(ncu)
+            eng.set_tiling(kt, 1, 0)
+            eng.set_fold(False)
+            Eu, gu = eng.outprod(x)
+            eng.set_fold(True)
+            print(f"fold {name} K={K} kt={kt} cus={ncu}", flush=True)
+            runs = [eng.outprod(x) for _ in range(2)]
+            for E, g in runs:
+                assert _close(E, g, Eo, go, 1e-12, 1e-9), (kt, ncu)
+                assert _close(E, g, Eu, gu, 1e-13, 1e-11), (kt, ncu)
+            assert runs[0][0] == runs[1][0] and np.array_equal(runs[0][1], runs[1][1])
+            Ef, tape = eng.energy_fwd_t(xd)
+            gt = eng.energy_bwd_t(xd, tape).cpu().numpy()
+            assert float(Ef) == runs[0][0] and np.array_equal(gt, runs[0][1])
+            assert abs(eng.energy(x) - Eo) <= 1e-12 * max(abs(Eo), 1.0)
     eng.set_launch_cus(0)
     eng.set_tiling(0, 0, 0)
     eng.check()

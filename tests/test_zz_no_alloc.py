@@ -75,7 +75,7 @@ def test_noalloc_refuses_what_does_not_fit():
     Er, gr = eng.outprod(small.cpu().numpy())
     assert f.item() == Er and np.array_equal(g.cpu().numpy(), gr)
     fits = []
-    for K in (500, 600, 700, 800):    # where the 64 MiB scratch ends for LiH
+    for K in (300, 400, 500, 600, 700, 800):    # where the 64 MiB scratch ends for LiH
         x = torch.as_tensor(necg.pack_x(*necg.generate_parameters(s, K, 5)), device="cuda:0")
         rc, f, g = _call(lib, "necg_outprod_noalloc", eng, x)
         assert rc in (0, _lib.NECG_ERR_SCRATCH)
@@ -83,4 +83,4 @@ def test_noalloc_refuses_what_does_not_fit():
             fits.append(K)
     eng.check()
     print(f"LiH K that fit the scratch: {fits}", flush=True)
-    assert 500 in fits and 800 not in fits
+    assert 300 in fits and 800 not in fits and fits == sorted(fits)
