@@ -243,7 +243,9 @@ int necg_batch_layout(necg_batch *batch, int64_t *x_offsets);
 
 /* Energy and gradient of every system: f[s] = E_s, g = dE_s/dx_s per segment.
  * A fixed sequence of launches (no host sync, no memset): capturable in a
- * hipGraph.  x, f, g: device.  Async. */
+ * hipGraph.  Whole-batch evaluations of one batch must not overlap (they
+ * share the pair pass's work counter as well as every result).  x, f, g:
+ * device.  Async. */
 int necg_batch_outprod(necg_batch *batch, const double *x, double *f, double *g, void *stream);
 
 /* As necg_batch_outprod for the systems s with active[s] != 0 only (active:

@@ -138,21 +138,29 @@ __global__ void __launch_bounds__(64 * pair_waves(N), 1) b_pair(
 // Two-sided persistent pair kernel: the lower-triangle items (tri_tiles) of
 // every system whose terms allow the two-sided form, all particle counts
 // n <= 5 in one launch (items sorted by decreasing cost, so the small
-// systems' items fill the tail).  LDS is sized for n = 5.
+// systems' items fill the tail).  LDS is sized for n = 5.  Workgroup g takes
+// item g first; with a work counter (ctr, zero at launch) each workgroup then
+// takes the next unclaimed item as it finishes one (the longest items first,
+// the tail filled by whichever workgroup is free); without one it strides by
+// the grid.  Every item writes its own partial slots, so the results do not
+// depend on which workgroup took it.
 __global__ void __launch_bounds__(256, 1) b_pair2_all(
     const BatchSys *__restrict__ sys, const Item *__restrict__ items, int nitems,
     const int32_t *__restrict__ active, const double *__restrict__ bra,
     const double *__restrict__ ket, const unsigned *__restrict__ tperm,
     double *__restrict__ part, double *__restrict__ kpart, double *__restrict__ hpart,
-    double *__restrict__ spart, int *__restrict__ flag) {
+    double *__restrict__ spart, int *__restrict__ flag, int *__restrict__ ctr) {
   static_assert(pair_waves(5) == 4 && pair_waves(1) == 4, "one workgroup shape for n <= 5");
   __shared__ double acc_s[PairLds<5, MODE_SPLIT>::ACC];
   __shared__ double kacc_s[4 * 64];
   __shared__ double bra_s[BraLds<5>::NF * 64];
   __shared__ double kst_s[4 * 64];
-  for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+  for (int it = blockIdx.x; it < nitems;) {
     const Item w = items[it];
-    if (active && !active[w.sys]) continue;  // uniform over the workgroup
+    if (active && !active[w.sys]) {  // uniform over the workgroup
+      it += gridDim.x;
+      continue;
+    }
     const BatchSys &b = sys[w.sys];
     const int K = b.K;
     const int z = w.tile % b.ts, bx = w.k0 >> 6;
@@ -196,8 +204,23 @@ __global__ void __launch_bounds__(256, 1) b_pair2_all(
       default: break;
     }
     __syncthreads();  // the next item overwrites the LDS tables
+    if (ctr) {
+      // the claim passes through kacc_s[0], which the next tile clears only
+      // after the second barrier
+      if (threadIdx.x == 0)
+        kacc_s[0] = __builtin_bit_cast(double, (long long)(atomicAdd(ctr, 1) + (int)gridDim.x));
+      __syncthreads();
+      it = (int)__builtin_bit_cast(long long, kacc_s[0]);
+      __syncthreads();
+    } else {
+      it += gridDim.x;
+    }
   }
 }
+
+// Clears the work counter of b_pair2_all (a kernel, not a memset: the batch
+// evaluation is captured in HIP graphs).
+__global__ void b_clear_counter(int *__restrict__ ctr) { *ctr = 0; }
 
 // one 256-lane workgroup per (system, pair of 64 x 64 blocks I >= J)
 __global__ void __launch_bounds__(256) b_hs_finish(const BatchSys *__restrict__ sys, int nsys,
@@ -318,9 +341,10 @@ void launch_pair(int grid, hipStream_t st, const BatchSys *sys, const Item *item
 void launch_pair2(int grid, hipStream_t st, const BatchSys *sys, const Item *items, int nitems,
                   const int32_t *active, const double *bra, const double *ket,
                   const unsigned *tperm, double *part, double *kpart, double *hpart,
-                  double *spart, int *flag) {
+                  double *spart, int *flag, int *ctr) {
+  if (ctr) hipLaunchKernelGGL(b_clear_counter, dim3(1), dim3(1), 0, st, ctr);
   hipLaunchKernelGGL(b_pair2_all, dim3(grid), dim3(256), 0, st, sys, items, nitems, active, bra,
-                     ket, tperm, part, kpart, hpart, spart, flag);
+                     ket, tperm, part, kpart, hpart, spart, flag, ctr);
 }
 
 int hip_rc(hipError_t e) {
@@ -379,7 +403,7 @@ struct necg_batch {
          *spart = nullptr, *part = nullptr, *red = nullptr, *HC = nullptr, *SC = nullptr,
          *hcp = nullptr, *scp = nullptr, *scal = nullptr, *kpart = nullptr;
   unsigned *tperm = nullptr;  // target-slot tables of the two-sided systems
-  int *flag = nullptr;
+  int *flag = nullptr;   // [0]: latched status; [1]: the pair kernel's work counter
   // pair work lists per (n, form)
   struct Cls {
     int n = 0;
@@ -664,7 +688,7 @@ int necg_batch_create(const necg_system *systems, int32_t nsys, const int32_t *K
       (rc = dalloc(&B->HC, (size_t)vo)) || (rc = dalloc(&B->SC, (size_t)vo)) ||
       (rc = dalloc(&B->hcp, (size_t)hco)) || (rc = dalloc(&B->scp, (size_t)hco)) ||
       (rc = dalloc(&B->scal, (size_t)8 * nsys)) || (rc = dalloc(&B->d_sys, (size_t)nsys)) ||
-      (rc = dalloc(&B->flag, 1)) || (rc = upload_prefix(&B->d_fpre, B->fpre)) ||
+      (rc = dalloc(&B->flag, 2)) || (rc = upload_prefix(&B->d_fpre, B->fpre)) ||
       (rc = upload_prefix(&B->d_fwpre, B->fwpre)) ||
       (rc = upload_prefix(&B->d_kwpre, B->kwpre)) || (rc = upload_prefix(&B->d_hpre, B->hpre)) ||
       (rc = upload_prefix(&B->d_r2pre, B->r2pre))) {
@@ -675,7 +699,7 @@ int necg_batch_create(const necg_system *systems, int32_t nsys, const int32_t *K
       (rc = hip_rc(hipMemcpy(B->sms, h_sms.data(), h_sms.size() * 8, hipMemcpyHostToDevice))) ||
       (rc = hip_rc(hipMemcpy(B->scvv, h_scvv.data(), h_scvv.size() * 8,
                              hipMemcpyHostToDevice))) ||
-      (rc = hip_rc(hipMemset(B->flag, 0, sizeof(int))))) {
+      (rc = hip_rc(hipMemset(B->flag, 0, 2 * sizeof(int))))) {
     necg_batch_destroy(B);
     return rc;
   }
@@ -739,8 +763,11 @@ int necg_batch_outprod_masked(necg_batch *B, const int32_t *active, const double
                      B->d_kwpre, active, B->sml, B->sms, B->scvv, B->bra, B->ket);
   for (const auto &c : B->cls) {
     if (c.two) {
+      // the work counter serves whole-batch evaluations only: masked ones may
+      // run concurrently on other streams and keep to the fixed stride
       launch_pair2(c.grid, st, B->d_sys, c.d_items, c.nitems, active, B->bra, B->ket, B->tperm,
-                   B->part, B->kpart, B->hpart, B->spart, B->flag);
+                   B->part, B->kpart, B->hpart, B->spart, B->flag,
+                   active ? nullptr : B->flag + 1);
     } else {
       SWITCH_N(c.n, launch_pair<NN>(c.grid, st, B->d_sys, c.d_items, c.nitems, active, B->bra,
                                     B->ket, B->part, B->hpart, B->spart, B->flag));
