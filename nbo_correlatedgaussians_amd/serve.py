@@ -11,6 +11,11 @@ request kinds arrive over time:
   modified.
 * ``full``: energy and analytic gradient of system s at its resident x
   (``outprod``, python/outprod.py:20).
+* ``accept``: make a replacement resident: function k of system s becomes the
+  new (L_k, s_k) in the resident x, the system is evaluated again (energy and
+  gradient at the new x, returned like a full request, with H, S, H C, S C
+  and the quotient refreshed) and the factors of its pencil are formed again,
+  so that later trials on the system start from the new basis.
 
 The scheduler does continuous batching: on every pass it collects the
 requests that have arrived, admits as many trials as the free memory budget
@@ -18,9 +23,10 @@ allows into one batched trial launch on a free trial stream, and admits the
 waiting full evaluations (one per system) into one masked batch evaluation on
 a free evaluation stream; launches on different streams overlap on the device
 and complete independently, and a new pass admits whatever arrived meanwhile.
-A system with a full evaluation in flight takes no new work until it ends, a
-full evaluation waits for its system's trials in flight, and a trial that
-arrived after a waiting full evaluation of its system waits behind it.
+A system with a full evaluation or an accept in flight takes no new work
+until it ends, a full evaluation or an accept waits for its system's trials
+in flight, and a trial that arrived after a waiting full evaluation or accept
+of its system waits behind it.
 Latency is measured from a request's arrival to its results being on the
 host.
 """
@@ -48,11 +54,11 @@ def _torch():
 @dataclass
 class Request:
     rid: int
-    kind: str                 # "trial" or "full"
+    kind: str                 # "trial", "full" or "accept"
     sys: int
     arrival: float            # seconds from the start of the run
-    func: int = -1            # trial: replaced function (0-based)
-    newp: np.ndarray = None   # trial: new [vech L | s] block
+    func: int = -1            # trial, accept: replaced function (0-based)
+    newp: np.ndarray = None   # trial, accept: new [vech L | s] block
     done: float = None
     E: float = None
     Eeig: float = None        # trial: lowest eigenvalue of (H', S')
@@ -92,7 +98,7 @@ class Server:
         from .batch import Batch, autotune
         self.device = torch.device(device if device is not None else "cuda:0")
         self.systems = list(systems)
-        self.xs = [np.asarray(x, dtype=np.float64) for x in xs]
+        self.xs = [np.array(x, dtype=np.float64) for x in xs]   # own copies: accepts change them
         Ks = []
         for s, x in zip(self.systems, self.xs):
             if x.size % (s.blk + 1):
@@ -154,23 +160,32 @@ class Server:
     def _make_factors(self):
         """V, lam of every system's pencil (H, S) from the resident matrices."""
         torch = _torch()
-        from .driver import generalized_eigh
         self.factor_bytes = 0
         for s, K in enumerate(self.Ks):
-            H = torch.empty(K, K, dtype=torch.float64, device=self.device)
-            S = torch.empty(K, K, dtype=torch.float64, device=self.device)
-            st = ct.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
-            _lib.raise_for(self.lib.necg_batch_matrices(self.batch._h, s, ct.c_void_p(H.data_ptr()),
-                                                        ct.c_void_p(S.data_ptr()), st),
-                           "necg_batch_matrices")
-            lam, V = generalized_eigh(H, S)
-            del H, S
-            self._fac.append((V, lam))
-            _lib.raise_for(self.lib.necg_pool_set_factors(self._pool, s, ct.c_void_p(V.data_ptr()),
-                                                          ct.c_void_p(lam.data_ptr())),
-                           "necg_pool_set_factors")
+            self._fac.append(None)
+            self._factor(s)
             self.factor_bytes += 8 * (K * K + K)
         torch.cuda.synchronize(self.device)
+
+    def _factor(self, s: int):
+        """(Re)forms the factors of system s from its resident H and S, on the
+        current stream, and hands them to the pool (which synchronises)."""
+        torch = _torch()
+        from .driver import generalized_eigh
+        K = self.Ks[s]
+        H = torch.empty(K, K, dtype=torch.float64, device=self.device)
+        S = torch.empty(K, K, dtype=torch.float64, device=self.device)
+        st = ct.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+        _lib.raise_for(self.lib.necg_batch_matrices(self.batch._h, s, ct.c_void_p(H.data_ptr()),
+                                                    ct.c_void_p(S.data_ptr()), st),
+                       "necg_batch_matrices")
+        lam, V = generalized_eigh(H, S)
+        del H, S
+        torch.cuda.current_stream(self.device).synchronize()
+        _lib.raise_for(self.lib.necg_pool_set_factors(self._pool, s, ct.c_void_p(V.data_ptr()),
+                                                      ct.c_void_p(lam.data_ptr())),
+                       "necg_pool_set_factors")
+        self._fac[s] = (V, lam)     # the previous factors are released only now
 
     @staticmethod
     def _eval_bytes(system, K):
@@ -232,7 +247,8 @@ class Server:
         return _Launch("trial", slot, ev, reqs, self.trial_bytes * n, {r.sys for r in reqs})
 
     def _launch_full(self, slot, reqs):
-        """One masked batch evaluation of the requests' systems (distinct)."""
+        """One masked batch evaluation of the requests' systems (distinct);
+        an accept request first writes its block into the resident x."""
         torch = _torch()
         b = self._ebuf[slot]
         ha = b["h_active"].numpy()
@@ -240,6 +256,13 @@ class Server:
         for r in reqs:
             ha[r.sys] = 1
         with torch.cuda.stream(b["stream"]):
+            for r in reqs:
+                if r.kind == "accept":
+                    blk = self.systems[r.sys].blk
+                    lo = int(self.offsets[r.sys]) + r.func * blk
+                    newp = torch.as_tensor(np.asarray(r.newp, dtype=np.float64))
+                    self.xb[lo:lo + blk].copy_(newp.pin_memory(), non_blocking=True)
+                    self.xs[r.sys][r.func * blk:(r.func + 1) * blk] = r.newp
             b["active"].copy_(b["h_active"], non_blocking=True)
             self.batch.outprod_t(self.xb, b["f"], b["g"], active=b["active"])
             b["h_f"].copy_(b["f"], non_blocking=True)
@@ -255,6 +278,7 @@ class Server:
     def run(self, requests, on_pass=None):
         """Serve the requests (sorted by arrival, times in seconds from now);
         fills in done/E/g of each and returns them."""
+        torch = _torch()
         reqs = sorted(requests, key=lambda r: r.arrival)
         for r in reqs:
             self._validate(r)
@@ -300,7 +324,10 @@ class Server:
                     for r in L.reqs:
                         r.E = float(fh[r.sys])
                         r.g = gh[self.offsets[r.sys]:self.offsets[r.sys + 1]].copy()
-                        r.done = tdone
+                        if r.kind == "accept" and self.factors:
+                            with torch.cuda.stream(b["stream"]):
+                                self._factor(r.sys)     # the new pencil's factors
+                        r.done = time.perf_counter() - t0 if r.kind == "accept" else tdone
                         full_busy.discard(r.sys)
                     free_e.append(L.slot)
                 used -= L.nbytes
@@ -363,11 +390,11 @@ class Server:
         return reqs
 
     def _validate(self, r: Request):
-        if r.kind not in ("trial", "full"):
+        if r.kind not in ("trial", "full", "accept"):
             raise ValueError(f"request {r.rid}: unknown kind {r.kind!r}")
         if not 0 <= r.sys < len(self.systems):
             raise ValueError(f"request {r.rid}: no system {r.sys}")
-        if r.kind == "trial":
+        if r.kind in ("trial", "accept"):
             if not 0 <= r.func < self.Ks[r.sys]:
                 raise ValueError(f"request {r.rid}: system {r.sys} has no function {r.func}")
             blk = self.systems[r.sys].blk

@@ -173,3 +173,42 @@ def test_served_config_at_full_size(oracle_lib):
             assert np.max(np.abs(rh - Ho[r.func])) <= 1e-12 * np.max(np.abs(Ho)), r.rid
             assert np.max(np.abs(rs - So[r.func])) <= 1e-12 * np.max(np.abs(So)), r.rid
         print(f"  trial {r.rid}: system {r.sys} K={K} function {r.func} ok", flush=True)
+
+
+def test_accept_makes_replacement_resident():
+    """An accept request puts its block into the resident x: its own result
+    is outprod at the new x; a trial that arrives after it is computed on the
+    new basis (row, E and, from the re-formed factors, the lowest eigenvalue
+    against scipy.linalg.eigh); a later full request returns the accept's
+    (E, g) bitwise; a trial that arrived before it saw the old basis."""
+    from scipy.linalg import eigh
+    from nbo_correlatedgaussians_amd.engine import Engine
+    from nbo_correlatedgaussians_amd.serve import Request, Server
+    sx = _systems()
+    systems, xs = [s for s, _ in sx], [x.copy() for _, x in sx]
+    srv = Server(systems, xs, "cuda:0")
+    s = systems[0]
+    x0 = xs[0].copy()
+    p5, _ = necg.generate_parameters(s, 1, 71)
+    p7, _ = necg.generate_parameters(s, 1, 72)
+    E_old, _, _ = srv.trial_alone(Request(0, "trial", 0, 0.0, 5, p5))
+    reqs = [Request(0, "trial", 0, 0.0, 5, p5), Request(1, "accept", 0, 0.0005, 5, p5),
+            Request(2, "trial", 0, 0.001, 7, p7), Request(3, "full", 0, 0.0015),
+            Request(4, "full", 1, 0.0)]
+    out = {r.rid: r for r in srv.run(reqs)}
+    assert all(r.done is not None for r in out.values())
+    x1 = x0.copy()
+    x1[5 * s.blk:6 * s.blk] = p5
+    eng = Engine(s, "cuda:0")
+    Ea, ga = eng.outprod(x1)
+    assert out[0].E == E_old                                   # the trial before: old basis
+    assert abs(out[1].E - Ea) <= 1e-12 * max(1.0, abs(Ea))
+    assert np.max(np.abs(out[1].g - ga)) <= 1e-9 * max(1.0, np.max(np.abs(ga)))
+    assert out[3].E == out[1].E and np.array_equal(out[3].g, out[1].g)
+    x2 = x1.copy()
+    x2[7 * s.blk:8 * s.blk] = p7
+    E2, H2, S2 = eng.energy(x2, with_hs=True)
+    assert abs(out[2].E - E2) <= 1e-11 * max(1.0, abs(E2))
+    w0 = eigh(H2, S2, eigvals_only=True, subset_by_index=[0, 0])[0]
+    assert abs(out[2].Eeig - w0) <= 1e-9 * max(1.0, abs(w0))
+    assert np.array_equal(srv.xs[0], x1)
