@@ -336,9 +336,10 @@ def fp64_fma_rate(iters=200000):
     return r.value
 
 
-def traffic_probe(K, seed):
+def traffic_probe(K, seed, pair_mode=0):
     """For the rocprofv3 counter passes: a 256 MiB device copy (the
-    calibration of FETCH_SIZE / WRITE_SIZE) followed by two outprod calls."""
+    calibration of FETCH_SIZE / WRITE_SIZE) followed by two outprod calls
+    (pair_mode 2: with the per-ordering pair pass)."""
     import torch
     import nbo_correlatedgaussians_amd as necg
     from nbo_correlatedgaussians_amd.engine import Engine
@@ -351,11 +352,30 @@ def traffic_probe(K, seed):
     iv, C = necg.generate_parameters(s, K, seed)
     eng = Engine(s, "cuda:0")
     xd = torch.as_tensor(necg.pack_x(iv, C), device="cuda:0")
+    eng.set_pair_mode(pair_mode)
     for _ in range(2):
         eng.outprod_t(xd)
     torch.cuda.synchronize()
     eng.check("traffic probe")
     print("traffic probe done", flush=True)
+
+
+def batch_probe():
+    """For rocprofv3 passes over the batched pair kernel: the 64-system mix of
+    the batched leg, autotuned, then two evaluations with the chosen tiling."""
+    import torch
+    from nbo_correlatedgaussians_amd.batch import Batch, autotune, mixed_systems
+    systems, Ks, xs = mixed_systems()
+    b = Batch(systems, Ks, "cuda:0")
+    xa = b.pack(xs)
+    best, _ = autotune(b, xa, reps=1)
+    xd = torch.as_tensor(xa, device="cuda:0")
+    torch.cuda.synchronize()
+    for _ in range(2):
+        b.outprod_t(xd)
+    torch.cuda.synchronize()
+    b.check("batch probe")
+    print(f"batch probe done (item_flops {best[0]:g})", flush=True)
 
 
 def _recorded(fname, key):
@@ -399,9 +419,15 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=10000)
     ap.add_argument("--serve-rate", type=float, default=1000.0)
     ap.add_argument("--traffic-probe", action="store_true")
+    ap.add_argument("--pair-mode", type=int, default=0,
+                    help="with --traffic-probe: 0 the library's choice, 2 the per-ordering pass")
+    ap.add_argument("--batch-probe", action="store_true")
     args = ap.parse_args()
     if args.traffic_probe:
-        traffic_probe(args.K, args.seed)
+        traffic_probe(args.K, args.seed, args.pair_mode)
+        return
+    if args.batch_probe:
+        batch_probe()
         return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
