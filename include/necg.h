@@ -69,11 +69,13 @@ int necg_outprod(necg_ctx *ctx, int32_t K, const double *x, double *f, double *g
 /* necg_outprod without any device allocation.  No counterpart in the
  * reference: it is for callers that cannot allocate device memory (code
  * under HIP-graph capture, tight serving loops, harnesses that forbid
- * hipMalloc).  Same kernels, same results (bitwise) as necg_outprod at the
- * same K and launch shape; every workspace buffer (bra/ket tables, H, S,
- * partials, tile list) is carved out of one static 64 MiB device array per
- * GPU, NECG_ERR_SCRATCH if the evaluation needs more (LiH, n = 5 with 24
- * terms: K = 500 fits, K = 600 does not).  The context's own workspace, resident
+ * hipMalloc).  Same kernels as necg_outprod; every workspace buffer (bra/ket
+ * tables, H, S, partials, tile list) is carved out of one static 64 MiB
+ * device array per GPU.  Where necg_outprod's tiling fits the array the
+ * results are bitwise the same; otherwise the call takes the fastest tiling
+ * whose partials fit (the same sums in other groups: equal to rounding), and
+ * returns NECG_ERR_SCRATCH if none does (LiH, n = 5 with 24 terms: K = 900
+ * fits, K = 1200 does not).  The context's own workspace, resident
  * state and pins are untouched (its tape is dropped, as by any evaluation).
  * All calls on one GPU share that array, which keeps the last call's
  * contents: calls on one stream are ordered, but calls that overlap on one

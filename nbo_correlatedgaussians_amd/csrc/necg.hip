@@ -566,79 +566,52 @@ int finish_grad(necg_ctx *c, int K, int ntiles, const double *gbar, double *g, h
 
 // ----------------------------------------------------- lower triangle ----
 
-// Ket tile and term groups of the lower-triangle passes.  One workgroup per
-// CU at a time, its waves taking alternate kets; for each candidate ket tile
-// the term groups ts are doubled while there are fewer than two workgroups
-// per CU, and the launch is priced in rounds of ncu workgroups, each
-// workgroup (kets per wave) x (terms per group / nst) ket sweeps plus a
-// per-workgroup overhead of 0.07 sweep; a last partial round costs a whole
-// workgroup, or half of one when it is split (tri_split_count, ts = 1).  The
-// overhead is fitted on LiH K = 2048: ket tile 8 beats 4 (33 rounds of 1 ket
-// against 17 of 2: more than 0.06), and 8 beats 16 by 5 % unsplit (17 rounds
-// of 2 kets against 9 of 4) and by 2.3 % split (16 + a half round against 8
-// + a half).  With folded diagonal blocks (tri_fold) the fold tiles, 1.12
-// ordinary ones, start first, so one of them lies on the critical path; the
-// diagonal tiles (about a third of a workgroup each) take no slot of a round.
-double tiling2_cost(int K, int q, int fold, int ncu, int nst, int w, int req_ts, int &ts_out) {
-  long long n, nfull;
-  tri_counts(K, q, fold, n, nfull);
-  const long long nd = tri_fold_tiles(K, q, fold), nf = nd * (32 / q), nr = n - nd;
-  int ts = 1;
-  if (req_ts > 0) {
-    ts = std::min(req_ts, nst);
-  } else {
-    while (n * ts < 2LL * ncu && ts < nst) ts *= 2;
-    ts = std::min(ts, nst);
+// Accumulator slots per function of the two-sided pass for n particles.
+int nacc_of(int n) {
+  switch (n) {
+    case 1: return Dim<1>::NACC;
+    case 2: return Dim<2>::NACC;
+    case 3: return Dim<3>::NACC;
+    case 4: return Dim<4>::NACC;
+    case 5: return Dim<5>::NACC;
+    default: return Dim<6>::NACC;
   }
-  ts_out = ts;
-  const int tps = (nst + ts - 1) / ts;
-  const double frac = (double)tps / nst;
-  const double kw = (double)((q + w - 1) / w) * frac;
-  const double tt = kw + 0.07;
-  const long long wg = nr * ts;
-  double cost = (double)(wg / ncu) * tt;
-  if (wg % ncu) {
-    const bool split = ts == 1 && tri_split_count(nr, nfull, q, ncu);
-    cost += split ? (double)((q / 2 + w - 1) / w) * frac + 0.07 : tt;
-  }
-  if (nf > 0) cost += 0.12 * kw;
-  return cost + (double)(nd * ts) * (tt / 3.0) / ncu;
 }
 
+// Ket tile and term groups of the lower-triangle passes (tri_choose: the
+// tiling with the shortest modelled launch).  In an allocation-free call the
+// two-sided pass's partials must fit what the arena has left after the
+// per-K buffers (the same tiling as an ordinary call whenever that fits).
+// The choice is kept for the next evaluation with the same inputs (walking
+// the candidates' tile lists costs about a millisecond of host time at
+// K = 2048).
 void choose_tiling2(necg_ctx *c, int K, int &kt, int &ts) {
   const int ncu = std::max(1, c->ncu);
-  const int w = pair_waves(c->n);
-  // the choice walks every candidate's tile list (about 0.3 ms of host time
-  // at K = 2048): kept for the next evaluation with the same inputs
+  size_t budget = 0;
+  if (c->arena) {   // what is left after the per-K buffers, less the tile list
+    if (c->arena_budget == 0) {
+      const size_t left = c->arena_cap > c->arena_used ? c->arena_cap - c->arena_used : 0;
+      const size_t list = (size_t)16 * ((K + 63) / 64) * ((K + 63) / 64 + 1) * 32 + 4096;
+      c->arena_budget = left > list ? left - list : 1;
+    }
+    budget = c->arena_budget;
+  }
   if (c->tc_K == K && c->tc_req_kt == c->req_kt && c->tc_req_ts == c->req_ts &&
-      c->tc_ncu == ncu && c->tc_fold == c->fold_mode) {
+      c->tc_ncu == ncu && c->tc_fold == c->fold_mode && c->tc_budget == budget) {
     kt = c->tc_kt;
     ts = c->tc_ts;
     return;
   }
-  if (c->req_kt > 0) {
-    kt = c->req_kt;
-    (void)tiling2_cost(K, kt, c->fold_mode, ncu, c->nst, w, c->req_ts, ts);
-  } else {
-    static const int cand[] = {2, 4, 8, 16, 32, 64};   // divisors of 64 (upper pairs)
-    double best = -1.0;
-    kt = 8;
-    ts = 1;
-    for (int q : cand) {
-      int tq;
-      const double cost = tiling2_cost(K, q, c->fold_mode, ncu, c->nst, w, c->req_ts, tq);
-      if (best < 0.0 || cost < best) {
-        best = cost;
-        kt = q;
-        ts = tq;
-      }
-    }
-  }
+  const TriChoice ch = tri_choose(K, c->nst, c->n, pair_waves(c->n), c->fold_mode, ncu,
+                                  c->req_kt, c->req_ts, nacc_of(c->n), budget);
+  kt = ch.kt;
+  ts = ch.ts;
   c->tc_K = K;
   c->tc_req_kt = c->req_kt;
   c->tc_req_ts = c->req_ts;
   c->tc_ncu = ncu;
   c->tc_fold = c->fold_mode;
+  c->tc_budget = budget;
   c->tc_kt = kt;
   c->tc_ts = ts;
 }
@@ -1033,6 +1006,7 @@ struct ArenaScope {
     c->arena = base;
     c->arena_cap = cap;
     c->arena_used = 0;
+    c->arena_budget = 0;
   }
   ~ArenaScope() {
     c->Kcap = Kcap; c->tiles_K = tiles_K; c->tiles_kt = tiles_kt; c->tiles_fold = tiles_fold;
@@ -1046,7 +1020,7 @@ struct ArenaScope {
     c->tiles_cap = tiles_cap; c->d_tiles = d_tiles; c->d_xr = d_xr;
     c->gen = gen; c->full_gen = full_gen; c->timing = timing;
     c->arena = nullptr;
-    c->arena_cap = c->arena_used = 0;
+    c->arena_cap = c->arena_used = c->arena_budget = 0;
   }
 };
 

@@ -57,6 +57,7 @@ struct necg_ctx {
   // the lower-triangle passes' last tiling choice (choose_tiling2): kt, ts
   // for K = tc_K, requested (tc_req_kt, tc_req_ts) and tc_ncu CUs
   int tc_K = 0, tc_req_kt = -1, tc_req_ts = -1, tc_ncu = 0, tc_kt = 0, tc_ts = 0, tc_fold = -1;
+  size_t tc_budget = 0;   // the arena bytes the choice was made for (0: no arena)
   // per-(bra tile, term group) ket-side accumulator records, [t][l][slot]
   double *kpart = nullptr;
   size_t kpart_cap = 0;
@@ -81,6 +82,9 @@ struct necg_ctx {
   // CU count) for the life of the context
   char *arena = nullptr;
   size_t arena_cap = 0, arena_used = 0;
+  // bytes the call's tiling choice may give the pair partials, fixed at the
+  // call's first choice (later choices in the same call see the same budget)
+  size_t arena_budget = 0;
   std::map<long long, std::vector<int>> arena_lists;
   // pair-kernel timing (necg_timing)
   bool timing = false;

@@ -4,6 +4,8 @@
 #pragma once
 
 #include <algorithm>
+#include <functional>
+#include <queue>
 #include <vector>
 
 #include "../../include/necg.h"
@@ -235,6 +237,119 @@ inline int tri_tile_list(int K, int kt, int ncu, int fold, std::vector<int> &lis
     }
   }
   return r;
+}
+
+// Cost of one workgroup of a lower-triangle launch, in term evaluations per
+// lane, for tile t (tri_tile) with tp terms and w waves (waves take
+// alternate kets or sweeps): a regular tile of nk kets ceil(nk / w) (tp +
+// 0.06) + ovh, a fold tile of kt sweeps ceil(kt / w) (1.14 tp + 0.12) + ovh
+// (two kets' wave sums per sweep), a diagonal tile 0.2 tp + ovh.  ovh is the
+// tile start and end (tri_overhead); 0.06 is a ket's record flush.
+inline double tri_tile_cost(const TriTile &t, int kt, int tp, int w, double ovh) {
+  if (t.fold == 2) return 0.2 * tp + ovh;
+  if (t.fold == 1) return (double)((kt + w - 1) / w) * (1.14 * tp + 0.12) + ovh;
+  return (double)((t.l1 - t.l0 + w - 1) / w) * (tp + 0.06) + ovh;
+}
+
+// Per-workgroup overhead of tri_tile_cost for n pseudo-particles, in term
+// evaluations.  Fitted with the constants of tri_tile_cost to the measured
+// two-sided pair pass over forced tilings (ket tile 4..64, term groups
+// 1..16): LiH (n = 5, 24 terms) at K = 256..4096, H2 (n = 3) and He (n = 2,
+// 2 terms each) at K = 300..4096.  With these values the choice of
+// tri_choose is the fastest measured tiling or within 0.3 % of it at every
+// measured size.
+inline double tri_overhead(int n) { return n == 3 ? 1.25 : 0.75; }
+
+// Makespan of a lower-triangle launch of (K, kt) with ts term groups of tps
+// terms (grid x = the tri_tile_list tiles, y = term groups, x fastest), on
+// ncu CUs that each run one workgroup at a time and take the next workgroup
+// in launch order when one finishes (list scheduling): the end of the last
+// workgroup, in the units of tri_tile_cost.  nsplit = the list's split
+// tiles (tri_tile_list).
+inline double tri_makespan(int K, int kt, int ts, int tps, int fold, int ncu, int nst, int w,
+                           double ovh, int &nsplit) {
+  std::vector<int> list, xr;
+  nsplit = tri_tile_list(K, kt, ncu, fold, list, xr);
+  const int n = (int)(list.size() / 2);
+  std::vector<TriTile> tiles((size_t)n);
+  for (int i = 0; i < n; ++i) tiles[(size_t)i] = tri_tile(list.data(), i, K, kt);
+  std::priority_queue<double, std::vector<double>, std::greater<double>> free_at;
+  for (int i = 0; i < std::max(1, ncu); ++i) free_at.push(0.0);
+  double end = 0.0;
+  for (int z = 0; z < ts; ++z) {
+    const int tp = std::min(nst, (z + 1) * tps) - z * tps;
+    for (const TriTile &t : tiles) {
+      const double e = free_at.top() + tri_tile_cost(t, kt, tp, w, ovh);
+      free_at.pop();
+      free_at.push(e);
+      end = std::max(end, e);
+    }
+  }
+  return end;
+}
+
+// Device bytes of the two-sided gradient pass's partials for (K, kt, ts):
+// bra partials of every ket tile and split half, ket partials of every bra
+// tile, the element partials of H and S (nacc accumulator slots per
+// function).
+inline size_t tri_grad_bytes(int K, int kt, int ts, int nsplit, int nacc) {
+  const size_t nkt = (size_t)((K + kt - 1) / kt), nbt = (size_t)((K + 63) / 64);
+  return 8 * ((nkt + (size_t)nsplit) * ts * nacc * (size_t)K + nbt * ts * (size_t)K * nacc +
+              2 * (size_t)ts * K * K);
+}
+
+// Ket tile and term groups of a lower-triangle pass: of the ket tiles
+// 2..64 (divisors of 64, for the upper pairs) and the term groups ts = 1, 2,
+// 4, ... (as many as give distinct terms per group, while the launch has at
+// most 8 workgroups per CU), the pair with the shortest tri_makespan.
+// req_kt / req_ts > 0 fix one or both.  budget > 0: tilings whose partials
+// (tri_grad_bytes) exceed it are passed over unless none fits, and then the
+// smallest is taken.  Ket tiles whose list would exceed 2^18 tiles are passed
+// over when a larger one is a candidate (the list is walked on the host), and
+// so are ket tiles smaller than the wave count (idle waves) once the list has
+// more than 16 rounds.
+struct TriChoice {
+  int kt = 8, ts = 1;
+  double cost = -1.0;
+};
+
+inline TriChoice tri_choose(int K, int nst, int n, int w, int fold_mode, int ncu, int req_kt,
+                            int req_ts, int nacc, size_t budget) {
+  std::vector<int> kts, tss;
+  if (req_kt > 0) kts.push_back(req_kt);
+  else kts = {2, 4, 8, 16, 32, 64};
+  if (req_ts > 0) tss.push_back(std::min(req_ts, nst));
+  else for (int t = 1; t <= nst; t *= 2) tss.push_back(t);
+  ncu = std::max(1, ncu);
+  const double ovh = tri_overhead(n);
+  const long long nbt = (K + 63) / 64;
+  TriChoice best, small;
+  size_t small_bytes = 0;
+  for (int q : kts) {
+    const long long approx = nbt * (nbt + 1) / 2 * ((64 + q - 1) / q);
+    if (req_kt <= 0 && q < 64 && approx > (1LL << 18)) continue;
+    if (req_kt <= 0 && q < w && approx > 16LL * ncu) continue;   // idle waves, many rounds
+    const int fold = tri_fold(q, fold_mode) ? 1 : 0;
+    long long nt, nfull;
+    tri_counts(K, q, fold, nt, nfull);
+    int last_tps = -1;
+    for (int tq : tss) {
+      const int tps = (nst + tq - 1) / tq, ts = (nst + tps - 1) / tps;
+      if (tps == last_tps) continue;
+      last_tps = tps;
+      if (ts > 1 && req_ts <= 0 && nt * ts > 8LL * ncu) break;
+      int nsplit = 0;
+      const double c = tri_makespan(K, q, ts, tps, fold, ncu, nst, w, ovh, nsplit);
+      const size_t bytes = tri_grad_bytes(K, q, ts, nsplit, nacc);
+      if (small.cost < 0.0 || bytes < small_bytes) {
+        small = {q, ts, c};
+        small_bytes = bytes;
+      }
+      if (budget > 0 && bytes > budget) continue;
+      if (best.cost < 0.0 || c < best.cost) best = {q, ts, c};
+    }
+  }
+  return best.cost >= 0.0 ? best : small;
 }
 
 }  // namespace necg

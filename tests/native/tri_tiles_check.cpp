@@ -186,6 +186,45 @@ static int check_list(int K, int kt, int ncu, int fold, int &nsplit_lists) {
   return 0;
 }
 
+// The tiling model (tri_choose) against the measured two-sided pair pass:
+// at each size below, a sweep over forced tilings (ket tile 4..64, term
+// groups 1..16, pair pass time over 5-10 launches) found the expected
+// tiling fastest, or within 0.2 % of the fastest (LiH K = 700: 16 x 4 took
+// 0.7528 ms, 8 x 2 0.7536; K = 4096: 32 x 1 22.668 ms, 16 x 1 22.719).  With a
+// byte budget below every tiling's partials the smallest is taken.
+int check_choose(bool quick) {
+  struct Want { int n, nst, K, kt, ts; };
+  const Want want[] = {
+      {5, 24, 256, 8, 4},  {5, 24, 400, 8, 8},   {5, 24, 700, 8, 2},  {5, 24, 1300, 8, 1},
+      {5, 24, 1600, 32, 2}, {5, 24, 3000, 8, 1}, {5, 24, 4096, 16, 1},
+      {2, 2, 300, 4, 1},   {2, 2, 1024, 16, 1},  {2, 2, 2048, 32, 1}, {2, 2, 4096, 32, 1},
+      {3, 2, 300, 4, 1},   {3, 2, 1024, 32, 1},  {3, 2, 2048, 32, 1}, {3, 2, 4096, 32, 1}};
+  int bad = 0;
+  for (const Want &w : want) {
+    if (quick && w.K > 2048) continue;
+    const TriChoice c = tri_choose(w.K, w.nst, w.n, 4, 1, 256, 0, 0, 62, 0);
+    if (c.kt != w.kt || c.ts != w.ts) {
+      std::printf("tri_choose n=%d K=%d: %d x %d, measured fastest %d x %d\n", w.n, w.K, c.kt,
+                  c.ts, w.kt, w.ts);
+      ++bad;
+    }
+  }
+  // a budget: the pick's partials fit it; below every tiling, the smallest
+  const TriChoice a = tri_choose(400, 24, 5, 4, 1, 256, 0, 0, 62, (size_t)24 << 20);
+  std::vector<int> l, x;
+  const int na = tri_tile_list(400, a.kt, 256, tri_fold(a.kt, 1) ? 1 : 0, l, x);
+  if (tri_grad_bytes(400, a.kt, a.ts, na, 62) > ((size_t)24 << 20)) {
+    std::printf("tri_choose K=400 with a 24 MiB budget: %d x %d does not fit\n", a.kt, a.ts);
+    ++bad;
+  }
+  const TriChoice b = tri_choose(400, 24, 5, 4, 1, 256, 0, 0, 62, 1);
+  if (b.ts != 1) {
+    std::printf("tri_choose K=400 with no room: %d x %d, not the smallest\n", b.kt, b.ts);
+    ++bad;
+  }
+  return bad;
+}
+
 int main(int argc, char **argv) {
   // "quick": a smaller sweep (for builds with sanitizers, several times slower)
   const bool quick = argc > 1 && argv[1][0] == 'q';
@@ -222,6 +261,7 @@ int main(int argc, char **argv) {
       ++bad;
     }
   }
+  bad += check_choose(quick);
   if (nsplit_lists < (quick ? 20 : 50)) {
     std::printf("only %d lists with split tiles\n", nsplit_lists);
     ++bad;

@@ -1,8 +1,10 @@
 """necg_outprod_noalloc through the C ABI: the allocation-free variant of
 necg_outprod (workspace carved from the library's static device scratch)
 gives the same E and g, bitwise, as necg_outprod on the same context, on two
-calls in a row with the second one smaller; an evaluation larger than the
-scratch is refused with NECG_ERR_SCRATCH and leaves the context working."""
+calls in a row with the second one smaller (sizes whose ordinary tiling fits
+the scratch); an evaluation larger than the scratch is refused with
+NECG_ERR_SCRATCH and leaves the context working, and sizes in between take a
+tiling that fits and agree with necg_outprod to rounding."""
 
 import ctypes as ct
 
@@ -75,12 +77,22 @@ def test_noalloc_refuses_what_does_not_fit():
     Er, gr = eng.outprod(small.cpu().numpy())
     assert f.item() == Er and np.array_equal(g.cpu().numpy(), gr)
     fits = []
-    for K in (300, 400, 500, 600, 700, 800):    # where the 64 MiB scratch ends for LiH
+    tried = (300, 400, 600, 900, 1200)
+    for K in tried:    # where the 64 MiB scratch ends for LiH
         x = torch.as_tensor(necg.pack_x(*necg.generate_parameters(s, K, 5)), device="cuda:0")
+        print(f"LiH K={K}: fits or refused", flush=True)
         rc, f, g = _call(lib, "necg_outprod_noalloc", eng, x)
         assert rc in (0, _lib.NECG_ERR_SCRATCH)
+        eng.check()
         if rc == 0:
             fits.append(K)
-    eng.check()
+            # where the ordinary call's tiling does not fit, the allocation-free
+            # one takes a smaller one (other term groups or ket tiles): the same
+            # sums in another grouping
+            Er, gr = eng.outprod(x.cpu().numpy())
+            assert abs(f.item() - Er) <= 1e-13 * abs(Er), (K, f.item(), Er)
+            np.testing.assert_allclose(g.cpu().numpy(), gr, rtol=1e-9,
+                                       atol=1e-11 * np.max(np.abs(gr)))
     print(f"LiH K that fit the scratch: {fits}", flush=True)
-    assert 300 in fits and 800 not in fits and fits == sorted(fits)
+    assert fits == list(tried[:len(fits)])          # a prefix of the sizes tried
+    assert 900 in fits and 1200 not in fits
