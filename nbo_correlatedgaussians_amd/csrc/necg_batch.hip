@@ -442,9 +442,11 @@ int plan(necg_batch *B) {
     const double tc = b.two ? term_cost2(b.n) : term_cost(b.n);
     const double per_ket = nb * nst * tc;
     int kt = (int)std::max(1.0, std::floor(target / per_ket));
-    if (b.two) {  // a divisor of 64 (upper pairs), at least one ket per wave
+    // two-sided: a divisor of 32 (for the upper pairs and the folded
+    // diagonal blocks), at least one ket per wave
+    if (b.two) {
       int q = w;
-      while (q * 2 <= std::min(kt, 64)) q *= 2;
+      while (q * 2 <= std::min(kt, 32)) q *= 2;
       kt = q;
     }
     kt = std::min(kt, K);
