@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <functional>
 #include <queue>
+#include <utility>
 #include <vector>
 
 #include "../../include/necg.h"
@@ -301,7 +302,9 @@ inline size_t tri_grad_bytes(int K, int kt, int ts, int nsplit, int nacc) {
 // Ket tile and term groups of a lower-triangle pass: of the ket tiles
 // 2..64 (divisors of 64, for the upper pairs) and the term groups ts = 1, 2,
 // 4, ... (as many as give distinct terms per group, while the launch has at
-// most 8 workgroups per CU), the pair with the shortest tri_makespan.
+// most 8 workgroups per CU), the pair with the shortest tri_makespan; of
+// those with its term groups and within 0.3 % of it, the one with the
+// smallest partials.
 // req_kt / req_ts > 0 fix one or both.  budget > 0: tilings whose partials
 // (tri_grad_bytes) exceed it are passed over unless none fits, and then the
 // smallest is taken.  Ket tiles whose list would exceed 2^18 tiles are passed
@@ -323,8 +326,9 @@ inline TriChoice tri_choose(int K, int nst, int n, int w, int fold_mode, int ncu
   ncu = std::max(1, ncu);
   const double ovh = tri_overhead(n);
   const long long nbt = (K + 63) / 64;
-  TriChoice best, small;
+  TriChoice small;
   size_t small_bytes = 0;
+  std::vector<std::pair<TriChoice, size_t>> cands;   // (tiling, partial bytes) within budget
   for (int q : kts) {
     const long long approx = nbt * (nbt + 1) / 2 * ((64 + q - 1) / q);
     if (req_kt <= 0 && q < 64 && approx > (1LL << 18)) continue;
@@ -346,10 +350,22 @@ inline TriChoice tri_choose(int K, int nst, int n, int w, int fold_mode, int ncu
         small_bytes = bytes;
       }
       if (budget > 0 && bytes > budget) continue;
-      if (best.cost < 0.0 || c < best.cost) best = {q, ts, c};
+      cands.push_back({{q, ts, c}, bytes});
     }
   }
-  return best.cost >= 0.0 ? best : small;
+  if (cands.empty()) return small;
+  // of the tilings with the shortest one's term groups and within 0.3 % of
+  // it (the model's resolution), the smallest partials: less to write, and
+  // less for the reduction after the pass
+  const std::pair<TriChoice, size_t> *best = &cands[0];
+  for (const auto &e : cands)
+    if (e.first.cost < best->first.cost) best = &e;
+  const std::pair<TriChoice, size_t> *pick = best;
+  for (const auto &e : cands)
+    if (e.first.ts == best->first.ts && e.first.cost <= 1.003 * best->first.cost &&
+        e.second < pick->second)
+      pick = &e;
+  return pick->first;
 }
 
 }  // namespace necg

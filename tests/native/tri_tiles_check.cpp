@@ -190,13 +190,14 @@ static int check_list(int K, int kt, int ncu, int fold, int &nsplit_lists) {
 // at each size below, a sweep over forced tilings (ket tile 4..64, term
 // groups 1..16, pair pass time over 5-10 launches) found the expected
 // tiling fastest, or within 0.2 % of the fastest (LiH K = 700: 16 x 4 took
-// 0.7528 ms, 8 x 2 0.7536; K = 4096: 32 x 1 22.668 ms, 16 x 1 22.719).  With a
+// 0.7528 ms, 8 x 2 0.7536; K = 3000: 8 x 1 12.349 ms, 16 x 1 12.365; K = 2048,
+// four alternating runs: 16 x 1 5.720-5.744 ms, 8 x 1 5.735-5.743).  With a
 // byte budget below every tiling's partials the smallest is taken.
 int check_choose(bool quick) {
   struct Want { int n, nst, K, kt, ts; };
   const Want want[] = {
       {5, 24, 256, 8, 4},  {5, 24, 400, 8, 8},   {5, 24, 700, 8, 2},  {5, 24, 1300, 8, 1},
-      {5, 24, 1600, 32, 2}, {5, 24, 3000, 8, 1}, {5, 24, 4096, 16, 1},
+      {5, 24, 1600, 32, 2}, {5, 24, 2048, 16, 1}, {5, 24, 3000, 16, 1}, {5, 24, 4096, 32, 1},
       {2, 2, 300, 4, 1},   {2, 2, 1024, 16, 1},  {2, 2, 2048, 32, 1}, {2, 2, 4096, 32, 1},
       {3, 2, 300, 4, 1},   {3, 2, 1024, 32, 1},  {3, 2, 2048, 32, 1}, {3, 2, 4096, 32, 1}};
   int bad = 0;
