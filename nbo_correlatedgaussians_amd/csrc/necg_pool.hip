@@ -156,25 +156,63 @@ __global__ void __launch_bounds__(256) t_energy(const PoolSys *__restrict__ sys,
   }
 }
 
+// v from the DPP partner lane (CTRL: 0xB1 quad_perm [1,0,3,2], 0x4E
+// quad_perm [2,3,0,1], 0x141 half-row mirror, 0x140 row mirror); every lane
+// has a partner and every row and bank is enabled.
+template <int CTRL>
+__device__ __forceinline__ double dpp_partner(double v) {
+  const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = __builtin_amdgcn_mov_dpp((int)(unsigned)u, CTRL, 0xf, 0xf, false);
+  const unsigned hi = __builtin_amdgcn_mov_dpp((int)(unsigned)(u >> 32), CTRL, 0xf, 0xf, false);
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+// v plus its value in the other 16-lane row of the pair (permlane16 swap,
+// W = 16) or the other 32-lane half (permlane32 swap, W = 32).
+template <int W>
+__device__ __forceinline__ double swap_add(double v) {
+  const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+  const unsigned l = (unsigned)u, h = (unsigned)(u >> 32);
+  unsigned l0, l1, h0, h1;
+  if constexpr (W == 16) {
+    const auto a = __builtin_amdgcn_permlane16_swap(l, l, false, false);
+    const auto b = __builtin_amdgcn_permlane16_swap(h, h, false, false);
+    l0 = a[0]; l1 = a[1]; h0 = b[0]; h1 = b[1];
+  } else {
+    const auto a = __builtin_amdgcn_permlane32_swap(l, l, false, false);
+    const auto b = __builtin_amdgcn_permlane32_swap(h, h, false, false);
+    l0 = a[0]; l1 = a[1]; h0 = b[0]; h1 = b[1];
+  }
+  return __builtin_bit_cast(double, ((unsigned long long)h0 << 32) | l0) +
+         __builtin_bit_cast(double, ((unsigned long long)h1 << 32) | l1);
+}
+
+// Sum of v over the 64 lanes, the same bits in every lane: each step adds
+// the partner's partial sum, and a + b = b + a exactly.
+__device__ __forceinline__ double wave_allsum(double v) {
+  v += dpp_partner<0xB1>(v);
+  v += dpp_partner<0x4E>(v);
+  v += dpp_partner<0x141>(v);
+  v += dpp_partner<0x140>(v);
+  v = swap_add<16>(v);
+  return swap_add<32>(v);
+}
+
 // Sum of four values over a 256-lane workgroup in a fixed order; every lane
-// receives the totals.
-__device__ __forceinline__ void block_sum4(double v[4], double *red) {
+// receives the totals.  red: 32 doubles, used in two halves by turns (buf),
+// so one barrier per call suffices: a half is written again only two calls
+// later, after every lane has passed the barrier of the call between.
+__device__ __forceinline__ void block_sum4(double v[4], double *red, int &buf) {
+  double *r = red + 16 * buf;
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
-    double a = v[q];
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) a += __shfl_down(a, o, 64);
-    if ((threadIdx.x & 63) == 0) red[(threadIdx.x >> 6) * 4 + q] = a;
-  }
-  __syncthreads();
-  if (threadIdx.x < 4) {
-    const int q = threadIdx.x;
-    red[16 + q] = (red[q] + red[4 + q]) + (red[8 + q] + red[12 + q]);
+    const double a = wave_allsum(v[q]);
+    if ((threadIdx.x & 63) == 0) r[(threadIdx.x >> 6) * 4 + q] = a;
   }
   __syncthreads();
 #pragma unroll
-  for (int q = 0; q < 4; ++q) v[q] = red[16 + q];
-  __syncthreads();
+  for (int q = 0; q < 4; ++q) v[q] = (r[q] + r[4 + q]) + (r[8 + q] + r[12 + q]);
+  buf ^= 1;
 }
 
 constexpr int kProjSlices = 8;  // i-range slices of the projection (partial rows)
@@ -251,7 +289,7 @@ struct TrialCount {
 __device__ TrialCount trial_count(const double *__restrict__ lam, const double *__restrict__ a,
                                   const double *__restrict__ beta,
                                   const double *__restrict__ gam, int K, double x,
-                                  double *red) {
+                                  double *red, int &buf) {
   double v[4] = {0.0, 0.0, 0.0, 0.0};  // below, aa, ab, bb
   for (int j = threadIdx.x; j < K; j += 256) {
     double d = lam[j] - x;
@@ -263,7 +301,7 @@ __device__ TrialCount trial_count(const double *__restrict__ lam, const double *
     v[2] = __builtin_fma(a[j] * bj, r, v[2]);
     v[3] = __builtin_fma(bj * bj, r, v[3]);
   }
-  block_sum4(v, red);
+  block_sum4(v, red, buf);
   const double p = -v[1], q = -1.0 - v[2], rr = -v[3];
   TrialCount c;
   c.detN = p * rr - q * q;
@@ -276,9 +314,15 @@ __device__ TrialCount trial_count(const double *__restrict__ lam, const double *
 // One workgroup (256) per trial: the lowest eigenvalue mu of the trial pencil.
 // Bracket: mu <= lam_2 (a rank-2 change moves the lowest eigenvalue at most
 // past lam_2: nu(D) = 2 just above it) and mu <= E, the trial's Rayleigh
-// quotient; below, step down until the count is 0.  Then bisection while a
-// pole lam_j lies inside the bracket, and Illinois regula falsi on det N once
-// none does, down to adjacent doubles.  NaN if the system has no resident
+// quotient; below, step down until the count is 0.  While a pole lam_j lies
+// inside the bracket, the count is taken just above (or below) the lowest
+// such pole, which moves an end past it or onto it.  In a bracket free of
+// poles, between lam_{m-1} and lam_m, det N has simple poles at both (its
+// double-pole terms cancel), so Illinois regula falsi runs on the smooth
+// g(x) = det N(x) (x - lam_{m-1}) (lam_m - x), which changes sign where the
+// count does; a step that would land within an ulp of an end takes the end's
+// neighbour instead.  Down to adjacent doubles: 9-14 counts per trial where
+// bisection to the pole took 26-49.  NaN if the system has no resident
 // factors or S' is not positive definite by a margin of 1e-10 (relative to S).
 __global__ void __launch_bounds__(256) t_eig(const PoolSys *__restrict__ sys,
                                              const int32_t *__restrict__ tsys,
@@ -287,7 +331,8 @@ __global__ void __launch_bounds__(256) t_eig(const PoolSys *__restrict__ sys,
                                              const double *__restrict__ E,
                                              double *__restrict__ bgs,
                                              double *__restrict__ Eeig, int *__restrict__ flag) {
-  __shared__ double red[20];
+  __shared__ double red[32];
+  int buf = 0;
   const int t = blockIdx.x;
   int k = 0;
   const PoolSys *bp = trial_sys(sys, nsys, tsys, tfun, t, k);
@@ -319,7 +364,7 @@ __global__ void __launch_bounds__(256) t_eig(const PoolSys *__restrict__ sys,
     v[2] = __builtin_fma(a[j], ga, v[2]);
     v[3] = __builtin_fma(ga, ga, v[3]);
   }
-  block_sum4(v, red);  // also orders the beta/gamma stores before the reads below
+  block_sum4(v, red, buf);  // also orders the beta/gamma stores before the reads below
   const double smin = 1.0 + v[2] - sqrt(v[1]) * sqrt(v[3]);
   double hi = E[t];
   if (!(smin > 1e-10) || !isfinite(hi)) {
@@ -330,43 +375,76 @@ __global__ void __launch_bounds__(256) t_eig(const PoolSys *__restrict__ sys,
   if (K >= 2) hi = fmin(hi, b.lam[1]);
   double step = 1e-12 * scale;
   hi += step;
-  TrialCount ch = trial_count(b.lam, a, beta, gam, K, hi, red);
+  TrialCount ch = trial_count(b.lam, a, beta, gam, K, hi, red, buf);
   for (int it = 0; ch.count < 1 && it < 64; ++it) {
     step *= 2.0;
     hi += step;
-    ch = trial_count(b.lam, a, beta, gam, K, hi, red);
+    ch = trial_count(b.lam, a, beta, gam, K, hi, red, buf);
   }
   step = 1e-3 * scale;
   double lo = fmin(hi, b.lam[0]) - step;
-  TrialCount cl = trial_count(b.lam, a, beta, gam, K, lo, red);
+  TrialCount cl = trial_count(b.lam, a, beta, gam, K, lo, red, buf);
   for (int it = 0; cl.count >= 1 && it < 200; ++it) {
     step *= 2.0;
     lo -= step;
-    cl = trial_count(b.lam, a, beta, gam, K, lo, red);
+    cl = trial_count(b.lam, a, beta, gam, K, lo, red, buf);
   }
+  // (x - lam_{m-1}) (lam_m - x): the factor that removes the poles at the
+  // ends of a bracket with m resident eigenvalues below it
+  auto gfac = [&](double x, int m) {
+    double f = 1.0;
+    if (m >= 1) f *= x - b.lam[m - 1];
+    if (m < K) f *= b.lam[m] - x;
+    return f;
+  };
+  const double eps = 1e-12 * scale;
+  bool have_g = false;
+  double glo = 0.0, ghi = 0.0;
   int side = 0;  // last end moved: -1 lo, +1 hi (Illinois)
-  double flo = cl.detN, fhi = ch.detN;
   for (int it = 0; it < 200; ++it) {
     const double mid = 0.5 * (lo + hi);
     if (!(mid > lo && mid < hi)) break;
     double x = mid;
-    if (cl.below == ch.below && ((flo < 0.0) != (fhi < 0.0))) {
-      const double xf = hi - fhi * (hi - lo) / (fhi - flo);
-      if (xf > lo && xf < hi) x = xf;
+    if (cl.below != ch.below) {   // a pole inside: the count beside the lowest one
+      const double pole = b.lam[cl.below];
+      x = pole + eps;
+      if (!(x > lo && x < hi)) {
+        x = pole - eps;
+        if (!(x > lo && x < hi)) x = mid;
+      }
+      have_g = false;
+    } else {
+      if (!have_g) {
+        glo = cl.detN * gfac(lo, cl.below);
+        ghi = ch.detN * gfac(hi, cl.below);
+        side = 0;
+        have_g = true;
+      }
+      if ((glo < 0.0) != (ghi < 0.0)) {
+        const double xf = hi - ghi * (hi - lo) / (ghi - glo);
+        if (xf > lo && xf < hi) x = xf;
+        else if (xf <= lo) x = nextafter(lo, hi);
+        else x = nextafter(hi, lo);
+        if (!(x > lo && x < hi)) x = mid;
+      }
     }
-    const TrialCount cx = trial_count(b.lam, a, beta, gam, K, x, red);
+    const TrialCount cx = trial_count(b.lam, a, beta, gam, K, x, red, buf);
     if (cx.count >= 1) {
       hi = x;
       ch = cx;
-      fhi = cx.detN;
-      if (side == 1) flo *= 0.5;
-      side = 1;
+      if (have_g) {
+        ghi = cx.detN * gfac(x, cx.below);
+        if (side == 1) glo *= 0.5;
+        side = 1;
+      }
     } else {
       lo = x;
       cl = cx;
-      flo = cx.detN;
-      if (side == -1) fhi *= 0.5;
-      side = -1;
+      if (have_g) {
+        glo = cx.detN * gfac(x, cx.below);
+        if (side == -1) ghi *= 0.5;
+        side = -1;
+      }
     }
     if (cx.detN == 0.0) break;
   }
