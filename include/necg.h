@@ -316,10 +316,25 @@ int necg_pool_set_factors(necg_pool *pool, int32_t s, const double *V, const dou
  * energy with C re-optimised for the replaced basis, from the resident factors
  * in O(K^2) per trial.  NaN where the system has no factors (and the
  * bad-input status is latched) or where S' is not positive definite by a
- * margin (smallest eigenvalue of S' relative to S below 1e-10; no status).  bg: device workspace of ntr * 2 * kmax doubles.  Async. */
+ * margin (smallest eigenvalue of S' relative to S below 1e-10; no status).
+ * bg: device workspace of ntr * 18 * kmax doubles (kmax: necg_pool_sizes).
+ * Async. */
 int necg_pool_eig(necg_pool *pool, int32_t ntr, const int32_t *tsys, const int32_t *tfun,
                   const double *rowH, const double *rowS, int32_t rstride, const double *E,
                   double *bg, double *Eeig, void *stream);
+
+/* necg_pool_trials and (bg not NULL) necg_pool_eig in one call, with the
+ * copies in and out: no counterpart in the reference; it is for serving
+ * loops, where every host-side call adds latency.  On `stream`: copies in_h
+ * (host, pinned for an asynchronous copy) to in_d (device, 8-byte aligned),
+ * laid out [tsys (ntr int32) | tfun (ntr int32) | newp (ntr x pstride
+ * doubles, from byte (8 ntr + 7) & ~7)]; runs the trials with E into out_d[0,
+ * ntr) and Eeig into out_d[ntr, 2 ntr); copies out_d's ntr (or 2 ntr) values
+ * to out_h (host).  Results as the two calls; the caller synchronises on
+ * `stream` before reading out_h.  ws, rowH, rowS, bg as there.  Async. */
+int necg_pool_trials_io(necg_pool *pool, int32_t ntr, const void *in_h, void *in_d,
+                        int32_t pstride, double *ws, double *rowH, double *rowS, int32_t rstride,
+                        double *bg, double *out_d, double *out_h, void *stream);
 
 /* A pool over the resident state of every system of a batch (after one full
  * evaluation of the batch).  The batch must outlive the pool. */

@@ -36,7 +36,7 @@ EXPORTS = ("necg_create", "necg_destroy", "necg_outprod", "necg_outprod_noalloc"
            "necg_batch_set_tiling", "necg_batch_status", "necg_batch_outprod_masked",
            "necg_batch_pool", "necg_pool_create",
            "necg_pool_destroy", "necg_pool_sizes", "necg_pool_trials", "necg_pool_status",
-           "necg_pool_set_factors", "necg_pool_eig")
+           "necg_pool_set_factors", "necg_pool_eig", "necg_pool_trials_io")
 
 
 class NecgError(RuntimeError):
@@ -131,6 +131,7 @@ def load() -> ct.CDLL:
         "necg_pool_trials": [P, I, P, P, P, I, P, P, P, P, I, P],
         "necg_pool_set_factors": [P, I, P, P],
         "necg_pool_eig": [P, I, P, P, P, P, I, P, P, P, P],
+        "necg_pool_trials_io": [P, I, P, P, I, P, P, P, I, P, P, P, P],
         "necg_pool_status": [P, P],
     }
     for name, args in sig.items():

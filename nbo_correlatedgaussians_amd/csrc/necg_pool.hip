@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdint>
 #include <new>
 #include <vector>
 
@@ -502,6 +503,55 @@ bool snapshot(necg_pool *P, std::vector<PoolSys> &out) {
   return true;
 }
 
+// A context-backed pool: whether the contexts still hold the state the pool
+// was made from (a batch-backed pool always does).
+bool pool_current(necg_pool *P) {
+  if (P->ctx.empty()) return true;
+  std::vector<PoolSys> now;
+  if (!snapshot(P, now)) return false;
+  for (size_t i = 0; i < now.size(); ++i)
+    if (now[i].bra != P->hsys[i].bra || now[i].H != P->hsys[i].H || now[i].K != P->hsys[i].K)
+      return false;
+  return true;
+}
+
+// The pool's device current for the scope of a call, the caller's restored.
+struct PoolDevice {
+  int prev = -1, dev;
+  explicit PoolDevice(int d) : dev(d) {
+    (void)hipGetDevice(&prev);
+    if (prev != dev) (void)hipSetDevice(dev);
+  }
+  ~PoolDevice() {
+    if (prev >= 0 && prev != dev) (void)hipSetDevice(prev);
+  }
+};
+
+// The trial kernels: new records, rows of H' and S', the quotient.
+void launch_trials(necg_pool *P, int ntr, const int32_t *tsys, const int32_t *tfun,
+                   const double *newp, int pstride, double *ws, double *E, double *rowH,
+                   double *rowS, int rstride, hipStream_t st) {
+  const int nsys = (int)P->hsys.size();
+  hipLaunchKernelGGL(t_prep, dim3(ntr), dim3(256), 0, st, P->d_sys, tsys, tfun, nsys, newp,
+                     pstride, ws, (long long)P->wstride, P->flag);
+  hipLaunchKernelGGL(t_rows, dim3((P->kmax + 63) / 64, ntr), dim3(64), 0, st, P->d_sys, tsys, tfun,
+                     nsys, ws, (long long)P->wstride, rowH, rowS, rstride, P->flag);
+  hipLaunchKernelGGL(t_energy, dim3(ntr), dim3(256), 0, st, P->d_sys, tsys, tfun, nsys, rowH,
+                     rowS, rstride, E, P->flag);
+}
+
+// The eigenvalue kernels: projection on the resident factors, the root.
+void launch_eig(necg_pool *P, int ntr, const int32_t *tsys, const int32_t *tfun,
+                const double *rowH, const double *rowS, int rstride, const double *E,
+                double *bg, double *Eeig, hipStream_t st) {
+  const int nsys = (int)P->hsys.size();
+  double *bgs = bg + (size_t)ntr * kProjSlices * 2 * P->kmax;
+  hipLaunchKernelGGL(t_proj, dim3((P->kmax + 255) / 256, ntr, kProjSlices), dim3(256), 0, st,
+                     P->d_sys, tsys, tfun, nsys, rowH, rowS, rstride, bg, P->kmax);
+  hipLaunchKernelGGL(t_eig, dim3(ntr), dim3(256), 0, st, P->d_sys, tsys, tfun, nsys, bg, P->kmax,
+                     E, bgs, Eeig, P->flag);
+}
+
 }  // namespace
 
 int necg::pool_create_from(int device, const std::vector<PoolSys> &recs, necg_pool **out) {
@@ -578,29 +628,42 @@ int necg_pool_trials(necg_pool *P, int32_t ntr, const int32_t *tsys, const int32
   if (ntr == 0) return NECG_OK;
   NECG_REQUIRE_DEVICE((const void *)tsys, (const void *)tfun, (const void *)newp,
                       (const void *)ws, (const void *)E, (const void *)rowH, (const void *)rowS);
-  // a context-backed pool: the contexts must still hold the state the pool
-  // was made from
-  if (!P->ctx.empty()) {
-    std::vector<PoolSys> now;
-    if (!snapshot(P, now)) return NECG_ERR_ARG;
-    for (size_t i = 0; i < now.size(); ++i)
-      if (now[i].bra != P->hsys[i].bra || now[i].H != P->hsys[i].H || now[i].K != P->hsys[i].K)
-        return NECG_ERR_ARG;
-  }
-  int prev = -1;
-  (void)hipGetDevice(&prev);
-  if (prev != P->device) (void)hipSetDevice(P->device);
+  if (!pool_current(P)) return NECG_ERR_ARG;
+  PoolDevice pd(P->device);
+  launch_trials(P, ntr, tsys, tfun, newp, pstride, ws, E, rowH, rowS, rstride,
+                (hipStream_t)stream);
+  return hipGetLastError() == hipSuccess ? NECG_OK : NECG_ERR_DEVICE;
+}
+
+int necg_pool_trials_io(necg_pool *P, int32_t ntr, const void *in_h, void *in_d,
+                        int32_t pstride, double *ws, double *rowH, double *rowS, int32_t rstride,
+                        double *bg, double *out_d, double *out_h, void *stream) {
+  if (!P || ntr < 0 ||
+      (ntr > 0 && (!in_h || !in_d || !ws || !rowH || !rowS || !out_d || !out_h)) ||
+      rstride < P->kmax || pstride < 1)
+    return NECG_ERR_ARG;
+  if (ntr == 0) return NECG_OK;
+  NECG_REQUIRE_DEVICE((const void *)in_d, (const void *)ws, (const void *)rowH,
+                      (const void *)rowS, (const void *)bg, (const void *)out_d);
+  if (((uintptr_t)in_d & 7u) != 0) return NECG_ERR_ARG;
+  if (!pool_current(P)) return NECG_ERR_ARG;
+  PoolDevice pd(P->device);
   hipStream_t st = (hipStream_t)stream;
-  const int nsys = (int)P->hsys.size();
-  hipLaunchKernelGGL(t_prep, dim3(ntr), dim3(256), 0, st, P->d_sys, tsys, tfun, nsys, newp,
-                     pstride, ws, (long long)P->wstride, P->flag);
-  hipLaunchKernelGGL(t_rows, dim3((P->kmax + 63) / 64, ntr), dim3(64), 0, st, P->d_sys, tsys, tfun,
-                     nsys, ws, (long long)P->wstride, rowH, rowS, rstride, P->flag);
-  hipLaunchKernelGGL(t_energy, dim3(ntr), dim3(256), 0, st, P->d_sys, tsys, tfun, nsys, rowH,
-                     rowS, rstride, E, P->flag);
-  const int rc = hipGetLastError() == hipSuccess ? NECG_OK : NECG_ERR_DEVICE;
-  if (prev >= 0 && prev != P->device) (void)hipSetDevice(prev);
-  return rc;
+  const size_t off = ((size_t)8 * ntr + 7) & ~(size_t)7;
+  const size_t bytes = off + (size_t)8 * ntr * pstride;
+  if (hipMemcpyAsync(in_d, in_h, bytes, hipMemcpyHostToDevice, st) != hipSuccess)
+    return NECG_ERR_DEVICE;
+  const int32_t *tsys = (const int32_t *)in_d;
+  const int32_t *tfun = tsys + ntr;
+  const double *newp = (const double *)((const char *)in_d + off);
+  launch_trials(P, ntr, tsys, tfun, newp, pstride, ws, out_d, rowH, rowS, rstride, st);
+  if (bg) launch_eig(P, ntr, tsys, tfun, rowH, rowS, rstride, out_d, bg, out_d + ntr, st);
+  if (hipGetLastError() != hipSuccess) return NECG_ERR_DEVICE;
+  const size_t nout = (size_t)(bg ? 2 : 1) * ntr;
+  if (hipMemcpyAsync(out_h, out_d, nout * sizeof(double), hipMemcpyDeviceToHost, st) !=
+      hipSuccess)
+    return NECG_ERR_DEVICE;
+  return NECG_OK;
 }
 
 int necg_pool_set_factors(necg_pool *P, int32_t s, const double *V, const double *lam) {
@@ -626,19 +689,9 @@ int necg_pool_eig(necg_pool *P, int32_t ntr, const int32_t *tsys, const int32_t 
   if (ntr == 0) return NECG_OK;
   NECG_REQUIRE_DEVICE((const void *)tsys, (const void *)tfun, (const void *)rowH,
                       (const void *)rowS, (const void *)E, (const void *)bg, (const void *)Eeig);
-  int prev = -1;
-  (void)hipGetDevice(&prev);
-  if (prev != P->device) (void)hipSetDevice(P->device);
-  hipStream_t st = (hipStream_t)stream;
-  const int nsys = (int)P->hsys.size();
-  double *bgs = bg + (size_t)ntr * kProjSlices * 2 * P->kmax;
-  hipLaunchKernelGGL(t_proj, dim3((P->kmax + 255) / 256, ntr, kProjSlices), dim3(256), 0, st,
-                     P->d_sys, tsys, tfun, nsys, rowH, rowS, rstride, bg, P->kmax);
-  hipLaunchKernelGGL(t_eig, dim3(ntr), dim3(256), 0, st, P->d_sys, tsys, tfun, nsys, bg, P->kmax,
-                     E, bgs, Eeig, P->flag);
-  const int rc = hipGetLastError() == hipSuccess ? NECG_OK : NECG_ERR_DEVICE;
-  if (prev >= 0 && prev != P->device) (void)hipSetDevice(prev);
-  return rc;
+  PoolDevice pd(P->device);
+  launch_eig(P, ntr, tsys, tfun, rowH, rowS, rstride, E, bg, Eeig, (hipStream_t)stream);
+  return hipGetLastError() == hipSuccess ? NECG_OK : NECG_ERR_DEVICE;
 }
 
 int necg_pool_status(necg_pool *P, void *stream) {

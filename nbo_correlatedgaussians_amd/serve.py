@@ -28,7 +28,8 @@ until it ends, a full evaluation or an accept waits for its system's trials
 in flight, and a trial that arrived after a waiting full evaluation or accept
 of its system waits behind it.
 Latency is measured from a request's arrival to its results being on the
-host.
+host.  The loop polls (event queries, arrivals) and sleeps only while it is
+idle and the next arrival is more than a millisecond away.
 """
 
 from __future__ import annotations
@@ -133,16 +134,20 @@ class Server:
         self._tbuf = []
         for _ in range(trial_streams):
             T = max_trials
+            # inputs packed as [tsys (n int32) | tfun (n int32) | newp (n x pmax
+            # doubles, 8-byte aligned)] and outputs as [E (n) | Eeig (n)]: one
+            # copy each way per launch
+            nin = 8 * T + 8 * T * self.pmax + 8
             self._tbuf.append(dict(
                 stream=torch.cuda.Stream(device=self.device),
-                tsys=torch.empty(T, **i32), tfun=torch.empty(T, **i32),
-                newp=torch.empty(T * self.pmax, **f64), ws=torch.empty(T * self.wstride, **f64),
-                E=torch.empty(T, **f64), rowH=torch.empty(T * self.kmax, **f64),
-                rowS=torch.empty(T * self.kmax, **f64),
-                bg=torch.empty(T * 18 * self.kmax, **f64), Eeig=torch.empty(T, **f64),
-                h_in=torch.empty(T * (2 + self.pmax), dtype=torch.float64).pin_memory(),
-                h_E=torch.empty(T, dtype=torch.float64).pin_memory(),
-                h_Eeig=torch.empty(T, dtype=torch.float64).pin_memory()))
+                d_in=torch.empty(nin, dtype=torch.uint8, device=self.device),
+                h_in=torch.empty(nin, dtype=torch.uint8).pin_memory(),
+                ws=torch.empty(T * self.wstride, **f64),
+                rowH=torch.empty(T * self.kmax, **f64), rowS=torch.empty(T * self.kmax, **f64),
+                bg=torch.empty(T * 18 * self.kmax, **f64),
+                d_out=torch.empty(2 * T, **f64),
+                h_out=torch.empty(2 * T, dtype=torch.float64).pin_memory(),
+                ev=torch.cuda.Event()))   # reused: a slot runs one launch at a time
         self._ebuf = []
         for _ in range(eval_streams):
             self._ebuf.append(dict(
@@ -151,7 +156,8 @@ class Server:
                 g=torch.empty(self.batch.nx, **f64),
                 h_active=torch.zeros(nsys, dtype=torch.int32).pin_memory(),
                 h_f=torch.empty(nsys, dtype=torch.float64).pin_memory(),
-                h_g=torch.empty(self.batch.nx, dtype=torch.float64).pin_memory()))
+                h_g=torch.empty(self.batch.nx, dtype=torch.float64).pin_memory(),
+                ev=torch.cuda.Event()))
         self._engines = None
         # bytes a request holds while in flight (admission budget)
         self.trial_bytes = 8 * (self.pmax + self.wstride + 2 + 20 * self.kmax)
@@ -212,39 +218,29 @@ class Server:
 
     # -- launches ---------------------------------------------------------
     def _launch_trials(self, slot, reqs):
-        torch = _torch()
         b = self._tbuf[slot]
         n = len(reqs)
-        hin = b["h_in"].numpy()
-        hin[:n] = [r.sys for r in reqs]
-        hin[n:2 * n] = [r.func for r in reqs]
         P = self.pmax
+        hb = b["h_in"].numpy()
+        off_p = (8 * n + 7) & ~7                      # newp: 8-byte aligned after the indices
+        nbytes = off_p + 8 * n * P
+        idx = hb[:8 * n].view(np.int32)
+        idx[:n] = [r.sys for r in reqs]
+        idx[n:] = [r.func for r in reqs]
+        newp = hb[off_p:nbytes].view(np.float64)
         for i, r in enumerate(reqs):
-            hin[2 * n + i * P: 2 * n + i * P + r.newp.size] = r.newp
-        with torch.cuda.stream(b["stream"]):
-            staged = b["h_in"][:2 * n + n * P].to(self.device, non_blocking=True)
-            b["tsys"][:n].copy_(staged[:n].to(torch.int32))
-            b["tfun"][:n].copy_(staged[n:2 * n].to(torch.int32))
-            b["newp"][:n * P].copy_(staged[2 * n:])
-            st = ct.c_void_p(b["stream"].cuda_stream)
-            rc = self.lib.necg_pool_trials(
-                self._pool, n, ct.c_void_p(b["tsys"].data_ptr()), ct.c_void_p(b["tfun"].data_ptr()),
-                ct.c_void_p(b["newp"].data_ptr()), P, ct.c_void_p(b["ws"].data_ptr()),
-                ct.c_void_p(b["E"].data_ptr()), ct.c_void_p(b["rowH"].data_ptr()),
-                ct.c_void_p(b["rowS"].data_ptr()), self.kmax, st)
-            _lib.raise_for(rc, "necg_pool_trials")
-            if self.factors:
-                rc = self.lib.necg_pool_eig(
-                    self._pool, n, ct.c_void_p(b["tsys"].data_ptr()),
-                    ct.c_void_p(b["tfun"].data_ptr()), ct.c_void_p(b["rowH"].data_ptr()),
-                    ct.c_void_p(b["rowS"].data_ptr()), self.kmax, ct.c_void_p(b["E"].data_ptr()),
-                    ct.c_void_p(b["bg"].data_ptr()), ct.c_void_p(b["Eeig"].data_ptr()), st)
-                _lib.raise_for(rc, "necg_pool_eig")
-                b["h_Eeig"][:n].copy_(b["Eeig"][:n], non_blocking=True)
-            b["h_E"][:n].copy_(b["E"][:n], non_blocking=True)
-            ev = torch.cuda.Event()
-            ev.record(b["stream"])
-        return _Launch("trial", slot, ev, reqs, self.trial_bytes * n, {r.sys for r in reqs})
+            newp[i * P:i * P + r.newp.size] = r.newp
+        st = ct.c_void_p(b["stream"].cuda_stream)
+        # copy in, the trial and eigenvalue kernels, copy out: one call
+        rc = self.lib.necg_pool_trials_io(
+            self._pool, n, ct.c_void_p(b["h_in"].data_ptr()), ct.c_void_p(b["d_in"].data_ptr()),
+            P, ct.c_void_p(b["ws"].data_ptr()), ct.c_void_p(b["rowH"].data_ptr()),
+            ct.c_void_p(b["rowS"].data_ptr()), self.kmax,
+            ct.c_void_p(b["bg"].data_ptr()) if self.factors else None,
+            ct.c_void_p(b["d_out"].data_ptr()), ct.c_void_p(b["h_out"].data_ptr()), st)
+        _lib.raise_for(rc, "necg_pool_trials_io")
+        b["ev"].record(b["stream"])
+        return _Launch("trial", slot, b["ev"], reqs, self.trial_bytes * n, {r.sys for r in reqs})
 
     def _launch_full(self, slot, reqs):
         """One masked batch evaluation of the requests' systems (distinct);
@@ -269,10 +265,9 @@ class Server:
             for r in reqs:
                 lo, hi = int(self.offsets[r.sys]), int(self.offsets[r.sys + 1])
                 b["h_g"][lo:hi].copy_(b["g"][lo:hi], non_blocking=True)
-            ev = torch.cuda.Event()
-            ev.record(b["stream"])
+            b["ev"].record(b["stream"])
         nbytes = sum(self.full_bytes[r.sys] for r in reqs)
-        return _Launch("full", slot, ev, list(reqs), nbytes, {r.sys for r in reqs})
+        return _Launch("full", slot, b["ev"], list(reqs), nbytes, {r.sys for r in reqs})
 
     # -- the loop ---------------------------------------------------------
     def run(self, requests, on_pass=None):
@@ -310,11 +305,11 @@ class Server:
                 tdone = time.perf_counter() - t0
                 if L.kind == "trial":
                     b = self._tbuf[L.slot]
-                    E = b["h_E"].numpy()
-                    Ee = b["h_Eeig"].numpy()
+                    out = b["h_out"].numpy()
+                    nl = len(L.reqs)
                     for i, r in enumerate(L.reqs):
-                        r.E = float(E[i])
-                        r.Eeig = float(Ee[i]) if self.factors else None
+                        r.E = float(out[i])
+                        r.Eeig = float(out[nl + i]) if self.factors else None
                         r.done = tdone
                         trial_busy[r.sys] -= 1
                     free_t.append(L.slot)
@@ -381,9 +376,13 @@ class Server:
             if on_pass:
                 on_pass(len(inflight), len(ready_t) + len(ready_f))
             if not inflight and ptr < total and not ready_t and not ready_f:
+                # idle: sleep only while the next arrival is more than 1 ms
+                # away, and wake 0.5 ms early; the rest is polled (a sleep
+                # overshoots by tens of microseconds, which a trial of about
+                # 60 us on the device would pay in full)
                 wait = reqs[ptr].arrival - (time.perf_counter() - t0)
-                if wait > 0:
-                    time.sleep(min(wait, 1e-3))
+                if wait > 1e-3:
+                    time.sleep(wait - 5e-4)
         self.batch.check("serving")
         _lib.raise_for(self.lib.necg_pool_status(self._pool, ct.c_void_p(
             self._tbuf[0]["stream"].cuda_stream)), "serving trials")
@@ -456,9 +455,9 @@ class Server:
         _lib.raise_for(self.lib.necg_pool_status(self._pool, ct.c_void_p(
             self._tbuf[0]["stream"].cuda_stream)), "trial")
         b = self._tbuf[0]
-        return float(b["h_E"][0]), (b["rowH"][:self.Ks[r.sys]].cpu().numpy(),
-                                    b["rowS"][:self.Ks[r.sys]].cpu().numpy()), (
-            float(b["h_Eeig"][0]) if self.factors else None)
+        return float(b["h_out"][0]), (b["rowH"][:self.Ks[r.sys]].cpu().numpy(),
+                                      b["rowS"][:self.Ks[r.sys]].cpu().numpy()), (
+            float(b["h_out"][1]) if self.factors else None)
 
 
 def poisson_trace(systems, Ks, count: int, rate: float, trial_frac: float = 0.7,

@@ -212,3 +212,55 @@ def test_accept_makes_replacement_resident():
     w0 = eigh(H2, S2, eigvals_only=True, subset_by_index=[0, 0])[0]
     assert abs(out[2].Eeig - w0) <= 1e-9 * max(1.0, abs(w0))
     assert np.array_equal(srv.xs[0], x1)
+
+
+def test_trials_io_equals_separate_calls():
+    """necg_pool_trials_io (copies in and out around the trial and eigenvalue
+    kernels, one call) against necg_pool_trials + necg_pool_eig on device
+    buffers: bitwise the same E and Eeig for a batch of trials."""
+    import ctypes as ct
+    import torch
+    from nbo_correlatedgaussians_amd import _lib
+    from nbo_correlatedgaussians_amd.serve import Request, Server
+    sx = _systems()
+    srv = Server([s for s, _ in sx], [x for _, x in sx], "cuda:0", trial_streams=1,
+                 eval_streams=1)
+    rng = np.random.default_rng(9)
+    reqs = []
+    for i in range(7):
+        si = int(rng.integers(len(sx)))
+        k = int(rng.integers(srv.Ks[si]))
+        iv1, _ = necg.generate_parameters(sx[si][0], 1, int(rng.integers(1 << 30)))
+        reqs.append(Request(i, "trial", si, 0.0, k, iv1))
+    n, P, kmax = len(reqs), srv.pmax, srv.kmax
+    L = srv._launch_trials(0, reqs)
+    L.event.synchronize()
+    out_io = srv._tbuf[0]["h_out"][:2 * n].clone().numpy()
+    dev = dict(device="cuda:0")
+    tsys = torch.tensor([r.sys for r in reqs], dtype=torch.int32, **dev)
+    tfun = torch.tensor([r.func for r in reqs], dtype=torch.int32, **dev)
+    newp = torch.zeros(n * P, dtype=torch.float64, **dev)
+    for i, r in enumerate(reqs):
+        newp[i * P:i * P + r.newp.size] = torch.as_tensor(r.newp)
+    ws = torch.empty(n * srv.wstride, dtype=torch.float64, **dev)
+    rowH = torch.empty(n * kmax, dtype=torch.float64, **dev)
+    rowS = torch.empty_like(rowH)
+    bg = torch.empty(n * 18 * kmax, dtype=torch.float64, **dev)
+    E = torch.empty(n, dtype=torch.float64, **dev)
+    Ee = torch.empty_like(E)
+    st = ct.c_void_p(torch.cuda.current_stream().cuda_stream)
+    p = lambda t: ct.c_void_p(t.data_ptr())   # noqa: E731
+    lib = _lib.load()
+    assert lib.necg_pool_trials(srv._pool, n, p(tsys), p(tfun), p(newp), P, p(ws), p(E), p(rowH),
+                                p(rowS), kmax, st) == 0
+    assert lib.necg_pool_eig(srv._pool, n, p(tsys), p(tfun), p(rowH), p(rowS), kmax, p(E), p(bg),
+                             p(Ee), st) == 0
+    torch.cuda.synchronize()
+    assert lib.necg_pool_status(srv._pool, st) == 0
+    assert np.array_equal(out_io[:n], E.cpu().numpy())
+    assert np.array_equal(out_io[n:], Ee.cpu().numpy())
+    # a misaligned device input buffer is refused before anything is enqueued
+    assert lib.necg_pool_trials_io(srv._pool, 1, p(srv._tbuf[0]["h_in"]),
+                                   ct.c_void_p(srv._tbuf[0]["d_in"].data_ptr() + 4), P, p(ws),
+                                   p(rowH), p(rowS), kmax, p(bg), p(E),
+                                   p(srv._tbuf[0]["h_out"]), st) == _lib.NECG_ERR_ARG
