@@ -265,6 +265,12 @@ int necg_batch_matrices(necg_batch *batch, int32_t s, double *H, double *S, void
  * the autotuner.  Blocking. */
 int necg_batch_set_tiling(necg_batch *batch, double item_flops, int32_t grid_per_cu);
 
+/* Size the batch's persistent pair launches for ncu CUs (0: the device's
+ * count), leaving the others free of pair workgroups while one runs: a
+ * serving loop keeps CUs for its trial kernels this way.  No counterpart in
+ * the reference.  Re-plans; NECG_ERR_ARG while a pin is held.  Blocking. */
+int necg_batch_set_launch_cus(necg_batch *batch, int32_t ncu);
+
 /* Work-list pin for HIP-graph capture (pin = 1 adds, 0 removes one): a
  * captured batch evaluation holds the addresses of the batch's work lists
  * and partial buffers, which necg_batch_set_tiling replaces.  While any pin

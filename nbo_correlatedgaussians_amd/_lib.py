@@ -36,7 +36,8 @@ EXPORTS = ("necg_create", "necg_destroy", "necg_outprod", "necg_outprod_noalloc"
            "necg_batch_set_tiling", "necg_batch_status", "necg_batch_outprod_masked",
            "necg_batch_pool", "necg_pool_create",
            "necg_pool_destroy", "necg_pool_sizes", "necg_pool_trials", "necg_pool_status",
-           "necg_pool_set_factors", "necg_pool_eig", "necg_pool_trials_io")
+           "necg_pool_set_factors", "necg_pool_eig", "necg_pool_trials_io",
+           "necg_batch_set_launch_cus")
 
 
 class NecgError(RuntimeError):
@@ -132,6 +133,7 @@ def load() -> ct.CDLL:
         "necg_pool_set_factors": [P, I, P, P],
         "necg_pool_eig": [P, I, P, P, P, P, I, P, P, P, P],
         "necg_pool_trials_io": [P, I, P, P, I, P, P, P, I, P, P, P, P],
+        "necg_batch_set_launch_cus": [P, I],
         "necg_pool_status": [P, P],
     }
     for name, args in sig.items():

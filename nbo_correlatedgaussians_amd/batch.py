@@ -108,6 +108,12 @@ class Batch:
                        "necg_batch_set_tiling")
         self.tiling = (float(item_flops), int(grid_per_cu))
 
+    def set_launch_cus(self, ncu: int = 0):
+        """Size the persistent pair launches for ncu CUs (0: the device's),
+        leaving the others free of pair workgroups while one runs."""
+        _lib.raise_for(self.lib.necg_batch_set_launch_cus(self._h, int(ncu)),
+                       "necg_batch_set_launch_cus")
+
     def check(self, where: str = ""):
         _lib.raise_for(self.lib.necg_batch_status(self._h, self._stream()), where)
 

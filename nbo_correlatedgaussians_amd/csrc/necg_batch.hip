@@ -415,6 +415,7 @@ struct necg_batch {
   std::vector<Cls> cls;
   double item_flops = 0.0;  // target cost of one pair tile (model FLOP)
   int grid_per_cu = 0;      // persistent workgroups per CU (0: occupancy)
+  int launch_cus = 0;       // CUs the persistent grids are sized for (0: the device's)
   long long nx = 0;
   int pins = 0;  // necg_batch_pin: captured graphs hold the work lists
   std::mutex mu;
@@ -548,7 +549,8 @@ int plan(necg_batch *B) {
         occ = 1;
       per_cu = occ;
     }
-    c.grid = std::max(1, std::min(c.nitems, per_cu * B->ncu));
+    const int cus = B->launch_cus > 0 ? std::min(B->launch_cus, B->ncu) : B->ncu;
+    c.grid = std::max(1, std::min(c.nitems, per_cu * cus));
     B->cls.push_back(c);
   }
   if ((rc = hip_rc(hipMemcpy(B->d_sys, B->hsys.data(), B->hsys.size() * sizeof(BatchSys),
@@ -734,6 +736,15 @@ int necg_batch_set_tiling(necg_batch *B, double item_flops, int32_t grid_per_cu)
   DevGuard g(B->device);
   B->item_flops = item_flops;
   B->grid_per_cu = grid_per_cu;
+  return plan(B);
+}
+
+int necg_batch_set_launch_cus(necg_batch *B, int32_t ncu) {
+  if (!B || ncu < 0) return NECG_ERR_ARG;
+  std::lock_guard<std::mutex> lk(B->mu);
+  if (B->pins) return NECG_ERR_ARG;
+  DevGuard g(B->device);
+  B->launch_cus = ncu;
   return plan(B);
 }
 

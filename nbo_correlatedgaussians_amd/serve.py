@@ -94,7 +94,8 @@ class Server:
 
     def __init__(self, systems, xs, device=None, mem_budget: int = 4 << 30,
                  max_trials: int = 512, trial_streams: int = 2, eval_streams: int = 2,
-                 item_flops: float = 0.0, tune: bool = False, factors: bool = True):
+                 item_flops: float = 0.0, tune: bool = False, factors: bool = True,
+                 reserve_cus: int = 0):
         torch = _torch()
         from .batch import Batch, autotune
         self.device = torch.device(device if device is not None else "cuda:0")
@@ -112,6 +113,9 @@ class Server:
         self.xb = torch.as_tensor(self.batch.pack(self.xs), device=self.device)
         if tune:                                # pair-tile size timed on these systems
             autotune(self.batch, self.xb.cpu().numpy(), reps=2)
+        if reserve_cus > 0:                     # CUs that full evaluations leave to trials
+            ncu = torch.cuda.get_device_properties(self.device).multi_processor_count
+            self.batch.set_launch_cus(max(1, ncu - int(reserve_cus)))
         self.batch.outprod_t(self.xb)           # resident state of every system
         torch.cuda.synchronize(self.device)
         self.batch.check("initial evaluation")

@@ -101,6 +101,42 @@ def test_batch_graph_replays_and_tilings():
         assert np.max(np.abs(gi - g2)) <= 1e-11 * max(1.0, np.max(np.abs(gi)))
 
 
+def test_launch_cus_keeps_results():
+    """Persistent pair launches sized for fewer CUs (as a serving loop keeps
+    CUs for its trials): every partial keeps its slot, so E and g of whole and
+    masked evaluations are bitwise those of the full-width launches; a pin
+    refuses the re-plan."""
+    import torch
+    from nbo_correlatedgaussians_amd import _lib
+    from nbo_correlatedgaussians_amd.batch import Batch, BatchGraph
+    cases = _mixed_small()
+    systems = [c[0] for c in cases]
+    Ks = [c[1].size // (c[0].blk + 1) for c in cases]
+    b = Batch(systems, Ks, "cuda:0")
+    xs = [c[1] for c in cases]
+    xd = torch.as_tensor(b.pack(xs), device="cuda:0")
+    mask = torch.zeros(len(systems), dtype=torch.int32, device="cuda:0")
+    mask[1] = mask[4] = 1
+    ref = b.outprod(xs)
+    fm, gm = b.outprod_t(xd, active=mask)
+    ref_m = (fm.cpu().numpy().copy(), gm.cpu().numpy().copy())
+    for ncu in (200, 17, 1):
+        b.set_launch_cus(ncu)
+        for (E, gi), (E2, g2) in zip(ref, b.outprod(xs)):
+            assert E == E2 and np.array_equal(gi, g2), ncu
+        fm, gm = b.outprod_t(xd, active=mask)
+        b.check()
+        for i in (1, 4):
+            lo, hi = b.offsets[i], b.offsets[i + 1]
+            assert fm[i].item() == ref_m[0][i], (ncu, i)
+            assert np.array_equal(gm[lo:hi].cpu().numpy(), ref_m[1][lo:hi]), (ncu, i)
+    b.set_launch_cus(0)
+    G = BatchGraph(b, b.pack(xs))
+    with pytest.raises(_lib.NecgError):
+        b.set_launch_cus(100)
+    G.release()
+
+
 def test_mixed_64_systems_config():
     """64 systems (He/H2/LiH, K log-uniform in [32, 1024], seeds 100..163):
     a sample of systems against their single-system evaluation."""
