@@ -1,4 +1,5 @@
-"""Write the full-size fixture of the LiH K = 2048 evaluation (seed 1).
+"""Write the full-size fixture of the LiH K = 2048 evaluation (seed 1), or of
+another K and seed (K = 4096 and K = 256 are stored too).
 
     python tests/golden/make_k2048_oracle.py [--K 2048] [--seed 1] [--threads 0]
 
@@ -16,7 +17,8 @@ the long-double sums.
 The oracle is pinned to the reference at small sizes by the golden vectors
 of make_golden.py (E within 2e-14, g within 1e-12) and by the sampled K = 2048
 pair sums of golden_lih_k2048_s1_pairs.npz; this file extends that pin to the
-whole evaluation at the headline size.  About 7 minutes on 8 cores.
+whole evaluation at the headline size.  About 7 minutes on 8 cores at
+K = 2048, 30 minutes at K = 4096.
 """
 
 from __future__ import annotations

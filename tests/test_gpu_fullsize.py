@@ -1,6 +1,6 @@
-"""Full-size parity: the whole energy and gradient of LiH K = 2048 (seed 1)
-and K = 256 (seed 0) against the oracle's evaluation of the same x, stored
-by tests/golden/make_k2048_oracle.py.
+"""Full-size parity: the whole energy and gradient of LiH K = 4096 and
+K = 2048 (seed 1) and K = 256 (seed 0) against the oracle's evaluation of the
+same x, stored by tests/golden/make_k2048_oracle.py.
 
 The stored x is regenerated here and must hash to the stored SHA-256.  The
 reference value is the oracle's evaluation with every sum in long double
@@ -33,7 +33,7 @@ def _fixture(K, seed):
     return s, x, d
 
 
-@pytest.mark.parametrize("K,seed", [(256, 0), (2048, 1)])
+@pytest.mark.parametrize("K,seed", [(256, 0), (2048, 1), (4096, 1)])
 def test_full_evaluation_vs_oracle_fixture(K, seed):
     import torch
     from nbo_correlatedgaussians_amd.autograd import ecg_energy
