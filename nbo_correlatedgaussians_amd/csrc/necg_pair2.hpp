@@ -213,12 +213,19 @@ __device__ __forceinline__ double lane_xchg_add(double x, double y, bool upper) 
 }
 
 // Sum over the 64 lanes of v[0..15]; lane j returns the sum of v[j >> 2].
-__device__ __forceinline__ double wave_sum16(const double *v, int lane) {
-  double w[8], u[4], t[2];
+// In two parts: wave_sum16_a halves across lane bits 5 and 4 (16 values to
+// 4 per lane, the steps with the most independent work), wave_sum16_b
+// finishes within each row of 16 lanes.
+__device__ __forceinline__ void wave_sum16_a(const double *v, double *u) {
+  double w[8];
 #pragma unroll
   for (int j = 0; j < 8; ++j) w[j] = lane_swap32_add(v[j], v[j + 8]);   // lane bit 5
 #pragma unroll
   for (int j = 0; j < 4; ++j) u[j] = lane_swap16_add(w[j], w[j + 4]);   // lane bit 4
+}
+
+__device__ __forceinline__ double wave_sum16_b(const double *u, int lane) {
+  double t[2];
   const bool b3 = (lane & 8) != 0, b2 = (lane & 4) != 0;
 #pragma unroll
   for (int j = 0; j < 2; ++j) t[j] = lane_xchg_add<0x140>(u[j], u[j + 2], b3);  // row mirror
@@ -228,26 +235,27 @@ __device__ __forceinline__ double wave_sum16(const double *v, int lane) {
   return r;
 }
 
+__device__ __forceinline__ double wave_sum16(const double *v, int lane) {
+  double u[4];
+  wave_sum16_a(v, u);
+  return wave_sum16_b(u, lane);
+}
+
 // One (bra, ket-under-term) evaluation, both sides.  kd: the ket record;
 // mb, mk: 1 where this lane's bra-side / ket-side contribution counts, else
-// 0; pw: this lane's four target slots (one byte per chunk, 255 = none) in
-// the ket's accumulator record for the values it holds after each chunk sum.
-// Adds sc*ov*alpha to hel and sc*ov to sel, the bra-side gradient (weight
-// scvv_P C_l) to the lane's LDS slots and the ket-side gradient (weight
-// scvv_P C_k, summed over the wave) to the ket accumulator kacc.  R1: the
-// mass matrix is diag(mdiag) + mc 1 1^T (SysConst::mrank1), so M a1 takes
-// one column sum instead of a full product.
-// FOLD (a sweep of a folded diagonal block): kd is this lane's ket record
-// (one of the sweep's two), the lane's ket-side values go to kacc where
-// mA = 1 and to kaccB where mA = 0 (two wave sums), and the bra side's two
-// weight sums, sigma_H and sigma_S, are added to sig[0], sig[1] (registers)
-// instead of their LDS slots, which the fold tile gives to the second ket.
+// 0.  Adds sc*ov*alpha to hel and sc*ov to sel and the bra-side gradient
+// (weight scvv_P C_l) to the lane's LDS slots, and writes this lane's
+// ket-side values (weight scvv_P C_k) to kv[0 .. Two<N>::KV), which ket_sums
+// adds up over the wave.  R1: the mass matrix is diag(mdiag) + mc 1 1^T
+// (SysConst::mrank1), so M a1 takes one column sum instead of a full product.
+// FOLD (a sweep of a folded diagonal block): the bra side's two weight sums,
+// sigma_H and sigma_S, are added to sig[0], sig[1] (registers) instead of
+// their LDS slots, which the fold tile gives to the second ket.
 template <int N, bool R1, bool FOLD = false>
 __device__ __forceinline__ bool pair_term2(const BraLds<N> &b, const double *__restrict__ kd,
                                            const SysConst &sc_, double mb, double mk,
-                                           unsigned pw, double &hel, double &sel, LdsSink &sink,
-                                           double *kacc, int lane, double mA = 1.0,
-                                           double *kaccB = nullptr, double *sig = nullptr) {
+                                           double &hel, double &sel, LdsSink &sink, double *kv,
+                                           double *sig = nullptr) {
   using D = Dim<N>;
   constexpr int NS = D::NS, N3 = D::N3;
   const double *__restrict__ Ap = kd + D::K_AP;
@@ -464,23 +472,29 @@ __device__ __forceinline__ bool pair_term2(const BraLds<N> &b, const double *__r
       }
   }
 
-  // Bra side: Phi = Phk + 8 M a1, G = B^-1 (12 A'MA' + La) B^-1.
+  // Bra side: Phi = Phk + 8 M a1, G = B^-1 U B^-1 with U = 12 A'MA' + La.
+  // With e = 2 wo dif1 (the Y_S values) the slots are
+  //   X_S = -3 wo B^-1 - sum_x e_i d_j,   X_H = alpha X_S + wo (G - dp),
+  //   Y_H = alpha e + wo Phi,             Y_S = e,
+  // dp_ij = <d_i, Phi_j> + <Phi_i, d_j>; G row by row as (B^-1 U)_i B^-1.
   {
-    double Phi[N3], d1[N3];
+    double Phi[N3], d1[N3], e1[N3];
+    const double wo = wb * ov;
+    const double wo2 = 2.0 * wo;
 #pragma unroll
     for (int t = 0; t < N3; ++t) {
       Phi[t] = __builtin_fma(8.0, Ma1[t], Phk[t]);
       d1[t] = st[t] - b.S(t);
+      e1[t] = wo2 * d1[t];
     }
-    const double wo = wb * ov;
-    const double woa = wo * alpha;
     // the shift and weight slots first: their LDS adds then drain under the
     // matrix products below instead of in front of the ket side's LDS reads
 #pragma unroll
     for (int t = 0; t < N3; ++t) {
-      sink.add(D::A_YH + t, __builtin_fma(2.0 * woa, d1[t], wo * Phi[t]));
-      sink.add(D::A_YS + t, 2.0 * wo * d1[t]);
+      sink.add(D::A_YH + t, __builtin_fma(alpha, e1[t], wo * Phi[t]));
+      sink.add(D::A_YS + t, e1[t]);
     }
+    const double woa = wo * alpha;
     if constexpr (FOLD) {
       sig[0] += woa;
       sig[1] += wo;
@@ -488,113 +502,148 @@ __device__ __forceinline__ bool pair_term2(const BraLds<N> &b, const double *__r
       sink.add(D::A_SH, woa);
       sink.add(D::A_SS, wo);
     }
-    double Tm[N * N];
+    double U[NS];
 #pragma unroll
-    for (int i = 0; i < N; ++i)
+    for (int t = 0; t < NS; ++t) U[t] = __builtin_fma(12.0, Np[t], La[t]);
+    const double m3wo = -3.0 * wo;
 #pragma unroll
-      for (int j = 0; j < N; ++j) {
-        double t = 0.0;
+    for (int i = 0; i < N; ++i) {
+      double V[N];   // row i of B^-1 U
 #pragma unroll
-        for (int m = 0; m < N; ++m)
-          t = __builtin_fma(__builtin_fma(12.0, Np[sx(i, m)], La[sx(i, m)]), Bi[sx(m, j)], t);
-        Tm[i * N + j] = t;
+      for (int c = 0; c < N; ++c) {
+        double v = Bi[sx(i, 0)] * U[sx(0, c)];
+#pragma unroll
+        for (int m = 1; m < N; ++m) v = __builtin_fma(Bi[sx(i, m)], U[sx(m, c)], v);
+        V[c] = v;
       }
-#pragma unroll
-    for (int i = 0; i < N; ++i)
 #pragma unroll
       for (int j = 0; j <= i; ++j) {
-        double g = 0.0;
+        double g = V[0] * Bi[sx(0, j)];
 #pragma unroll
-        for (int m = 0; m < N; ++m) g = __builtin_fma(Bi[sx(i, m)], Tm[m * N + j], g);
-        double dd = 0.0, dp = 0.0;
+        for (int m = 1; m < N; ++m) g = __builtin_fma(V[m], Bi[sx(m, j)], g);
+        double xs = m3wo * Bi[sx(i, j)];
 #pragma unroll
         for (int x = 0; x < 3; ++x) {
-          dd = __builtin_fma(d1[3 * i + x], d1[3 * j + x], dd);
-          dp = __builtin_fma(d1[3 * i + x], Phi[3 * j + x], dp);
-          dp = __builtin_fma(Phi[3 * i + x], d1[3 * j + x], dp);
+          g = __builtin_fma(-d1[3 * i + x], Phi[3 * j + x], g);
+          g = __builtin_fma(-Phi[3 * i + x], d1[3 * j + x], g);
+          xs = __builtin_fma(-e1[3 * i + x], d1[3 * j + x], xs);
         }
-        const double x0 = __builtin_fma(-3.0, Bi[sx(i, j)], -2.0 * dd);
         const int t = sx(i, j);
-        sink.add(D::A_XH + t, __builtin_fma(woa, x0, wo * (g - dp)));
-        sink.add(D::A_XS + t, wo * x0);
+        sink.add(D::A_XH + t, __builtin_fma(alpha, xs, wo * g));
+        sink.add(D::A_XS + t, xs);
       }
+    }
   }
 
-  // Ket side: Phi' = Phk, G' = B^-1 (12 A M A + La) B^-1, dif2 = st - S'.
+  // Ket side: Phi' = Phk, G' = B^-1 U' B^-1 with U' = 12 A M A + La,
+  // dif2 = st - S', the slots as on the bra side, into kv (summed over the
+  // wave by ket_sums).
   {
     const double wk = mk * scv * b.f(BraLds<N>::F_C);   // scvv_P C_k
-    double kv[Two<N>::KV];
-    double d2[N3];
-#pragma unroll
-    for (int t = 0; t < N3; ++t) d2[t] = st[t] - Sp[t];
-    double Tm[N * N];
-#pragma unroll
-    for (int i = 0; i < N; ++i)
-#pragma unroll
-      for (int j = 0; j < N; ++j) {
-        double t = 0.0;
-#pragma unroll
-        for (int m = 0; m < N; ++m)
-          t = __builtin_fma(__builtin_fma(12.0, b.AMA(sx(i, m)), La[sx(i, m)]), Bi[sx(m, j)], t);
-        Tm[i * N + j] = t;
-      }
     const double wo = wk * ov;
-    const double woa = wo * alpha;
-#pragma unroll
-    for (int i = 0; i < N; ++i)
-#pragma unroll
-      for (int j = 0; j <= i; ++j) {
-        double g = 0.0;
-#pragma unroll
-        for (int m = 0; m < N; ++m) g = __builtin_fma(Bi[sx(i, m)], Tm[m * N + j], g);
-        double dd = 0.0, dp = 0.0;
-#pragma unroll
-        for (int x = 0; x < 3; ++x) {
-          dd = __builtin_fma(d2[3 * i + x], d2[3 * j + x], dd);
-          dp = __builtin_fma(d2[3 * i + x], Phk[3 * j + x], dp);
-          dp = __builtin_fma(Phk[3 * i + x], d2[3 * j + x], dp);
-        }
-        const double x0 = __builtin_fma(-3.0, Bi[sx(i, j)], -2.0 * dd);
-        const int t = sx(i, j);
-        kv[2 * t] = __builtin_fma(woa, x0, wo * (g - dp));
-        kv[2 * t + 1] = wo * x0;
-      }
+    const double wo2 = 2.0 * wo;
+    double d2[N3], e2[N3];
 #pragma unroll
     for (int t = 0; t < N3; ++t) {
-      kv[2 * NS + 2 * t] = __builtin_fma(2.0 * woa, d2[t], wo * Phk[t]);
-      kv[2 * NS + 2 * t + 1] = 2.0 * wo * d2[t];
+      d2[t] = st[t] - Sp[t];
+      e2[t] = wo2 * d2[t];
     }
-    kv[2 * NS + 2 * N3] = woa;
+    double U[NS];
+#pragma unroll
+    for (int t = 0; t < NS; ++t) U[t] = __builtin_fma(12.0, b.AMA(t), La[t]);
+    const double m3wo = -3.0 * wo;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      double V[N];   // row i of B^-1 U'
+#pragma unroll
+      for (int c = 0; c < N; ++c) {
+        double v = Bi[sx(i, 0)] * U[sx(0, c)];
+#pragma unroll
+        for (int m = 1; m < N; ++m) v = __builtin_fma(Bi[sx(i, m)], U[sx(m, c)], v);
+        V[c] = v;
+      }
+#pragma unroll
+      for (int j = 0; j <= i; ++j) {
+        double g = V[0] * Bi[sx(0, j)];
+#pragma unroll
+        for (int m = 1; m < N; ++m) g = __builtin_fma(V[m], Bi[sx(m, j)], g);
+        double xs = m3wo * Bi[sx(i, j)];
+#pragma unroll
+        for (int x = 0; x < 3; ++x) {
+          g = __builtin_fma(-d2[3 * i + x], Phk[3 * j + x], g);
+          g = __builtin_fma(-Phk[3 * i + x], d2[3 * j + x], g);
+          xs = __builtin_fma(-e2[3 * i + x], d2[3 * j + x], xs);
+        }
+        const int t = sx(i, j);
+        kv[2 * t] = __builtin_fma(alpha, xs, wo * g);
+        kv[2 * t + 1] = xs;
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < N3; ++t) {
+      kv[2 * NS + 2 * t] = __builtin_fma(alpha, e2[t], wo * Phk[t]);
+      kv[2 * NS + 2 * t + 1] = e2[t];
+    }
+    kv[2 * NS + 2 * N3] = wo * alpha;
     kv[2 * NS + 2 * N3 + 1] = wo;
 #pragma unroll
     for (int t = Two<N>::NV; t < Two<N>::KV; ++t) kv[t] = 0.0;
-#pragma unroll
-    for (int ch = 0; ch < Two<N>::NCH; ++ch) {
-      const unsigned tgt = (pw >> (8 * ch)) & 255u;
-      if constexpr (FOLD) {
-        // split by ket: a = this lane's values where it pairs with the first
-        // ket, c = the rest (exact: mA is 0 or 1)
-        double a[16], c[16];
-#pragma unroll
-        for (int i = 0; i < 16; ++i) {
-          a[i] = kv[16 * ch + i] * mA;
-          c[i] = kv[16 * ch + i] - a[i];
-        }
-        const double ra = wave_sum16(a, lane);
-        const double rc = wave_sum16(c, lane);
-        if ((lane & 3) == 0 && tgt != 255u) {
-          __hip_atomic_fetch_add(kacc + tgt, ra, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
-          __hip_atomic_fetch_add(kaccB + tgt, rc, __ATOMIC_RELAXED,
-                                 __HIP_MEMORY_SCOPE_WAVEFRONT);
-        }
-      } else {
-        const double r = wave_sum16(kv + 16 * ch, lane);
-        if ((lane & 3) == 0 && tgt != 255u)
-          __hip_atomic_fetch_add(kacc + tgt, r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
-      }
-    }
   }
   return ok;
+}
+
+// The ket side of one term summed over the wave into the ket accumulator,
+// in two parts.  ket_sums_a (at the end of the term) halves each chunk of 16
+// values to 4 per lane, into u[4 * NCH] (FOLD: first the values where this
+// lane pairs with the first ket, mA = 1, then the rest, u[8 * NCH]).
+// ket_sums_b (while the next term runs) finishes the sums and adds them: pw
+// holds this lane's four target slots (one byte per chunk, 255 = none), the
+// same for the four lanes of a quad, which after a chunk's sum all hold value
+// 16 c + (lane >> 2) of it.  Every lane adds, so that no branch splits the
+// caller's loop body: the first lane of a quad its value, the other three
+// +0.0 to the same slot, and where there is no target all four add +0.0 to
+// the spare slot 63, which is never read (the adds of +0.0 leave every sum
+// unchanged, in whatever order the LDS unit applies them).  FOLD: the first
+// ket's sums go to kacc, the second's to kaccB.
+template <int N, bool FOLD = false>
+__device__ __forceinline__ void ket_sums_a(const double *kv, double *u, double mA = 1.0) {
+#pragma unroll
+  for (int ch = 0; ch < Two<N>::NCH; ++ch) {
+    if constexpr (FOLD) {
+      // split by ket (exact: mA is 0 or 1)
+      double a[16], c[16];
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        a[i] = kv[16 * ch + i] * mA;
+        c[i] = kv[16 * ch + i] - a[i];
+      }
+      wave_sum16_a(a, u + 4 * ch);
+      wave_sum16_a(c, u + 4 * Two<N>::NCH + 4 * ch);
+    } else {
+      wave_sum16_a(kv + 16 * ch, u + 4 * ch);
+    }
+  }
+}
+
+template <int N, bool FOLD = false>
+__device__ __forceinline__ void ket_sums_b(const double *u, unsigned pw, double *kacc, int lane,
+                                           double *kaccB = nullptr) {
+  static_assert(Dim<N>::NACC < 64, "slot 63 is the spare");
+  const bool lead = (lane & 3) == 0;
+#pragma unroll
+  for (int ch = 0; ch < Two<N>::NCH; ++ch) {
+    const unsigned tg = (pw >> (8 * ch)) & 255u;
+    const bool has = tg != 255u;
+    const int slot = has ? (int)tg : 63;
+    const double r = wave_sum16_b(u + 4 * ch, lane);
+    __hip_atomic_fetch_add(kacc + slot, lead && has ? r : 0.0, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_WAVEFRONT);
+    if constexpr (FOLD) {
+      const double rc = wave_sum16_b(u + 4 * Two<N>::NCH + 4 * ch, lane);
+      __hip_atomic_fetch_add(kaccB + slot, lead && has ? rc : 0.0, __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_WAVEFRONT);
+    }
+  }
 }
 
 // One two-sided pair tile: bras [k0, k0+64) x kets [l0, l1) x terms [p0, p1),
@@ -661,16 +710,30 @@ __device__ __forceinline__ void pair2_tile(const double *__restrict__ bra,
     const double mb = valid ? 1.0 : 0.0;
     const double mk = (valid && k != l) ? 1.0 : 0.0;
     double hel = 0.0, sel = 0.0;
+    // the previous term's ket side, halved to 4 values per chunk, is summed
+    // while the next term runs (one basic block, so the two interleave);
+    // none before the first term
+    constexpr int NU = 4 * Two<N>::NCH;
+    double up[NU];
+#pragma unroll
+    for (int t = 0; t < NU; ++t) up[t] = 0.0;
+    unsigned pwp = 0xffffffffu;
     for (int p = p0; p < p1; ++p) {
       // the record after (l, p): (l, p + 1), else the wave's next ket at p0
       const int pn = p + 1 < p1 ? p + 1 : p0;
       const int ln = p + 1 < p1 ? l : l + W;
       double nx = 0.0;
       if (ln < l1 && lane < KF) nx = ket[((size_t)ln * nst + pn) * D::KS + lane];
-      ok &= pair_term2<N, R1>(b, kst, sc, mb, mk, tperm[p * 64 + lane], hel, sel, sink, kacc, lane);
+      const unsigned pw = tperm[p * 64 + lane];
+      ket_sums_b<N>(up, pwp, kacc, lane);
+      double kv[Two<N>::KV];
+      ok &= pair_term2<N, R1>(b, kst, sc, mb, mk, hel, sel, sink, kv);
+      ket_sums_a<N>(kv, up);
+      pwp = pw;
       // after every read of the term (the wave's LDS operations complete in order)
       if (lane < KF) kst[lane] = nx;
     }
+    ket_sums_b<N>(up, pwp, kacc, lane);
     if (valid) {
       const size_t o = upper ? (size_t)k * K + l : (size_t)l * K + k;
       hp[o] = hel;
@@ -783,6 +846,11 @@ __device__ __forceinline__ void pair2_fold_tile(const double *__restrict__ bra,
     const double mA = first ? 1.0 : 0.0;
     const double *kd = first ? kstA : kstB;
     double hel = 0.0, sel = 0.0;
+    constexpr int NU = 8 * Two<N>::NCH;   // the previous term's ket side, as in pair2_tile
+    double up[NU];
+#pragma unroll
+    for (int t = 0; t < NU; ++t) up[t] = 0.0;
+    unsigned pwp = 0xffffffffu;
     for (int p = p0; p < p1; ++p) {
       const int pn = p + 1 < p1 ? p + 1 : p0;
       const int sn = p + 1 < p1 ? s : s + W;
@@ -791,13 +859,18 @@ __device__ __forceinline__ void pair2_fold_tile(const double *__restrict__ bra,
         nxA = ket[((size_t)(k0 + sn) * nst + pn) * D::KS + lane];
         nxB = ket[((size_t)(k0 + sn + 32) * nst + pn) * D::KS + lane];
       }
-      ok &= pair_term2<N, R1, true>(b, kd, sc, 1.0, mk, tperm[p * 64 + lane], hel, sel, sink,
-                                    kaccA, lane, mA, kaccB, sig);
+      const unsigned pw = tperm[p * 64 + lane];
+      ket_sums_b<N, true>(up, pwp, kaccA, lane, kaccB);
+      double kv[Two<N>::KV];
+      ok &= pair_term2<N, R1, true>(b, kd, sc, 1.0, mk, hel, sel, sink, kv, sig);
+      ket_sums_a<N, true>(kv, up, mA);
+      pwp = pw;
       if (lane < KF) {   // after every read of the term (in-order LDS operations)
         kstA[lane] = nxA;
         kstB[lane] = nxB;
       }
     }
+    ket_sums_b<N, true>(up, pwp, kaccA, lane, kaccB);
     const size_t o = k < l ? (size_t)k * K + l : (size_t)l * K + k;
     hp[o] = hel;
     sp[o] = sel;
@@ -867,9 +940,11 @@ __device__ __forceinline__ void pair2_diag_tile(const double *__restrict__ bra,
   const double mb = lane >= 32 ? 1.0 : 0.0;
   double hel = 0.0, sel = 0.0;
   bool ok = true;
-  for (int p = p0 + wv; p < p1; p += W)
-    ok &= pair_term2<N, R1>(b, ket + ((size_t)k * nst + p) * D::KS, sc, mb, 0.0,
-                            tperm[p * 64 + lane], hel, sel, sink, kacc, lane);
+  for (int p = p0 + wv; p < p1; p += W) {
+    double kv[Two<N>::KV];   // the ket side is zero here and not summed
+    ok &= pair_term2<N, R1>(b, ket + ((size_t)k * nst + p) * D::KS, sc, mb, 0.0, hel, sel, sink,
+                            kv);
+  }
   if (lane >= 32 && !ok) atomicOr(flag, (int)F_NOT_PD);
   kst_s[wv * 64 + lane] = hel;   // the staging buffers are free here
   kacc[lane] = sel;

@@ -13,10 +13,10 @@ Two models:
 * ``flops_per_eval``: the two-sided algorithm the pair kernel runs: per
   unordered (pair, term) one factorisation, the elements, the Coulomb and
   Stark derivatives, and both gradient halves (``two_sided_term``; LiH,
-  whose mass matrix is diagonal plus a constant: 4,339 FLOP per pair-term,
-  104,136 per off-diagonal pair).  The identities
+  whose mass matrix is diagonal plus a constant: 4,083 FLOP per pair-term,
+  97,992 per off-diagonal pair).  The identities
   it uses (A' dif2 = -A dif1, B^-1 A' = I - B^-1 A, the collapsed kinetic and
-  Coulomb gradient products) make it 4.0x smaller than the v0 model for LiH,
+  Coulomb gradient products) make it 4.2x smaller than the v0 model for LiH,
   so the v0 model overstates the work of the run and is not the roofline
   basis.
 """
@@ -67,10 +67,10 @@ def two_sided_term(n: int, rank1: bool = False) -> dict:
     stark = 2 * n + 4 + 3 + 2                            # zsum, alpha, H and S sums
     shared = 2 * n3 * n + 2 * n + 2 * n3 * n + 2 * n3 + 2 * n3 * n   # Lb st, Stark column, A M a1,
     #                                                      F - 8 A M a1, B^-1 F (the ket side's Phi)
-    side = (n3                                           # dif
-            + 2 * ns + 2 * n * n * n                     # 12 A'MA' + La, times B^-1
-            + ns * (2 * n + 6 + 12 + 3 + 4 + 1)          # G, dd, dp, x0, X_H, X_S
-            + 3 + n3 * 6                                 # weights, Y_H, Y_S
+    side = (n3 + 5 + n3                                  # dif, the weights, e = 2 wo dif
+            + 3 * n3                                     # Y_H = alpha e + wo Phi (Y_S = e)
+            + 2 * ns + n * n * (2 * n - 1)               # U = 12 A'MA' + La, V = B^-1 U
+            + ns * ((2 * n - 1) + 12 + 7 + 3)            # G = V B^-1 less dp, X_S, X_H
             + (2 * ns + 2 * n3 + 2))                     # accumulation of the values
     return {"factorisation": chol + inv + det, "elements": elem, "kinetic": kin,
             "coulomb": coul, "stark": stark, "shared_gradient": shared,
