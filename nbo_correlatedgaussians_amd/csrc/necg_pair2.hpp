@@ -224,21 +224,33 @@ __device__ __forceinline__ void wave_sum16_a(const double *v, double *u) {
   for (int j = 0; j < 4; ++j) u[j] = lane_swap16_add(w[j], w[j + 4]);   // lane bit 4
 }
 
-__device__ __forceinline__ double wave_sum16_b(const double *u, int lane) {
+// The row steps after wave_sum16_a: each lane then holds a partial of value
+// lane >> 2 of the chunk, to be summed over the four lanes of its quad.
+__device__ __forceinline__ double wave_sum16_rows(const double *u, int lane) {
   double t[2];
   const bool b3 = (lane & 8) != 0, b2 = (lane & 4) != 0;
 #pragma unroll
   for (int j = 0; j < 2; ++j) t[j] = lane_xchg_add<0x140>(u[j], u[j + 2], b3);  // row mirror
-  double r = lane_xchg_add<0x141>(t[0], t[1], b2);                     // half-row mirror
-  r += lane_dpp<0x4E>(r);   // quad_perm [2,3,0,1]: lane bit 1
-  r += lane_dpp<0xB1>(r);   // quad_perm [1,0,3,2]: lane bit 0
-  return r;
+  return lane_xchg_add<0x141>(t[0], t[1], b2);                         // half-row mirror
 }
 
 __device__ __forceinline__ double wave_sum16(const double *v, int lane) {
   double u[4];
   wave_sum16_a(v, u);
-  return wave_sum16_b(u, lane);
+  double r = wave_sum16_rows(u, lane);
+  r += lane_dpp<0x4E>(r);   // quad_perm [2,3,0,1]: lane bit 1
+  r += lane_dpp<0xB1>(r);   // quad_perm [1,0,3,2]: lane bit 0
+  return r;
+}
+
+// Four chunks' quad partials r[0..3] summed over each quad with a
+// transposing exchange: lane q of a quad (q = lane & 3) returns the full sum
+// of r[q], so the four lanes hold four different chunks' values.
+__device__ __forceinline__ double quad_sum4(const double *r, int lane) {
+  const bool b1 = (lane & 2) != 0, b0 = (lane & 1) != 0;
+  const double s0 = lane_xchg_add<0x4E>(r[0], r[2], b1);   // quad_perm [2,3,0,1]
+  const double s1 = lane_xchg_add<0x4E>(r[1], r[3], b1);
+  return lane_xchg_add<0xB1>(s0, s1, b0);                  // quad_perm [1,0,3,2]
 }
 
 // One (bra, ket-under-term) evaluation, both sides.  kd: the ket record;
@@ -596,14 +608,14 @@ __device__ __forceinline__ bool pair_term2(const BraLds<N> &b, const double *__r
 // in two parts.  ket_sums_a (at the end of the term) halves each chunk of 16
 // values to 4 per lane, into u[4 * NCH] (FOLD: first the values where this
 // lane pairs with the first ket, mA = 1, then the rest, u[8 * NCH]).
-// ket_sums_b (while the next term runs) finishes the sums and adds them: pw
-// holds this lane's four target slots (one byte per chunk, 255 = none), the
-// same for the four lanes of a quad, which after a chunk's sum all hold value
-// 16 c + (lane >> 2) of it.  Every lane adds, so that no branch splits the
-// caller's loop body: the first lane of a quad its value, the other three
-// +0.0 to the same slot, and where there is no target all four add +0.0 to
-// the spare slot 63, which is never read (the adds of +0.0 leave every sum
-// unchanged, in whatever order the LDS unit applies them).  FOLD: the first
+// ket_sums_b (while the next term runs) finishes the sums within each row of
+// 16 lanes, then sums the quads with a transposing exchange over the (at most
+// four) chunks, after which lane j holds value 16 (j & 3) + (j >> 2): its
+// target slot is byte j & 3 of pw (this lane's four target slots, one byte
+// per chunk, 255 = none).  Every lane adds its own value with one LDS add, so
+// that no branch splits the caller's loop body; where there is no target the
+// lane adds +0.0 to the spare slot 63, which is never read (those adds leave
+// it at zero in whatever order the LDS unit applies them).  FOLD: the first
 // ket's sums go to kacc, the second's to kaccB.
 template <int N, bool FOLD = false>
 __device__ __forceinline__ void ket_sums_a(const double *kv, double *u, double mA = 1.0) {
@@ -629,20 +641,23 @@ template <int N, bool FOLD = false>
 __device__ __forceinline__ void ket_sums_b(const double *u, unsigned pw, double *kacc, int lane,
                                            double *kaccB = nullptr) {
   static_assert(Dim<N>::NACC < 64, "slot 63 is the spare");
-  const bool lead = (lane & 3) == 0;
+  static_assert(Two<N>::NCH <= 4, "one transposing quad sum");
+  const unsigned tg = (pw >> (8 * (lane & 3))) & 255u;
+  const bool has = tg != 255u;
+  const int slot = has ? (int)tg : 63;
+  double r[4];
 #pragma unroll
-  for (int ch = 0; ch < Two<N>::NCH; ++ch) {
-    const unsigned tg = (pw >> (8 * ch)) & 255u;
-    const bool has = tg != 255u;
-    const int slot = has ? (int)tg : 63;
-    const double r = wave_sum16_b(u + 4 * ch, lane);
-    __hip_atomic_fetch_add(kacc + slot, lead && has ? r : 0.0, __ATOMIC_RELAXED,
+  for (int ch = 0; ch < 4; ++ch) r[ch] = ch < Two<N>::NCH ? wave_sum16_rows(u + 4 * ch, lane) : 0.0;
+  const double v = quad_sum4(r, lane);
+  __hip_atomic_fetch_add(kacc + slot, has ? v : 0.0, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_WAVEFRONT);
+  if constexpr (FOLD) {
+#pragma unroll
+    for (int ch = 0; ch < 4; ++ch)
+      r[ch] = ch < Two<N>::NCH ? wave_sum16_rows(u + 4 * Two<N>::NCH + 4 * ch, lane) : 0.0;
+    const double vb = quad_sum4(r, lane);
+    __hip_atomic_fetch_add(kaccB + slot, has ? vb : 0.0, __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_WAVEFRONT);
-    if constexpr (FOLD) {
-      const double rc = wave_sum16_b(u + 4 * Two<N>::NCH + 4 * ch, lane);
-      __hip_atomic_fetch_add(kaccB + slot, lead && has ? rc : 0.0, __ATOMIC_RELAXED,
-                             __HIP_MEMORY_SCOPE_WAVEFRONT);
-    }
   }
 }
 
