@@ -613,10 +613,10 @@ __device__ __forceinline__ bool pair_term2(const BraLds<N> &b, const double *__r
 // four) chunks, after which lane j holds value 16 (j & 3) + (j >> 2): its
 // target slot is byte j & 3 of pw (this lane's four target slots, one byte
 // per chunk, 255 = none).  Every lane adds its own value with one LDS add, so
-// that no branch splits the caller's loop body; where there is no target the
-// lane adds +0.0 to the spare slot 63, which is never read (those adds leave
-// it at zero in whatever order the LDS unit applies them).  FOLD: the first
-// ket's sums go to kacc, the second's to kaccB.
+// that no branch splits the caller's loop body.  A lane without a target holds
+// the sum of a padding value (kv zero there, so the sum is +0.0 exactly, or a
+// chunk past NCH, also +0.0) and adds it to the spare slot 63, which is never
+// read.  FOLD: the first ket's sums go to kacc, the second's to kaccB.
 template <int N, bool FOLD = false>
 __device__ __forceinline__ void ket_sums_a(const double *kv, double *u, double mA = 1.0) {
 #pragma unroll
@@ -649,15 +649,13 @@ __device__ __forceinline__ void ket_sums_b(const double *u, unsigned pw, double 
 #pragma unroll
   for (int ch = 0; ch < 4; ++ch) r[ch] = ch < Two<N>::NCH ? wave_sum16_rows(u + 4 * ch, lane) : 0.0;
   const double v = quad_sum4(r, lane);
-  __hip_atomic_fetch_add(kacc + slot, has ? v : 0.0, __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_WAVEFRONT);
+  __hip_atomic_fetch_add(kacc + slot, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
   if constexpr (FOLD) {
 #pragma unroll
     for (int ch = 0; ch < 4; ++ch)
       r[ch] = ch < Two<N>::NCH ? wave_sum16_rows(u + 4 * Two<N>::NCH + 4 * ch, lane) : 0.0;
     const double vb = quad_sum4(r, lane);
-    __hip_atomic_fetch_add(kaccB + slot, has ? vb : 0.0, __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_WAVEFRONT);
+    __hip_atomic_fetch_add(kaccB + slot, vb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
   }
 }
 
