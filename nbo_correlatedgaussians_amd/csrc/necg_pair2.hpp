@@ -24,9 +24,10 @@
 //
 // Cross-lane sums.  Lanes are 64 bras of one ket, so the ket side is summed
 // over the wave: 16 values at a time by a halving exchange (permlane32 swap,
-// permlane16 swap, row mirror, half-row mirror, then two quad exchanges),
-// after which lane j holds the full sum of value j >> 2 of the chunk.  Every
-// sum is formed in a fixed order, so results are bitwise reproducible.
+// permlane16 swap, row mirror, half-row mirror), then the quads of the four
+// chunks together by a transposing exchange, after which every lane holds a
+// different value's full sum.  Every sum is formed in a fixed order, so
+// results are bitwise reproducible.
 #pragma once
 
 #include "necg_pair.hpp"
@@ -212,10 +213,10 @@ __device__ __forceinline__ double lane_xchg_add(double x, double y, bool upper) 
   return keep + lane_dpp<CTRL>(send);
 }
 
-// Sum over the 64 lanes of v[0..15]; lane j returns the sum of v[j >> 2].
-// In two parts: wave_sum16_a halves across lane bits 5 and 4 (16 values to
-// 4 per lane, the steps with the most independent work), wave_sum16_b
-// finishes within each row of 16 lanes.
+// The halving exchange of 16 values over the wave: wave_sum16_a across lane
+// bits 5 and 4 (16 values to 4 per lane), wave_sum16_rows within each row of
+// 16 lanes (4 values to 1: a partial of value lane >> 2, to be summed over
+// the lane's quad).
 __device__ __forceinline__ void wave_sum16_a(const double *v, double *u) {
   double w[8];
 #pragma unroll
@@ -232,15 +233,6 @@ __device__ __forceinline__ double wave_sum16_rows(const double *u, int lane) {
 #pragma unroll
   for (int j = 0; j < 2; ++j) t[j] = lane_xchg_add<0x140>(u[j], u[j + 2], b3);  // row mirror
   return lane_xchg_add<0x141>(t[0], t[1], b2);                         // half-row mirror
-}
-
-__device__ __forceinline__ double wave_sum16(const double *v, int lane) {
-  double u[4];
-  wave_sum16_a(v, u);
-  double r = wave_sum16_rows(u, lane);
-  r += lane_dpp<0x4E>(r);   // quad_perm [2,3,0,1]: lane bit 1
-  r += lane_dpp<0xB1>(r);   // quad_perm [1,0,3,2]: lane bit 0
-  return r;
 }
 
 // Four chunks' quad partials r[0..3] summed over each quad with a
@@ -604,23 +596,31 @@ __device__ __forceinline__ bool pair_term2(const BraLds<N> &b, const double *__r
   return ok;
 }
 
-// The ket side of one term summed over the wave into the ket accumulator,
-// in two parts.  ket_sums_a (at the end of the term) halves each chunk of 16
-// values to 4 per lane, into u[4 * NCH] (FOLD: first the values where this
-// lane pairs with the first ket, mA = 1, then the rest, u[8 * NCH]).
-// ket_sums_b (while the next term runs) finishes the sums within each row of
-// 16 lanes, then sums the quads with a transposing exchange over the (at most
-// four) chunks, after which lane j holds value 16 (j & 3) + (j >> 2): its
-// target slot is byte j & 3 of pw (this lane's four target slots, one byte
-// per chunk, 255 = none).  Every lane adds its own value with one LDS add, so
-// that no branch splits the caller's loop body.  A lane without a target holds
-// the sum of a padding value (kv zero there, so the sum is +0.0 exactly, or a
+// The ket side of one term summed over the wave into the ket accumulator.
+// Each chunk of 16 values is halved across lane bits 5 and 4 and then within
+// each row of 16 lanes, leaving a partial per lane that its quad sums; the
+// quads are summed with a transposing exchange over the (at most four)
+// chunks, after which lane j holds value 16 (j & 3) + (j >> 2): its target
+// slot is byte j & 3 of pw (this lane's four target slots, one byte per chunk,
+// 255 = none).  Every lane adds its own value with one LDS add, so that no
+// branch splits the caller's loop body.  A lane without a target holds the
+// sum of a padding value (kv zero there, so the sum is +0.0 exactly, or a
 // chunk past NCH, also +0.0) and adds it to the spare slot 63, which is never
-// read.  FOLD: the first ket's sums go to kacc, the second's to kaccB.
+// read.  FOLD: the lane's values go to kacc where mA = 1 and to kaccB where
+// mA = 0 (two sums per chunk).
 template <int N, bool FOLD = false>
-__device__ __forceinline__ void ket_sums_a(const double *kv, double *u, double mA = 1.0) {
+__device__ __forceinline__ void ket_sums(const double *kv, unsigned pw, double *kacc, int lane,
+                                         double mA = 1.0, double *kaccB = nullptr) {
+  static_assert(Dim<N>::NACC < 64, "slot 63 is the spare");
+  static_assert(Two<N>::NCH <= 4, "one transposing quad sum");
+  const unsigned tg = (pw >> (8 * (lane & 3))) & 255u;
+  const int slot = tg != 255u ? (int)tg : 63;
+  double r[4], rc[4];
 #pragma unroll
-  for (int ch = 0; ch < Two<N>::NCH; ++ch) {
+  for (int ch = 0; ch < 4; ++ch) {
+    r[ch] = rc[ch] = 0.0;
+    if (ch >= Two<N>::NCH) continue;
+    double u[4];
     if constexpr (FOLD) {
       // split by ket (exact: mA is 0 or 1)
       double a[16], c[16];
@@ -629,34 +629,20 @@ __device__ __forceinline__ void ket_sums_a(const double *kv, double *u, double m
         a[i] = kv[16 * ch + i] * mA;
         c[i] = kv[16 * ch + i] - a[i];
       }
-      wave_sum16_a(a, u + 4 * ch);
-      wave_sum16_a(c, u + 4 * Two<N>::NCH + 4 * ch);
+      wave_sum16_a(a, u);
+      r[ch] = wave_sum16_rows(u, lane);
+      wave_sum16_a(c, u);
+      rc[ch] = wave_sum16_rows(u, lane);
     } else {
-      wave_sum16_a(kv + 16 * ch, u + 4 * ch);
+      wave_sum16_a(kv + 16 * ch, u);
+      r[ch] = wave_sum16_rows(u, lane);
     }
   }
-}
-
-template <int N, bool FOLD = false>
-__device__ __forceinline__ void ket_sums_b(const double *u, unsigned pw, double *kacc, int lane,
-                                           double *kaccB = nullptr) {
-  static_assert(Dim<N>::NACC < 64, "slot 63 is the spare");
-  static_assert(Two<N>::NCH <= 4, "one transposing quad sum");
-  const unsigned tg = (pw >> (8 * (lane & 3))) & 255u;
-  const bool has = tg != 255u;
-  const int slot = has ? (int)tg : 63;
-  double r[4];
-#pragma unroll
-  for (int ch = 0; ch < 4; ++ch) r[ch] = ch < Two<N>::NCH ? wave_sum16_rows(u + 4 * ch, lane) : 0.0;
-  const double v = quad_sum4(r, lane);
-  __hip_atomic_fetch_add(kacc + slot, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
-  if constexpr (FOLD) {
-#pragma unroll
-    for (int ch = 0; ch < 4; ++ch)
-      r[ch] = ch < Two<N>::NCH ? wave_sum16_rows(u + 4 * Two<N>::NCH + 4 * ch, lane) : 0.0;
-    const double vb = quad_sum4(r, lane);
-    __hip_atomic_fetch_add(kaccB + slot, vb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
-  }
+  __hip_atomic_fetch_add(kacc + slot, quad_sum4(r, lane), __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_WAVEFRONT);
+  if constexpr (FOLD)
+    __hip_atomic_fetch_add(kaccB + slot, quad_sum4(rc, lane), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_WAVEFRONT);
 }
 
 // One two-sided pair tile: bras [k0, k0+64) x kets [l0, l1) x terms [p0, p1),
@@ -723,14 +709,6 @@ __device__ __forceinline__ void pair2_tile(const double *__restrict__ bra,
     const double mb = valid ? 1.0 : 0.0;
     const double mk = (valid && k != l) ? 1.0 : 0.0;
     double hel = 0.0, sel = 0.0;
-    // the previous term's ket side, halved to 4 values per chunk, is summed
-    // while the next term runs (one basic block, so the two interleave);
-    // none before the first term
-    constexpr int NU = 4 * Two<N>::NCH;
-    double up[NU];
-#pragma unroll
-    for (int t = 0; t < NU; ++t) up[t] = 0.0;
-    unsigned pwp = 0xffffffffu;
     for (int p = p0; p < p1; ++p) {
       // the record after (l, p): (l, p + 1), else the wave's next ket at p0
       const int pn = p + 1 < p1 ? p + 1 : p0;
@@ -738,15 +716,12 @@ __device__ __forceinline__ void pair2_tile(const double *__restrict__ bra,
       double nx = 0.0;
       if (ln < l1 && lane < KF) nx = ket[((size_t)ln * nst + pn) * D::KS + lane];
       const unsigned pw = tperm[p * 64 + lane];
-      ket_sums_b<N>(up, pwp, kacc, lane);
       double kv[Two<N>::KV];
       ok &= pair_term2<N, R1>(b, kst, sc, mb, mk, hel, sel, sink, kv);
-      ket_sums_a<N>(kv, up);
-      pwp = pw;
+      ket_sums<N>(kv, pw, kacc, lane);
       // after every read of the term (the wave's LDS operations complete in order)
       if (lane < KF) kst[lane] = nx;
     }
-    ket_sums_b<N>(up, pwp, kacc, lane);
     if (valid) {
       const size_t o = upper ? (size_t)k * K + l : (size_t)l * K + k;
       hp[o] = hel;
@@ -859,11 +834,6 @@ __device__ __forceinline__ void pair2_fold_tile(const double *__restrict__ bra,
     const double mA = first ? 1.0 : 0.0;
     const double *kd = first ? kstA : kstB;
     double hel = 0.0, sel = 0.0;
-    constexpr int NU = 8 * Two<N>::NCH;   // the previous term's ket side, as in pair2_tile
-    double up[NU];
-#pragma unroll
-    for (int t = 0; t < NU; ++t) up[t] = 0.0;
-    unsigned pwp = 0xffffffffu;
     for (int p = p0; p < p1; ++p) {
       const int pn = p + 1 < p1 ? p + 1 : p0;
       const int sn = p + 1 < p1 ? s : s + W;
@@ -873,17 +843,14 @@ __device__ __forceinline__ void pair2_fold_tile(const double *__restrict__ bra,
         nxB = ket[((size_t)(k0 + sn + 32) * nst + pn) * D::KS + lane];
       }
       const unsigned pw = tperm[p * 64 + lane];
-      ket_sums_b<N, true>(up, pwp, kaccA, lane, kaccB);
       double kv[Two<N>::KV];
       ok &= pair_term2<N, R1, true>(b, kd, sc, 1.0, mk, hel, sel, sink, kv, sig);
-      ket_sums_a<N, true>(kv, up, mA);
-      pwp = pw;
+      ket_sums<N, true>(kv, pw, kaccA, lane, mA, kaccB);
       if (lane < KF) {   // after every read of the term (in-order LDS operations)
         kstA[lane] = nxA;
         kstB[lane] = nxB;
       }
     }
-    ket_sums_b<N, true>(up, pwp, kaccA, lane, kaccB);
     const size_t o = k < l ? (size_t)k * K + l : (size_t)l * K + k;
     hp[o] = hel;
     sp[o] = sel;
