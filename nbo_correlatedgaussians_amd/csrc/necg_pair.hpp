@@ -513,47 +513,56 @@ __device__ __forceinline__ bool pair_term(const Bra &b, const double *__restrict
       for (int m = 0; m < N; ++m) t = __builtin_fma(Bi[sx(i, m)], Ft[3 * m + x], t);
       Phi[3 * i + x] = t;
     }
-  // G = B^-1 (12 A'MA' + La) B^-1 (symmetric)
-  double Tm[N * N];
-#pragma unroll
-  for (int i = 0; i < N; ++i)
-#pragma unroll
-    for (int j = 0; j < N; ++j) {
-      double t = 0.0;
-#pragma unroll
-      for (int m = 0; m < N; ++m)
-        t = __builtin_fma(__builtin_fma(12.0, Np[sx(i, m)], La[sx(i, m)]), Bi[sx(m, j)], t);
-      Tm[i * N + j] = t;
-    }
-  double d1[3 * N];
-#pragma unroll
-  for (int t = 0; t < 3 * N; ++t) d1[t] = st[t] - b.S(t);
+  // G = B^-1 U B^-1 with U = 12 A'MA' + La (symmetric), row by row as
+  // (B^-1 U)_i B^-1.  With e = 2 wo dif1 (the Y_S values) the slots are
+  //   X_S = -3 wo B^-1 - sum_x e_i d_j,   X_H = am X_S + wo (G - dp),
+  //   Y_H = am e + wo Phi,                Y_S = e,
+  // dp_ij = <d_i, Phi_j> + <Phi_i, d_j>.
+  double d1[3 * N], e1[3 * N];
   const double wo = wc * ov;
-  const double am = (MODE == MODE_COMBINED) ? (alpha - eshift) : alpha;
-  const double woa = wo * am;
-#pragma unroll
-  for (int i = 0; i < N; ++i)
-#pragma unroll
-    for (int j = 0; j <= i; ++j) {
-      double g = 0.0;
-#pragma unroll
-      for (int m = 0; m < N; ++m) g = __builtin_fma(Bi[sx(i, m)], Tm[m * N + j], g);
-      double dd = 0.0, dp = 0.0;
-#pragma unroll
-      for (int x = 0; x < 3; ++x) {
-        dd = __builtin_fma(d1[3 * i + x], d1[3 * j + x], dd);
-        dp = __builtin_fma(d1[3 * i + x], Phi[3 * j + x], dp);
-        dp = __builtin_fma(Phi[3 * i + x], d1[3 * j + x], dp);
-      }
-      const double x0 = __builtin_fma(-3.0, Bi[sx(i, j)], -2.0 * dd);
-      const int t = sx(i, j);
-      sink.add(D::A_XH + t, __builtin_fma(woa, x0, wo * (g - dp)));
-      if (MODE == MODE_SPLIT) sink.add(D::A_XS + t, wo * x0);
-    }
+  const double wo2 = 2.0 * wo;
 #pragma unroll
   for (int t = 0; t < 3 * N; ++t) {
-    sink.add(D::A_YH + t, __builtin_fma(2.0 * woa, d1[t], wo * Phi[t]));
-    if (MODE == MODE_SPLIT) sink.add(D::A_YS + t, 2.0 * wo * d1[t]);
+    d1[t] = st[t] - b.S(t);
+    e1[t] = wo2 * d1[t];
+  }
+  const double am = (MODE == MODE_COMBINED) ? (alpha - eshift) : alpha;
+  const double woa = wo * am;
+  double U[Dim<N>::NS];
+#pragma unroll
+  for (int t = 0; t < Dim<N>::NS; ++t) U[t] = __builtin_fma(12.0, Np[t], La[t]);
+  const double m3wo = -3.0 * wo;
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    double V[N];   // row i of B^-1 U
+#pragma unroll
+    for (int c = 0; c < N; ++c) {
+      double v = Bi[sx(i, 0)] * U[sx(0, c)];
+#pragma unroll
+      for (int m = 1; m < N; ++m) v = __builtin_fma(Bi[sx(i, m)], U[sx(m, c)], v);
+      V[c] = v;
+    }
+#pragma unroll
+    for (int j = 0; j <= i; ++j) {
+      double g = V[0] * Bi[sx(0, j)];
+#pragma unroll
+      for (int m = 1; m < N; ++m) g = __builtin_fma(V[m], Bi[sx(m, j)], g);
+      double xs = m3wo * Bi[sx(i, j)];
+#pragma unroll
+      for (int x = 0; x < 3; ++x) {
+        g = __builtin_fma(-d1[3 * i + x], Phi[3 * j + x], g);
+        g = __builtin_fma(-Phi[3 * i + x], d1[3 * j + x], g);
+        xs = __builtin_fma(-e1[3 * i + x], d1[3 * j + x], xs);
+      }
+      const int t = sx(i, j);
+      sink.add(D::A_XH + t, __builtin_fma(am, xs, wo * g));
+      if (MODE == MODE_SPLIT) sink.add(D::A_XS + t, xs);
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < 3 * N; ++t) {
+    sink.add(D::A_YH + t, __builtin_fma(am, e1[t], wo * Phi[t]));
+    if (MODE == MODE_SPLIT) sink.add(D::A_YS + t, e1[t]);
   }
   sink.add(D::A_SH, woa);
   if (MODE == MODE_SPLIT) sink.add(D::A_SS, wo);
