@@ -341,9 +341,10 @@ __device__ __forceinline__ bool pair_term(const Bra &b, const double *__restrict
       for (int m = i; m < N; ++m) t = __builtin_fma(Ri[sx(m, i)], Ri[sx(m, j)], t);
       Bi[sx(i, j)] = t;
     }
-  double det32 = 1.0;  // det(B)^-3/2
+  double rp = rd[0];  // det(B)^-1/2 = prod 1 / R_jj; det(B)^-3/2 its cube
 #pragma unroll
-  for (int j = 0; j < N; ++j) det32 *= rd[j] * rd[j] * rd[j];
+  for (int j = 1; j < N; ++j) rp *= rd[j];
+  const double det32 = rp * rp * rp;
 
   // E, st = B^-1 E, i3 (python/matel.py:125-133)
   double Em[3 * N], st[3 * N];
@@ -363,9 +364,9 @@ __device__ __forceinline__ bool pair_term(const Bra &b, const double *__restrict
   const double ov = b.pk() * pl * det32 * exp(i3 - b.i1() - i2);
 
   // Kinetic (python/matel.py:144-155).  a1 = A dif1; the gradient needs
-  // only u = 8 A'M a1 from it, formed here so that dif1, a1 and A'M are dead
-  // before the Coulomb loop.
-  double u8[3 * N];
+  // only u = A'M a1 from it (8 u enters F), formed here so that dif1, a1 and
+  // A'M are dead before the Coulomb loop.
+  double u1[3 * N];
   double i3k = 0.0;  // -<a1, M a1>
   {
     double d1[3 * N], a1[3 * N];
@@ -401,16 +402,19 @@ __device__ __forceinline__ bool pair_term(const Bra &b, const double *__restrict
           if (mode_has_grad(MODE)) v = __builtin_fma(ApM[i * N + j], a1[3 * j + x], v);
         }
         i3k = __builtin_fma(-a1[3 * i + x], t, i3k);
-        u8[3 * i + x] = 8.0 * v;
+        u1[3 * i + x] = v;
       }
   }
   // 3 tr(M A B^-1 A') = 3 (tr(M A') - <A'MA', B^-1>), since A B^-1 = I - A' B^-1
   double ab = 0.0;
+  double abo = 0.0;  // the strictly lower part, counted twice below
 #pragma unroll
-  for (int i = 0; i < N; ++i)
+  for (int i = 0; i < N; ++i) {
+    ab = __builtin_fma(Np[sx(i, i)], Bi[sx(i, i)], ab);
 #pragma unroll
-    for (int j = 0; j <= i; ++j)
-      ab = __builtin_fma((i == j ? 1.0 : 2.0) * Np[sx(i, j)], Bi[sx(i, j)], ab);
+    for (int j = 0; j < i; ++j) abo = __builtin_fma(Np[sx(i, j)], Bi[sx(i, j)], abo);
+  }
+  ab = __builtin_fma(2.0, abo, ab);
   const double tfac = 2.0 * i3k + 3.0 * (kd[D::K_TRMP] - ab);
 
   // Coulomb (python/matel.py:162-197; gradients python/gradel.py:266-337).
@@ -420,9 +424,12 @@ __device__ __forceinline__ bool pair_term(const Bra &b, const double *__restrict
   // contain a (the first, (0, a), sets it), the off-diagonal entry (a, b) is
   // minus the pair's value.
   double csum2 = 0.0;  // twice sum_c c f r0
-  double La[NS], Lb[NS];
+  // La and Gl = Lb st (Lb: the pair's cb on both diagonal entries, -cb off the
+  // diagonal; a particle pair (a, b) adds cb d and -cb d, d = st_a - st_b, to
+  // rows a and b of Lb st, and a pair (0, b) sets row b to cb st_b)
+  double La[NS], Gl[3 * N];
   (void)La;
-  (void)Lb;
+  (void)Gl;
   {
     int pi = 0;
 #pragma unroll
@@ -455,14 +462,19 @@ __device__ __forceinline__ bool pair_term(const Bra &b, const double *__restrict
           const double ca = __builtin_fma(cb, z, f * h);
           if (p == 0) {
             La[sx(bq, bq)] = ca;
-            Lb[sx(bq, bq)] = cb;
+            Gl[3 * bq] = cb * dx;
+            Gl[3 * bq + 1] = cb * dy;
+            Gl[3 * bq + 2] = cb * dz;
           } else {
             La[sx(a, a)] += ca;
             La[sx(bq, bq)] += ca;
             La[sx(bq, a)] = -ca;
-            Lb[sx(a, a)] += cb;
-            Lb[sx(bq, bq)] += cb;
-            Lb[sx(bq, a)] = -cb;
+            Gl[3 * a] = __builtin_fma(cb, dx, Gl[3 * a]);
+            Gl[3 * a + 1] = __builtin_fma(cb, dy, Gl[3 * a + 1]);
+            Gl[3 * a + 2] = __builtin_fma(cb, dz, Gl[3 * a + 2]);
+            Gl[3 * bq] = __builtin_fma(-cb, dx, Gl[3 * bq]);
+            Gl[3 * bq + 1] = __builtin_fma(-cb, dy, Gl[3 * bq + 1]);
+            Gl[3 * bq + 2] = __builtin_fma(-cb, dz, Gl[3 * bq + 2]);
           }
         } else {
           csum2 = __builtin_fma(cr, boys_f0_value(z), csum2);
@@ -497,9 +509,7 @@ __device__ __forceinline__ bool pair_term(const Bra &b, const double *__restrict
   for (int i = 0; i < N; ++i)
 #pragma unroll
     for (int x = 0; x < 3; ++x) {
-      double t = u8[3 * i + x];
-#pragma unroll
-      for (int m = 0; m < N; ++m) t = __builtin_fma(Lb[sx(i, m)], st[3 * m + x], t);
+      double t = __builtin_fma(8.0, u1[3 * i + x], Gl[3 * i + x]);
       if (x == 2) t = __builtin_fma(-sc_.efs, sc_.q[i + 1], t);
       Ft[3 * i + x] = t;
     }

@@ -311,9 +311,10 @@ __device__ __forceinline__ bool pair_term2(const BraLds<N> &b, const double *__r
       for (int m = i; m < N; ++m) t = __builtin_fma(Ri[sx(m, i)], Ri[sx(m, j)], t);
       Bi[sx(i, j)] = t;
     }
-  double det32 = 1.0;
+  double rp = rd[0];  // det(B)^-1/2 = prod 1 / R_jj; det(B)^-3/2 its cube
 #pragma unroll
-  for (int j = 0; j < N; ++j) det32 *= rd[j] * rd[j] * rd[j];
+  for (int j = 1; j < N; ++j) rp *= rd[j];
+  const double det32 = rp * rp * rp;
 
   // E, st = B^-1 E, i3 (python/matel.py:125-133)
   double st[N3];
@@ -382,11 +383,14 @@ __device__ __forceinline__ bool pair_term2(const BraLds<N> &b, const double *__r
     }
   }
   double ab = 0.0;  // <AMA, B^-1> over the full symmetric matrices
+  double abo = 0.0;  // the strictly lower part, counted twice below
 #pragma unroll
-  for (int i = 0; i < N; ++i)
+  for (int i = 0; i < N; ++i) {
+    ab = __builtin_fma(b.AMA(sx(i, i)), Bi[sx(i, i)], ab);
 #pragma unroll
-    for (int j = 0; j <= i; ++j)
-      ab = __builtin_fma((i == j ? 1.0 : 2.0) * b.AMA(sx(i, j)), Bi[sx(i, j)], ab);
+    for (int j = 0; j < i; ++j) abo = __builtin_fma(b.AMA(sx(i, j)), Bi[sx(i, j)], abo);
+  }
+  ab = __builtin_fma(2.0, abo, ab);
   const double tfac = 2.0 * i3k + 3.0 * (b.f(BraLds<N>::F_TRMA) - ab);
 
   // Coulomb (python/matel.py:162-197; gradients python/gradel.py:266-337).
@@ -396,7 +400,10 @@ __device__ __forceinline__ bool pair_term2(const BraLds<N> &b, const double *__r
   // particle a sums the pairs that contain a (the first of them, (0, a),
   // sets it); the off-diagonal entry (a, b) is minus the pair's value.
   double csum2 = 0.0;  // twice the Coulomb sum
-  double La[NS], Lb[NS];
+  // La and Gl = Lb st: Lb has the pair's cb on both diagonal entries and -cb
+  // off the diagonal, so a particle pair (a, b) adds cb d and -cb d (d = st_a
+  // - st_b) to rows a and b of Lb st, and a pair (0, b) sets row b to cb st_b
+  double La[NS], Gl[N3];
   {
     int pi = 0;
 #pragma unroll
@@ -428,14 +435,19 @@ __device__ __forceinline__ bool pair_term2(const BraLds<N> &b, const double *__r
         const double ca = __builtin_fma(cb, z, f * h);
         if (p == 0) {
           La[sx(bq, bq)] = ca;
-          Lb[sx(bq, bq)] = cb;
+          Gl[3 * bq] = cb * dx;
+          Gl[3 * bq + 1] = cb * dy;
+          Gl[3 * bq + 2] = cb * dz;
         } else {
           La[sx(a, a)] += ca;
           La[sx(bq, bq)] += ca;
           La[sx(bq, a)] = -ca;
-          Lb[sx(a, a)] += cb;
-          Lb[sx(bq, bq)] += cb;
-          Lb[sx(bq, a)] = -cb;
+          Gl[3 * a] = __builtin_fma(cb, dx, Gl[3 * a]);
+          Gl[3 * a + 1] = __builtin_fma(cb, dy, Gl[3 * a + 1]);
+          Gl[3 * a + 2] = __builtin_fma(cb, dz, Gl[3 * a + 2]);
+          Gl[3 * bq] = __builtin_fma(-cb, dx, Gl[3 * bq]);
+          Gl[3 * bq + 1] = __builtin_fma(-cb, dy, Gl[3 * bq + 1]);
+          Gl[3 * bq + 2] = __builtin_fma(-cb, dz, Gl[3 * bq + 2]);
         }
       }
   }
@@ -456,12 +468,9 @@ __device__ __forceinline__ bool pair_term2(const BraLds<N> &b, const double *__r
     for (int i = 0; i < N; ++i)
 #pragma unroll
       for (int x = 0; x < 3; ++x) {
-        double t = 0.0, u = 0.0;
+        double t = Gl[3 * i + x], u = 0.0;
 #pragma unroll
-        for (int m = 0; m < N; ++m) {
-          t = __builtin_fma(Lb[sx(i, m)], st[3 * m + x], t);
-          u = __builtin_fma(b.A(sx(i, m)), Ma1[3 * m + x], u);
-        }
+        for (int m = 0; m < N; ++m) u = __builtin_fma(b.A(sx(i, m)), Ma1[3 * m + x], u);
         if (x == 2) t = __builtin_fma(-sc_.efs, sc_.q[i + 1], t);
         G[3 * i + x] = __builtin_fma(-8.0, u, t);
       }

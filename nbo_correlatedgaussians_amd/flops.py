@@ -13,10 +13,10 @@ Two models:
 * ``flops_per_eval``: the two-sided algorithm the pair kernel runs: per
   unordered (pair, term) one factorisation, the elements, the Coulomb and
   Stark derivatives, and both gradient halves (``two_sided_term``; LiH,
-  whose mass matrix is diagonal plus a constant: 4,083 FLOP per pair-term,
-  97,992 per off-diagonal pair).  The identities
+  whose mass matrix is diagonal plus a constant: 4,031 FLOP per pair-term,
+  96,744 per off-diagonal pair).  The identities
   it uses (A' dif2 = -A dif1, B^-1 A' = I - B^-1 A, the collapsed kinetic and
-  Coulomb gradient products) make it 4.2x smaller than the v0 model for LiH,
+  Coulomb gradient products) make it 4.3x smaller than the v0 model for LiH,
   so the v0 model overstates the work of the run and is not the roofline
   basis.
 """
@@ -56,17 +56,20 @@ def two_sided_term(n: int, rank1: bool = False) -> dict:
         chol += 2 * j + 2 + 1 + (n - 1 - j) * (2 * j + 1)
     inv = sum(2 * (i - j) + 1 for j in range(n) for i in range(j + 1, n))   # R^-1
     inv += sum((i + 1) * 2 * (n - i) for i in range(n))                     # B^-1 = R^-T R^-1
-    det = 3 * n                                          # det(B)^-3/2 from the pivots
+    det = (n - 1) + 2                                    # det(B)^-3/2: the pivots' product, cubed
     elem = n3 + 2 * n3 * n + 2 * n3 + 6                  # E, st = B^-1 E, i3, ov
     ma1 = 3 * (n - 1) + 3 + 2 * n3 if rank1 else 2 * n3 * n
     kin = n3 + 2 * n3 * n + ma1 + 2 * n3                 # dif1, a1 = A dif1, M a1, i3k
-    kin += 2 * ns + (ns - n) + 4                         # <AMA, B^-1>, tfac
-    ref_pair = 2 + 1 + 6 + 51 + 8                        # (0, q): rsq, 1/i2, z, Boys F0/2F0', ca cb
-    pp_pair = ref_pair + 3 + 3 + 4                       # (p, q): i2, dif, 4 diagonal updates
+    kin += 2 * ns + 2 + 4                                # <AMA, B^-1> (off-diagonal part doubled), tfac
+    base = 2 + 1 + 6 + 51 + 8                            # rsq, 1/i2, z, Boys F0/2F0', ca cb
+    ref_pair = base + 3                                  # (0, q): row q of Lb st set to cb st_q
+    pp_pair = base + 3 + 3 + 2 + 12                      # (p, q): i2, dif, 2 La diagonal updates,
+    #                                                      cb dif added to rows p, q of Lb st
     coul = n * ref_pair + (ns - n) * pp_pair
     stark = 2 * n + 4 + 3 + 2                            # zsum, alpha, H and S sums
-    shared = 2 * n3 * n + 2 * n + 2 * n3 * n + 2 * n3 + 2 * n3 * n   # Lb st, Stark column, A M a1,
-    #                                                      F - 8 A M a1, B^-1 F (the ket side's Phi)
+    shared = 2 * n + 2 * n3 * n + 2 * n3 + 2 * n3 * n   # Stark column, A M a1, F - 8 A M a1,
+    #                                                     B^-1 F (the ket side's Phi; Lb st is formed
+    #                                                     in the Coulomb loop)
     side = (n3 + 5 + n3                                  # dif, the weights, e = 2 wo dif
             + 3 * n3                                     # Y_H = alpha e + wo Phi (Y_S = e)
             + 2 * ns + n * n * (2 * n - 1)               # U = 12 A'MA' + La, V = B^-1 U

@@ -137,5 +137,5 @@ def test_mass_rank1_detection_and_op_count():
     # only the kinetic phase differs: M a1 as 75 FMAs against 12 adds, 3 products, 15 FMAs
     assert {k for k in full if full[k] != r1[k]} == {"kinetic"}
     assert full["kinetic"] - r1["kinetic"] == 150 - 45
-    assert two_sided_term_flops(5, rank1=True) == 4083
-    assert two_sided_term_flops(5, True, True) == 4083 - r1["ket"]
+    assert two_sided_term_flops(5, rank1=True) == 4031
+    assert two_sided_term_flops(5, True, True) == 4031 - r1["ket"]
