@@ -189,10 +189,40 @@ inline int tri_split_count(long long nt, long long nfull, int kt, int ncu) {
   return (int)r;
 }
 
+// Dispatch order for L2 locality.  The hardware hands the consecutive
+// workgroups of a launch to the device's XCDs in turn (8 on MI355X, each with
+// its own L2), and a tile (bx, by) reads the ket records of ket tile by.
+// Tiles [t0, t1) of a list (word pairs) are reordered so that the tile at
+// list position p has ket tile by = p mod 8 while that class lasts (else the
+// next class that has tiles left), each class in its previous order: each
+// XCD then streams the ket records of one eighth of the ket tiles (LiH
+// K = 2048: 3.1 MB of the 64 fields the two-sided term reads, within a 4 MB
+// L2) rather than a quarter or more of them.
+constexpr int kXcds = 8;
+
+inline void tri_xcd_order(std::vector<int> &list, size_t t0, size_t t1) {
+  if (t1 <= t0 + 1) return;
+  std::vector<std::vector<int>> q(kXcds);
+  for (size_t i = t0; i < t1; ++i) {
+    std::vector<int> &c = q[(size_t)(list[2 * i + 1] % kXcds)];
+    c.push_back(list[2 * i]);
+    c.push_back(list[2 * i + 1]);
+  }
+  std::vector<size_t> head(kXcds, 0);
+  for (size_t i = t0; i < t1; ++i) {
+    int c = (int)(i % kXcds);
+    for (int s = 0; s < kXcds && head[(size_t)c] >= q[(size_t)c].size(); ++s) c = (c + 1) % kXcds;
+    list[2 * i] = q[(size_t)c][head[(size_t)c]];
+    list[2 * i + 1] = q[(size_t)c][head[(size_t)c] + 1];
+    head[(size_t)c] += 2;
+  }
+}
+
 // The tile list of a lower-triangle pass as the kernels take it (TriTile
 // words): the tri_tiles list with its first r whole regular tiles split into
 // halves (tri_split_count) and moved to the end, first and second half of
-// each tile side by side.  The first tiles belong to the bra tiles with the
+// each tile side by side, and the other whole regular tiles in XCD order
+// (tri_xcd_order).  The first tiles belong to the bra tiles with the
 // fewest ket tiles, whose reduction (tile_reduce2) has the least to sum:
 // their extra slots leave the longest reduction, the last bra tile's, as it
 // was.  xr[2 bx], xr[2 bx + 1] = the extra-slot range of bra tile bx's split
@@ -209,10 +239,14 @@ inline int tri_tile_list(int K, int kt, int ncu, int fold, std::vector<int> &lis
   });
   const int nbt = (K + 63) / 64;
   xr.assign((size_t)2 * nbt, 0);
+  // the fold and diagonal tiles lead the list; the whole regular tiles follow
+  size_t nf = 0;
+  while (nf < (size_t)nt && ((unsigned)t[2 * nf] >> 30) == 3u) ++nf;
   // the diagonal tiles (short, dispatched first) take no slot of a round
   const int r = tri_split_count(nt - tri_fold_tiles(K, kt, fold), nfull, kt, ncu);
   if (r == 0) {
     list.swap(t);
+    tri_xcd_order(list, nf, nf + (size_t)nfull);
     return 0;
   }
   list.clear();
@@ -228,6 +262,8 @@ inline int tri_tile_list(int K, int kt, int ncu, int fold, std::vector<int> &lis
       list.push_back(t[i + 1]);
     }
   }
+  // the whole regular tiles left after the r split ones
+  tri_xcd_order(list, nf, nf + (size_t)(nfull - r));
   for (int e = 0; e < r; ++e) {
     const int bx = split[(size_t)2 * e], by = split[(size_t)2 * e + 1];
     if (xr[(size_t)2 * bx + 1] == 0) xr[(size_t)2 * bx] = e;
