@@ -7,7 +7,8 @@
 // wrote them, and no tile is empty.  The same for the lists the kernels take
 // (tri_tile_list, decoded by tri_tile), whose last round may be split into
 // half tiles: each function's bra partials then also come from the extra
-// slots xr names.  Folded and unfolded tilings.  Prints "ok" and exits 0, or
+// slots xr names; and the lists' whole regular tiles stand in XCD order
+// (tri_xcd_order).  Folded and unfolded tilings.  Prints "ok" and exits 0, or
 // names the first failure.
 #include <cstdio>
 #include <set>
@@ -119,6 +120,41 @@ static int check(int K, int kt, int fold) {
   return bad;
 }
 
+// The XCD order of a tile list's whole regular tiles (tri_xcd_order): the
+// section after the leading fold and diagonal tiles holds the whole regular
+// tiles that were not split, in tri_tiles order within each class by mod 8,
+// and list position i takes class i mod 8 while that class has tiles left
+// (else the next class that has).  Returns the number of positions that got
+// their own class i mod 8, or -1 on a mismatch.
+static long long check_xcd(const std::vector<int> &list, int K, int kt, int fold, int r,
+                           long long nfull) {
+  std::vector<int> t;   // the tiles in tri_tiles order, as tri_tile_list sees them
+  tri_tiles(K, kt, fold, [&](int bx, int by, int f) {
+    t.push_back((int)((unsigned)bx | (f ? 3u << 30 : 0u)));
+    t.push_back(by);
+  });
+  size_t nf = 0;
+  while (2 * nf < t.size() && ((unsigned)t[2 * nf] >> 30) == 3u) ++nf;
+  const size_t n0 = nf + (size_t)r, n1 = nf + (size_t)nfull;   // the section's tiles in t
+  std::vector<std::vector<int>> q(kXcds);
+  for (size_t i = n0; i < n1; ++i) {
+    q[(size_t)(t[2 * i + 1] % kXcds)].push_back(t[2 * i]);
+    q[(size_t)(t[2 * i + 1] % kXcds)].push_back(t[2 * i + 1]);
+  }
+  std::vector<size_t> head(kXcds, 0);
+  long long own = 0;
+  for (size_t i = nf; i < nf + (n1 - n0); ++i) {
+    int c = (int)(i % kXcds);
+    while (head[(size_t)c] >= q[(size_t)c].size()) c = (c + 1) % kXcds;
+    if (list[2 * i] != q[(size_t)c][head[(size_t)c]] ||
+        list[2 * i + 1] != q[(size_t)c][head[(size_t)c] + 1])
+      return -1;
+    head[(size_t)c] += 2;
+    own += c == (int)(i % kXcds);
+  }
+  return own;
+}
+
 // The kernels' list for ncu CUs; counts the lists that have split tiles.
 static int check_list(int K, int kt, int ncu, int fold, int &nsplit_lists) {
   std::vector<int> list, xr;
@@ -128,6 +164,11 @@ static int check_list(int K, int kt, int ncu, int fold, int &nsplit_lists) {
   if (nt != tri_count(K, kt, fold) || (long long)list.size() != 2 * (nt + r) ||
       r != tri_split_count(nt - tri_fold_tiles(K, kt, fold), nfull, kt, ncu)) {
     std::printf("K=%d kt=%d ncu=%d fold=%d: list size\n", K, kt, ncu, fold);
+    return 1;
+  }
+  if (check_xcd(list, K, kt, fold, r, nfull) < 0) {
+    std::printf("K=%d kt=%d ncu=%d fold=%d: whole regular tiles not in XCD order\n", K, kt,
+                ncu, fold);
     return 1;
   }
   if (r) ++nsplit_lists;
@@ -259,6 +300,17 @@ int main(int argc, char **argv) {
     }
     if (tri_count(2048, 8, 1) != 4128 || tri_tile_list(2048, 8, 256, 1, l, x) != 0) {
       std::printf("K=2048 kt=8 folded: expected 4,096 + 32 diagonal tiles, none split\n");
+      ++bad;
+    }
+    // LiH K = 2048 at ket tile 16, folded: 1,984 whole regular tiles, and all
+    // but the last few positions take their own XCD's class
+    long long nt, nfull;
+    tri_counts(2048, 16, 1, nt, nfull);
+    const int r16 = tri_tile_list(2048, 16, 256, 1, l, x);
+    const long long own = check_xcd(l, 2048, 16, 1, r16, nfull);
+    if (nfull != 1984 || own < 1984 * 95 / 100) {
+      std::printf("K=2048 kt=16 folded: %lld whole tiles, %lld in their own XCD class\n", nfull,
+                  own);
       ++bad;
     }
   }
